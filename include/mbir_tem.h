@@ -1,0 +1,189 @@
+/*
+ * mbir_tem.h -- C-ABI of the MI355X-native MBIR reconstruction engine.
+ *
+ * The boundary replaces the TBB CPU path behind OpenMBIR-TEM's reconstruction
+ * engine interface.  Reference citations are relative to the reference's
+ * Code/MBIRLib/ directory:
+ *
+ *   class HAADF_ReconstructionEngine : public AbstractFilter
+ *     HAADF/HAADF_ReconstructionEngine.h:67-99   (properties TomoInputs, Sinogram,
+ *     Geometry, AdvParams, ForwardModel; void execute(); getErrorCondition())
+ *   inner seam updateVoxels(...) -> uint8_t (1 = continue, 0 = converged)
+ *     HAADF/HAADF_ReconstructionEngine.h:120-129,
+ *     HAADF/HAADF_ReconstructionEngine_UpdateVoxels.cpp:594-817
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 (or a documented non-negative value) on
+ *     success, negative on error.  Codes mirror the reference: -1 bad input,
+ *     -2 detector response failure, -999 cancelled
+ *     (HAADF/HAADF_ReconstructionEngine.cpp:369-393,401,522-528).  Codes -100..-199
+ *     are HIP runtime failures of this library.
+ *   - The caller owns every host buffer.  The library owns device state from
+ *     mbir_job_create until mbir_job_destroy.
+ *   - A context may be driven from one host thread; mbir_job_cancel may be
+ *     called from any thread.
+ *   - Layouts are the reference's: volume [N_z][N_x][N_y] (y fastest),
+ *     sinograms [N_theta][N_r][N_t] (t fastest)
+ *     (GenericFilters/InitialReconstructionInitializer.cpp:158-160,
+ *     Common/TomoArray.hpp:176-198).
+ */
+#ifndef MBIR_TEM_H_
+#define MBIR_TEM_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBIR_OK 0
+#define MBIR_ERR_INPUT (-1)
+#define MBIR_ERR_RESPONSE (-2)
+#define MBIR_ERR_CANCELLED (-999)
+#define MBIR_ERR_HIP (-100)
+#define MBIR_ERR_NOMEM (-101)
+#define MBIR_ERR_UNSUPPORTED (-3)
+#define MBIR_ERR_BUDGET (-4)
+
+typedef struct mbir_ctx mbir_ctx;
+typedef struct mbir_job mbir_job;
+
+/* Context options (no reference counterpart: device selection and the HBM
+ * budget used for admission by the batch/serve paths). */
+typedef struct mbir_opts {
+  int32_t device;            /* HIP device ordinal */
+  int32_t verbose;           /* 0 quiet, 1 progress lines on stderr */
+  uint64_t hbm_budget_bytes; /* 0 = 90% of free device memory */
+} mbir_opts;
+
+/* Geometry produced by mbir_geometry_init; the fields of Sinogram and Geometry
+ * (Reconstruction/ReconstructionStructures.h:68-97) the engine needs. */
+typedef struct mbir_geometry {
+  int32_t N_theta, N_r, N_t;
+  int32_t N_x, N_y, N_z;
+  double delta_r, delta_t, delta_xz, delta_xy;
+  double R0, RMax, T0, TMax;
+  double x0, y0, z0;
+  double LengthX, LengthY, LengthZ;
+} mbir_geometry;
+
+/* Everything one reconstruction job needs: Sinogram + Geometry + TomoInputs +
+ * AdvancedParameters + the HAADF_ForwardModel's initial nuisance arrays
+ * (HAADF/HAADF_ReconstructionEngine.h:75-84, Reconstruction/ReconstructionStructures.h). */
+typedef struct mbir_problem {
+  mbir_geometry geom;
+  const double* angles_deg; /* [N_theta] tilts in degrees (Sinogram::angles before execute) */
+  const float* counts;      /* [N_theta][N_r][N_t] measured sinogram (Sinogram::counts) */
+  const float* init_volume; /* [N_z][N_x][N_y] or NULL -> default_recon */
+  double default_recon;     /* TomoInputs::defaultInitialRecon */
+  const double* init_gain;     /* [N_theta] or NULL -> target_gain */
+  const double* init_offset;   /* [N_theta] or NULL -> default_offset */
+  const double* init_variance; /* [N_theta] or NULL -> default_variance */
+  double target_gain;          /* TomoInputs::targetGain */
+  double default_offset;       /* TomoInputs::defaultOffset */
+  double default_variance;     /* TomoInputs::defaultVariance */
+  double sigma_x;              /* TomoInputs::SigmaX */
+  double mrf_q;                /* TomoInputs::p = diffuseness + 1 (MRF_Q); MRF_P = 2 */
+  double mrf_c;                /* 1e-3 HAADF (HAADF/HAADF_QGGMRFPriorModel.cpp:15) */
+  double stop_threshold;       /* TomoInputs::StopThreshold */
+  int32_t num_outer;           /* TomoInputs::NumOuterIter */
+  int32_t num_inner;           /* TomoInputs::NumIter */
+  int32_t joint_estimation;    /* AdvancedParameters::JOINT_ESTIMATION */
+  int32_t noise_model;         /* AdvancedParameters::NOISE_MODEL */
+  int32_t zero_skipping;       /* AdvancedParameters::ZERO_SKIPPING */
+  uint32_t seed;               /* update-order seed (the reference seeds from the clock) */
+  /* ICD schedule: 0 = conflict-free groups (every step is equivalent to a
+   * sequential ICD order); 1 = relaxed groups of rows_per_step spaced rows. */
+  int32_t schedule;
+  int32_t rows_per_step;
+} mbir_problem;
+
+/* Per-sweep statistics (updateVoxels, HAADF_ReconstructionEngine_UpdateVoxels.cpp:717-786) */
+typedef struct mbir_sweep_stats {
+  double sum_abs_update;   /* ROI sum |x_new - x_old| (AverageUpdate) */
+  double sum_abs_value;    /* ROI sum |x_old| (AverageMagnitudeOfRecon) */
+  int64_t zero_skipped;    /* voxels skipped by zero-skipping */
+  int64_t updated;         /* voxels updated */
+  int64_t nnz_updated;     /* sum over updated voxels of A-column entries x |VLR| */
+  double kernel_ms;        /* device time of the sweep's ICD kernels (when timing is on) */
+  int32_t steps;           /* kernel steps in the sweep */
+  int32_t status;          /* 1 = continue, 0 = converged (stop threshold met) */
+} mbir_sweep_stats;
+
+typedef struct mbir_run_stats {
+  int32_t sweeps;
+  int32_t outer_iterations;
+  int32_t status; /* last sweep status */
+  double seconds;
+} mbir_run_stats;
+
+/* ---- library / context ---- */
+int32_t mbir_version(void);
+int32_t mbir_create(const mbir_opts* opts, mbir_ctx** out);
+int32_t mbir_destroy(mbir_ctx* ctx);
+
+/* Geometry sizing of InitialReconstructionInitializer::execute
+ * (GenericFilters/InitialReconstructionInitializer.cpp:98-167) and
+ * MRCSinogramInitializer R0/T0 (GenericFilters/MRCSinogramInitializer.cpp:302-305).
+ * delta_xz_in is TomoInputs::delta_xz as set by the multi-resolution driver
+ * (HAADF/HAADF_MultiResolutionReconstruction.cpp:349-350), interpolate_factor is
+ * TomoInputs::interpolateFactor (:262).  Host only. */
+int32_t mbir_geometry_init(int32_t N_r, int32_t N_t, int32_t N_theta, double delta_r,
+                           double delta_t, double thickness, double delta_xz_in,
+                           double interpolate_factor, int32_t extend_object,
+                           const double* angles_deg, mbir_geometry* out);
+
+/* ---- one reconstruction job ----
+ * mbir_job_create performs execute()'s setup (HAADF/HAADF_ReconstructionEngine.cpp:441-691):
+ * voxel profiles, detector response, H_t and voxel-line responses, ROI mask, gain and
+ * offset initialisation, the initial forward projection, error sinogram and weights. */
+int32_t mbir_job_create(mbir_ctx* ctx, const mbir_problem* prob, mbir_job** out);
+int32_t mbir_job_destroy(mbir_job* job);
+
+/* One ICD sweep: updateVoxels(outer, inner, ...) (…_UpdateVoxels.cpp:594-817).
+ * Returns 1 = continue, 0 = converged (stop test met), negative on error. */
+int32_t mbir_job_sweep(mbir_job* job, int32_t outer, int32_t inner, mbir_sweep_stats* stats);
+
+/* jointEstimation (HAADF/HAADF_ReconstructionEngine_Extra.cpp:419-687) when enabled,
+ * then updateWeights (:795-881) when the noise model is on.  FP64 on the device. */
+int32_t mbir_job_nuisance_update(mbir_job* job);
+
+/* computeCost (HAADF/HAADF_ReconstructionEngine.cpp:1147-1364) in FP64 on the device.
+ * terms3 (may be NULL) receives {data, prior, noise}. */
+int32_t mbir_job_cost(mbir_job* job, double* total, double* terms3);
+
+/* execute()'s outer/inner loop (HAADF/HAADF_ReconstructionEngine.cpp:700-815).
+ * costs (may be NULL) receives up to cap FP64 cost values, one after the setup,
+ * one after every sweep and one after every nuisance step (COST_CALCULATE order). */
+int32_t mbir_job_run(mbir_job* job, mbir_run_stats* stats, double* costs, int32_t cap,
+                     int32_t* n_costs);
+
+/* Ŷ = I0 * A x (HAADF/HAADF_ForwardProject.cpp:80-120) for volume (host,
+ * [N_z][N_x][N_y]; NULL = the job's current volume), written as FP64 to out. */
+int32_t mbir_job_forward_project(mbir_job* job, const float* volume, double* out);
+
+/* A-matrix column (z, x) as computed on the device
+ * (HAADF/HAADF_ReconstructionEngine.cpp:1369-1524).  Returns the entry count
+ * (entries beyond cap are not written) or a negative error. */
+int32_t mbir_job_get_column(mbir_job* job, int32_t z, int32_t x, uint32_t* index, float* values,
+                            int32_t cap);
+
+int32_t mbir_job_get_volume(mbir_job* job, float* out);
+int32_t mbir_job_set_volume(mbir_job* job, const float* volume); /* also re-derives e */
+int32_t mbir_job_get_error_sinogram(mbir_job* job, float* out);
+int32_t mbir_job_get_nuisance(mbir_job* job, double* I0, double* mu, double* alpha);
+int32_t mbir_job_get_geometry(mbir_job* job, mbir_geometry* out);
+/* detector response H[N_theta][64] (GenericFilters/DetectorResponse.cpp:66-127) */
+int32_t mbir_job_get_detector_response(mbir_job* job, double* out);
+/* Thread-safe cooperative cancel (AbstractFilter::setCancel); the next sweep
+ * or run step returns MBIR_ERR_CANCELLED. */
+int32_t mbir_job_cancel(mbir_job* job);
+/* device bytes held by the job (volume, sinograms, tables) */
+int64_t mbir_job_device_bytes(const mbir_job* job);
+/* 1 = time every ICD kernel step with HIP events on the job's stream */
+int32_t mbir_job_set_timing(mbir_job* job, int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,809 @@
+// mbir_capi.cpp -- host side of the MI355X MBIR engine: FP64 setup of the
+// detector model, device state of one job, the ICD sweep schedule, and the
+// extern "C" boundary declared in include/mbir_tem.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "../../include/mbir_tem.h"
+#include "mbir_internal.h"
+
+using namespace mbir;
+
+#define MBIR_VERSION 10000
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess) {                                                              \
+      std::fprintf(stderr, "mbir: HIP error %s at %s:%d: %s\n", hipGetErrorName(_e),    \
+                   __FILE__, __LINE__, #expr);                                           \
+      return MBIR_ERR_HIP;                                                               \
+    }                                                                                    \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Host FP64 detector model (once per job).
+
+// calculateVoxelProfile, HAADF/HAADF_ReconstructionEngine.cpp:957-1029; the angles are
+// multiplied by pi/180 in place as in the reference (:978).
+void voxel_profile(std::vector<double>& angles, double dxz, std::vector<double>& prof)
+{
+  const int n = (int)angles.size();
+  prof.assign((size_t)n * kProfileResolution, 0.0);
+  for (int i = 0; i < n; ++i) {
+    angles[i] = angles[i] * (kPi / 180.0);
+    double angle = angles[i];
+    while (angle > kPi / 2) angle -= kPi / 2;
+    while (angle < 0) angle += kPi / 2;
+    const double maxv = angle <= kPi / 4 ? dxz / std::cos(angle) : dxz / std::cos(kPi / 2 - angle);
+    const double c45 = std::cos(kPi / 4);
+    const double d1 = c45 * std::cos(kPi / 4 - angle);
+    const double d2 = c45 * std::fabs(std::cos(kPi / 4 + angle));
+    const double lc = 1 - d1, ln = 1 - d2, rn = 1 + d2, rc = 1 + d1;
+    for (int j = 0; j < kProfileResolution; ++j) {
+      const double t = 2.0 * j / kProfileResolution;
+      double v;
+      if (t <= lc || t >= rc) v = 0;
+      else if (t > rn) v = maxv * (rc - t) / (rc - rn);
+      else if (t >= ln) v = maxv;
+      else v = maxv * (t - lc) / (ln - lc);
+      prof[(size_t)i * kProfileResolution + j] = v;
+    }
+  }
+}
+
+// DetectorParameters::initializeBeamProfile (GenericFilters/DetectorParameters.cpp:78-100)
+// and DetectorResponse::execute (GenericFilters/DetectorResponse.cpp:66-127).
+void detector_response(const mbir_geometry& g, const std::vector<double>& prof, double offsetR,
+                       std::vector<double>& H)
+{
+  double beam[kBeamResolution], sum = 0;
+  for (int i = 0; i < kBeamResolution; ++i) {
+    beam[i] = 0.54 - 0.46 * std::cos((2.0 * kPi / kBeamResolution) * i);
+    sum += beam[i];
+  }
+  for (int i = 0; i < kBeamResolution; ++i) beam[i] = beam[i] / sum / g.delta_t;
+  const double r0 = -g.delta_r / 2, step = g.delta_r / kBeamResolution;
+  const double tc = kProfileResolution / (2 * g.delta_xz);
+  H.assign((size_t)g.N_theta * kResponseBins, 0.0);
+  for (int k = 0; k < g.N_theta; ++k)
+    for (int i = 0; i < kResponseBins; ++i) {
+      const double rmin = i * offsetR - g.delta_xz;
+      double s = 0;
+      for (int p = 0; p < kBeamResolution; ++p) {
+        const double r = r0 + p * step;
+        if (r < rmin) continue;
+        int pi = (int16_t)(int32_t)std::floor((r - rmin) * tc);
+        if (pi < 0) pi = 0;
+        if (pi >= kProfileResolution) pi = kProfileResolution - 1;
+        s += prof[(size_t)k * kProfileResolution + pi] * beam[p];
+      }
+      H[(size_t)k * kResponseBins + i] = s;
+    }
+}
+
+// H_t row of view 0 (HAADF/HAADF_ForwardModel.cpp:408-451) and the voxel-line
+// responses (HAADF/HAADF_ReconstructionEngine_Extra.cpp:329-404).
+int voxel_line_responses(const mbir_geometry& g, double offsetT, std::vector<int>& t0,
+                         std::vector<int>& n, std::vector<float>& val)
+{
+  double Ht[kResponseBins];
+  for (int i = 0; i < kResponseBins; ++i) {
+    const double pct = i * offsetT;
+    double v;
+    if (g.delta_xy >= g.delta_t)
+      v = pct <= (g.delta_xy / 2 - g.delta_t / 2) ? g.delta_t : -pct + g.delta_xy / 2 + g.delta_t / 2;
+    else
+      v = pct <= g.delta_t / 2 - g.delta_xy / 2 ? g.delta_xy : -pct + g.delta_xy / 2 + g.delta_t / 2;
+    Ht[i] = v < 0 ? 0 : v;
+  }
+  t0.assign(g.N_y, 0);
+  n.assign(g.N_y, 0);
+  val.assign((size_t)g.N_y * kVlrCap, 0.f);
+  for (int y = 0; y < g.N_y; ++y) {
+    const double t = ((double)y + 0.5) * g.delta_xy + g.y0;
+    const double tmin = (t - g.delta_xy / 2) > g.T0 ? t - g.delta_xy / 2 : g.T0;
+    const double tmax = (t + g.delta_xy / 2) <= g.TMax ? t + g.delta_xy / 2 : g.TMax;
+    int smin = (int16_t)(uint16_t)std::floor((tmin - g.T0) / g.delta_t);
+    int smax = (int16_t)(uint16_t)std::floor((tmax - g.T0) / g.delta_t);
+    if (smin < 0) smin = 0;
+    if (smax >= g.N_t) smax = g.N_t - 1;
+    int cnt = 0, first = -1, last = -1;
+    for (int it = smin; it <= smax; ++it) {
+      const double ct = ((double)it + 0.5) * g.delta_t + g.T0;
+      const double dt = std::fabs(ct - t);
+      const int idt = (int16_t)(uint16_t)std::floor(dt / offsetT);
+      double pt = 0;
+      if (idt < kResponseBins) {
+        const double w3 = dt - idt * offsetT, w4 = (idt + 1) * offsetT - dt;
+        const int nx = idt + 1 < kResponseBins ? idt + 1 : kResponseBins - 1;
+        pt = (w4 / offsetT) * Ht[idt] + (w3 / offsetT) * Ht[nx];
+      }
+      if (pt != 0) {
+        // the reference walks index[0] .. index[0]+count-1 (…_UpdateVoxels.cpp:349-350),
+        // i.e. it treats the rows as contiguous
+        if (cnt >= kVlrCap) return MBIR_ERR_UNSUPPORTED;
+        if (first < 0) first = it;
+        val[(size_t)y * kVlrCap + cnt] = (float)pt;
+        last = it;
+        cnt++;
+      }
+    }
+    t0[y] = first < 0 ? 0 : first;
+    n[y] = cnt;
+    (void)last;
+  }
+  // slices must map to disjoint detector rows (holds whenever y0 = T0, the
+  // reference's geometry, GenericFilters/InitialReconstructionInitializer.cpp:167)
+  for (int y = 0; y + 1 < g.N_y; ++y)
+    if (n[y] && n[y + 1] && t0[y] + n[y] > t0[y + 1]) return MBIR_ERR_UNSUPPORTED;
+  return 0;
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count)
+  {
+    n = count;
+    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+};
+
+}  // namespace
+
+struct mbir_ctx {
+  int device = 0;
+  int verbose = 0;
+  uint64_t budget = 0;
+  std::mutex mu;
+};
+
+struct mbir_job {
+  mbir_ctx* ctx = nullptr;
+  mbir_problem prob{};
+  mbir_geometry g{};
+  hipStream_t stream = nullptr;
+  DevJob J{};
+  double offsetR = 0, offsetT = 0;
+  int spacing = 1;  // conflict-free x spacing of lines in one step
+  std::vector<double> angles_rad, H;
+  std::vector<double> I0h, muh, alphah;
+  // device buffers
+  DevBuf<float> x, e, y, wview, vlr_val, colv;
+  DevBuf<double> cosv, sinv, Hd, I0, mu, alpha, I0b, mub, sums6, yest, cost, prj;
+  DevBuf<int> vlr_t0, vlr_n, line_nnz, sched, colr;
+  DevBuf<uint8_t> mask;
+  DevBuf<SweepAccum> acc;
+  std::vector<int> sched_h, step_ptr;
+  std::atomic<int> cancel{0};
+  int timing = 0;
+  int sweep_counter = 0;
+  double stop_threshold = 0;
+  int64_t bytes = 0;
+  ~mbir_job()
+  {
+    x.release(); e.release(); y.release(); wview.release(); vlr_val.release(); colv.release();
+    cosv.release(); sinv.release(); Hd.release(); I0.release(); mu.release(); alpha.release();
+    I0b.release(); mub.release(); sums6.release(); yest.release(); cost.release(); prj.release();
+    vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
+    mask.release(); acc.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// ---------------------------------------------------------------------------
+extern "C" int32_t mbir_version(void) { return MBIR_VERSION; }
+
+extern "C" int32_t mbir_create(const mbir_opts* opts, mbir_ctx** out)
+{
+  if (!out) return MBIR_ERR_INPUT;
+  *out = nullptr;
+  auto* c = new mbir_ctx();
+  if (opts) {
+    c->device = opts->device;
+    c->verbose = opts->verbose;
+    c->budget = opts->hbm_budget_bytes;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || c->device < 0 || c->device >= n) {
+    delete c;
+    return MBIR_ERR_HIP;
+  }
+  if (hipSetDevice(c->device) != hipSuccess) {
+    delete c;
+    return MBIR_ERR_HIP;
+  }
+  *out = c;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_destroy(mbir_ctx* ctx)
+{
+  delete ctx;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_geometry_init(int32_t N_r, int32_t N_t, int32_t N_theta, double delta_r,
+                                      double delta_t, double thickness, double delta_xz_in,
+                                      double interp, int32_t extend_object,
+                                      const double* angles_deg, mbir_geometry* g)
+{
+  if (!g || !angles_deg || N_r <= 0 || N_t <= 0 || N_theta <= 0 || delta_r <= 0 ||
+      delta_t <= 0 || thickness <= 0 || delta_xz_in <= 0 || interp <= 0)
+    return MBIR_ERR_INPUT;
+  std::memset(g, 0, sizeof(*g));
+  g->N_r = N_r;
+  g->N_t = N_t;
+  g->N_theta = N_theta;
+  g->delta_r = delta_r;
+  g->delta_t = delta_t;
+  g->delta_xz = delta_r * delta_xz_in;  // :109
+  g->delta_xy = g->delta_xz;
+  double mx = 0;
+  for (int i = 0; i < N_theta; ++i) mx = std::max(mx, std::fabs(angles_deg[i]));
+  if (mx > 75.0) mx = 75.0;  // k_MaxAngleStretch
+  mx = mx * kPi / 180.0;
+  double LZ = thickness * 2.0;  // Z_STRETCH
+  LZ /= (interp * delta_r);
+  LZ = std::floor(LZ + 0.5) * interp * delta_r;
+  double LX = N_r * delta_r;
+  if (extend_object == 1) {
+    LX = 0.6 * ((N_r * delta_r) / std::cos(mx)) + LZ * std::tan(mx);
+    LX /= (interp * delta_r);
+    LX = std::floor(LX + 0.5) * interp * delta_r;
+  }
+  const double LY = N_t * delta_t;
+  g->N_x = (int32_t)(uint16_t)std::floor(LX / g->delta_xz);
+  g->N_z = (int32_t)(uint16_t)std::floor(LZ / g->delta_xz);
+  g->N_y = (int32_t)(uint16_t)std::floor(LY / g->delta_xy);
+  g->LengthX = LX;
+  g->LengthY = LY;
+  g->LengthZ = LZ;
+  g->x0 = -LX / 2;
+  g->z0 = -LZ / 2;
+  g->y0 = -LY / 2;
+  g->R0 = -(N_r * delta_r) / 2;
+  g->RMax = (N_r * delta_r) / 2;
+  g->T0 = -(N_t * delta_t) / 2;
+  g->TMax = (N_t * delta_t) / 2;
+  return (g->N_x > 0 && g->N_y > 0 && g->N_z > 0) ? MBIR_OK : MBIR_ERR_INPUT;
+}
+
+namespace {
+
+int setup_job(mbir_job* j)
+{
+  const mbir_problem& P = j->prob;
+  const mbir_geometry& g = j->g;
+  const int NT = g.N_theta;
+  const size_t M = (size_t)NT * g.N_r * g.N_t;
+  const size_t V = (size_t)g.N_z * g.N_x * g.N_y;
+  const size_t L = (size_t)g.N_z * g.N_x;
+
+  // ---- FP64 detector model on the host (execute(), :464-530) ----
+  std::vector<double> prof;
+  j->angles_rad.assign(P.angles_deg, P.angles_deg + NT);
+  voxel_profile(j->angles_rad, g.delta_xz, prof);  // first call, :464
+  j->offsetR = ((g.delta_xz / std::sqrt(3.0)) + g.delta_r / 2) / kResponseBins;  // :468
+  j->offsetT = ((g.delta_xz / 2) + g.delta_t / 2) / kResponseBins;                // :469
+  std::vector<double> cosv(NT), sinv(NT);
+  for (int i = 0; i < NT; ++i) {  // DetectorParameters.cpp:60-73
+    cosv[i] = std::cos(j->angles_rad[i]);
+    sinv[i] = std::sin(j->angles_rad[i]);
+  }
+  {
+    // second calculateVoxelProfile (:499) converts the radian angles once more;
+    // that profile feeds the detector response
+    std::vector<double> twice = j->angles_rad;
+    voxel_profile(twice, g.delta_xz, prof);
+  }
+  detector_response(g, prof, j->offsetR, j->H);
+  for (double h : j->H)
+    if (!std::isfinite(h)) return MBIR_ERR_RESPONSE;
+  std::vector<int> vt0, vn;
+  std::vector<float> vval;
+  int rc = voxel_line_responses(g, j->offsetT, vt0, vn, vval);
+  if (rc < 0) return rc;
+
+  // NB: detector bins a column can touch per view, floor((r+dxz-R0)/dr) - floor((r-dxz-R0)/dr) + 1
+  const int NB = (int)std::ceil(2.0 * g.delta_xz / g.delta_r - 1e-12) + 1;
+  if (NB > kMaxNB || NT * NB > kMaxSlots || g.N_y > kMaxNy) return MBIR_ERR_UNSUPPORTED;
+  if ((double)M >= 2147483647.0) return MBIR_ERR_UNSUPPORTED;
+
+  // conflict-free spacing: lines of one row dx apart never share a detector bin
+  // if dx * |cos th| >= 2 * 64 * OffsetR for every view (H support radius).
+  double cmin = 1.0;
+  for (int i = 0; i < NT; ++i) cmin = std::min(cmin, std::fabs(cosv[i]));
+  const double need = 2.0 * kResponseBins * j->offsetR / g.delta_xz;
+  int S = cmin > 1e-6 ? (int)std::ceil(need / cmin - 1e-9) : g.N_x;
+  S = std::max(2, std::min(S, g.N_x));
+  j->spacing = S;
+
+  // ROI mask (…_Extra.cpp:227-248)
+  std::vector<uint8_t> mask(L);
+  for (int i = 0; i < g.N_z; ++i)
+    for (int k = 0; k < g.N_x; ++k) {
+      const double xx = g.x0 + ((double)k + 0.5) * g.delta_xz;
+      const double zz = g.z0 + ((double)i + 0.5) * g.delta_xz;
+      mask[(size_t)i * g.N_x + k] = xx >= -(g.N_r * g.delta_r) / 2 &&
+                                    xx <= (g.N_r * g.delta_r) / 2 && zz >= -g.LengthZ / 2 &&
+                                    zz <= g.LengthZ / 2;
+    }
+
+  // gainAndOffsetInitialization (…_Extra.cpp:280-307) and initial variances
+  j->I0h.resize(NT);
+  j->muh.resize(NT);
+  j->alphah.resize(NT);
+  double sum = 0;
+  for (int k = 0; k < NT; ++k) {
+    const double g0 = P.init_gain ? P.init_gain[k] : P.target_gain;
+    j->I0h[k] = g0;
+    j->muh[k] = P.init_offset ? P.init_offset[k] : P.default_offset;
+    j->alphah[k] = P.init_variance ? P.init_variance[k] : P.default_variance;
+    sum += g0;
+  }
+  sum /= NT;
+  if (sum - P.target_gain > 1e-5)
+    for (int k = 0; k < NT; ++k) j->I0h[k] *= P.target_gain / sum;
+  for (int k = 0; k < NT; ++k)
+    if (!(j->I0h[k] != 0.0) || (P.noise_model && !(j->alphah[k] > 0.0))) return MBIR_ERR_INPUT;
+
+  // ---- device state ----
+  hipStream_t s = j->stream;
+  HIP_TRY(j->x.alloc(V));
+  HIP_TRY(j->e.alloc(M));
+  HIP_TRY(j->y.alloc(M));
+  HIP_TRY(j->wview.alloc(NT));
+  HIP_TRY(j->vlr_val.alloc((size_t)g.N_y * kVlrCap));
+  HIP_TRY(j->vlr_t0.alloc(g.N_y));
+  HIP_TRY(j->vlr_n.alloc(g.N_y));
+  HIP_TRY(j->cosv.alloc(NT));
+  HIP_TRY(j->sinv.alloc(NT));
+  HIP_TRY(j->Hd.alloc((size_t)NT * kResponseBins));
+  HIP_TRY(j->I0.alloc(NT));
+  HIP_TRY(j->mu.alloc(NT));
+  HIP_TRY(j->alpha.alloc(NT));
+  HIP_TRY(j->I0b.alloc(NT));
+  HIP_TRY(j->mub.alloc(NT));
+  HIP_TRY(j->sums6.alloc((size_t)NT * 6));
+  HIP_TRY(j->cost.alloc(4 + 2 * 1024));
+  HIP_TRY(j->mask.alloc(L));
+  HIP_TRY(j->line_nnz.alloc(L));
+  HIP_TRY(j->sched.alloc(L));
+  HIP_TRY(j->acc.alloc(1));
+  HIP_TRY(j->colv.alloc(kMaxSlots));
+  HIP_TRY(j->colr.alloc(kMaxSlots));
+  j->bytes = (int64_t)(V * 4 + 2 * M * 4 + L * 9 + (size_t)NT * 64 * 8);
+
+  HIP_TRY(hipMemcpyAsync(j->y.p, P.counts, M * sizeof(float), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->cosv.p, cosv.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->sinv.p, sinv.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->Hd.p, j->H.data(), j->H.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->vlr_t0.p, vt0.data(), vt0.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->vlr_n.p, vn.data(), vn.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->vlr_val.p, vval.data(), vval.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->mask.p, mask.data(), L, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->I0.p, j->I0h.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->mu.p, j->muh.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->alpha.p, j->alphah.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
+  // weights: identity noise model, w = 1 / alpha when the noise model is on (…_Extra.cpp:740-757)
+  std::vector<float> wv(NT);
+  for (int k = 0; k < NT; ++k) wv[k] = P.noise_model ? (float)(1.0 / j->alphah[k]) : 1.0f;
+  HIP_TRY(hipMemcpyAsync(j->wview.p, wv.data(), NT * sizeof(float), hipMemcpyHostToDevice, s));
+  if (P.init_volume)
+    HIP_TRY(hipMemcpyAsync(j->x.p, P.init_volume, V * sizeof(float), hipMemcpyHostToDevice, s));
+  else
+    HIP_TRY(launch_fill_f32(j->x.p, (float)P.default_recon, V, s));
+
+  DevJob& J = j->J;
+  J.N_theta = NT; J.N_r = g.N_r; J.N_t = g.N_t; J.N_x = g.N_x; J.N_y = g.N_y; J.N_z = g.N_z;
+  J.NB = NB;
+  J.nslots = NT * NB;
+  J.x0 = g.x0; J.z0 = g.z0; J.delta_xz = g.delta_xz; J.delta_r = g.delta_r;
+  J.R0 = g.R0; J.RMax = g.RMax; J.offsetR = j->offsetR;
+  J.cosv = j->cosv.p; J.sinv = j->sinv.p; J.H = j->Hd.p;
+  J.vlr_t0 = j->vlr_t0.p; J.vlr_n = j->vlr_n.p; J.vlr_val = j->vlr_val.p;
+  J.mask = j->mask.p; J.line_nnz = j->line_nnz.p;
+  J.x = j->x.p; J.e = j->e.p; J.y = j->y.p; J.wfull = nullptr; J.wview = j->wview.p;
+  J.I0 = j->I0.p; J.mu = j->mu.p;
+  // QGGMRF constants (HAADF/HAADF_QGGMRFPriorModel.cpp:10-19), MRF_P = 2
+  const double sp = std::pow(P.sigma_x, 2.0), spq = std::pow(P.sigma_x, 2.0 - P.mrf_q);
+  J.mrf_q = (float)P.mrf_q; J.mrf_c = (float)P.mrf_c; J.sigma_p = (float)sp;
+  J.sigma_pq = (float)spq; J.alpha = 1.5f; J.b0 = (float)(2.0 / (sp * P.mrf_c));
+  J.mrf_qd = P.mrf_q; J.mrf_cd = P.mrf_c; J.sigma_pd = sp; J.sigma_pqd = spq;
+
+  // columns: per-line nnz, zero lines with empty columns (:615-632)
+  HIP_TRY(launch_line_nnz(J, j->line_nnz.p, j->x.p, s));
+  // initial forward projection and error sinogram (:646-691)
+  HIP_TRY(j->yest.alloc(M));
+  HIP_TRY(launch_fill_f64(j->yest.p, 0.0, M, s));
+  HIP_TRY(launch_forward_project(J, j->x.p, j->yest.p, s));
+  HIP_TRY(launch_init_error(J, j->yest.p, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  j->yest.release();
+  j->stop_threshold = P.stop_threshold;
+  return MBIR_OK;
+}
+
+// Step schedule of one sweep.  schedule 0: step (z, c) = lines {(z, c + S m)},
+// S the conflict-free spacing, steps in a seeded random order.  schedule 1:
+// rows_per_step rows spaced N_z / rows_per_step apart share a step.
+void build_schedule(mbir_job* j)
+{
+  const int Nz = j->g.N_z, Nx = j->g.N_x, S = j->spacing;
+  const int R = j->prob.schedule == 1 ? std::max(1, std::min(j->prob.rows_per_step, Nz)) : 1;
+  const int Zs = (Nz + R - 1) / R;
+  std::vector<std::pair<int, int>> steps;
+  for (int zb = 0; zb < Zs; ++zb)
+    for (int c = 0; c < S; ++c) steps.push_back({zb, c});
+  std::mt19937 rng(j->prob.seed + 7919u * (uint32_t)j->sweep_counter);
+  std::shuffle(steps.begin(), steps.end(), rng);
+  j->sched_h.clear();
+  j->step_ptr.assign(1, 0);
+  for (auto& st : steps) {
+    for (int k = 0; k < R; ++k) {
+      const int z = st.first + k * Zs;
+      if (z >= Nz) continue;
+      for (int x = st.second; x < Nx; x += S) j->sched_h.push_back(z * Nx + x);
+    }
+    j->step_ptr.push_back((int)j->sched_h.size());
+  }
+}
+
+int run_sweep(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
+{
+  if (j->cancel.load()) return MBIR_ERR_CANCELLED;
+  hipStream_t s = j->stream;
+  build_schedule(j);
+  HIP_TRY(hipMemcpyAsync(j->sched.p, j->sched_h.data(), j->sched_h.size() * sizeof(int),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_fill_f64((double*)j->acc.p, 0.0, sizeof(SweepAccum) / sizeof(double), s));
+  const int nsteps = (int)j->step_ptr.size() - 1;
+  std::vector<hipEvent_t> ev;
+  if (j->timing) {
+    ev.resize(2 * nsteps);
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  }
+  for (int k = 0; k < nsteps; ++k) {
+    const int b = j->step_ptr[k], n = j->step_ptr[k + 1] - b;
+    if (j->timing) HIP_TRY(hipEventRecord(ev[2 * k], s));
+    HIP_TRY(launch_icd_step(j->J, j->sched.p + b, n, outer, inner, j->prob.zero_skipping,
+                            j->acc.p, s));
+    if (j->timing) HIP_TRY(hipEventRecord(ev[2 * k + 1], s));
+  }
+  SweepAccum a{};
+  HIP_TRY(hipMemcpyAsync(&a, j->acc.p, sizeof(a), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  double ms = 0;
+  if (j->timing) {
+    for (int k = 0; k < nsteps; ++k) {
+      float t = 0;
+      HIP_TRY(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
+      ms += t;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  j->sweep_counter++;
+  int status = 1;
+  // stop test (…_UpdateVoxels.cpp:771-786)
+  if (a.sum_abs_value > 0 && (a.sum_abs_update / a.sum_abs_value) < j->stop_threshold) status = 0;
+  if (st) {
+    st->sum_abs_update = a.sum_abs_update;
+    st->sum_abs_value = a.sum_abs_value;
+    st->zero_skipped = (int64_t)a.zero_skipped;
+    st->updated = (int64_t)a.updated;
+    st->nnz_updated = (int64_t)a.nnz_updated;
+    st->kernel_ms = ms;
+    st->steps = nsteps;
+    st->status = status;
+  }
+  return status;
+}
+
+int nuisance_update(mbir_job* j)
+{
+  hipStream_t s = j->stream;
+  const int NT = j->g.N_theta;
+  if (j->prob.joint_estimation) {
+    HIP_TRY(launch_joint_sums(j->J, j->sums6.p, s));
+    HIP_TRY(launch_joint_solve(NT, j->sums6.p, j->prob.target_gain, j->I0.p, j->mu.p, j->I0b.p,
+                               j->mub.p, s));
+    HIP_TRY(launch_joint_apply(j->J, j->I0.p, j->mu.p, j->I0b.p, j->mub.p, s));
+    std::swap(j->I0.p, j->I0b.p);
+    std::swap(j->mu.p, j->mub.p);
+    j->J.I0 = j->I0.p;
+    j->J.mu = j->mu.p;
+  }
+  if (j->prob.noise_model) HIP_TRY(launch_noise_update(j->J, j->alpha.p, j->wview.p, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return MBIR_OK;
+}
+
+int job_cost(mbir_job* j, double* total, double* terms3)
+{
+  hipStream_t s = j->stream;
+  HIP_TRY(launch_cost(j->J, j->alpha.p, j->prob.noise_model, j->cost.p, s));
+  double out[3];
+  HIP_TRY(hipMemcpyAsync(out, j->cost.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (terms3) std::memcpy(terms3, out, sizeof(out));
+  if (total) *total = out[0] + out[1] + (j->prob.noise_model ? out[2] : 0.0);
+  return MBIR_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t mbir_job_create(mbir_ctx* ctx, const mbir_problem* prob, mbir_job** out)
+{
+  if (!ctx || !prob || !out) return MBIR_ERR_INPUT;
+  *out = nullptr;
+  const mbir_geometry& g = prob->geom;
+  if (g.N_theta <= 0 || g.N_r <= 0 || g.N_t <= 0 || g.N_x <= 0 || g.N_y <= 0 || g.N_z <= 0 ||
+      !prob->angles_deg || !prob->counts || !(g.delta_r > 0) || !(g.delta_t > 0) ||
+      !(g.delta_xz > 0) || !(g.delta_xy > 0) || !(prob->sigma_x > 0) || !(prob->mrf_c > 0) ||
+      !(prob->mrf_q >= 1.0 && prob->mrf_q <= 2.0))
+    return MBIR_ERR_INPUT;
+  if (hipSetDevice(ctx->device) != hipSuccess) return MBIR_ERR_HIP;
+  auto* j = new mbir_job();
+  j->ctx = ctx;
+  j->prob = *prob;
+  j->g = g;
+  j->prob.angles_deg = nullptr;  // host pointers are not kept
+  j->prob.counts = nullptr;
+  j->prob.init_volume = nullptr;
+  j->prob.init_gain = j->prob.init_offset = j->prob.init_variance = nullptr;
+  if (hipStreamCreate(&j->stream) != hipSuccess) {
+    delete j;
+    return MBIR_ERR_HIP;
+  }
+  mbir_problem tmp = *prob;
+  std::swap(j->prob, tmp);
+  int rc = setup_job(j);
+  std::swap(j->prob, tmp);
+  if (rc != MBIR_OK) {
+    delete j;
+    return rc;
+  }
+  *out = j;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_destroy(mbir_job* job)
+{
+  if (!job) return MBIR_ERR_INPUT;
+  (void)hipStreamSynchronize(job->stream);
+  delete job;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_sweep(mbir_job* job, int32_t outer, int32_t inner, mbir_sweep_stats* st)
+{
+  if (!job) return MBIR_ERR_INPUT;
+  return run_sweep(job, outer, inner, st);
+}
+
+extern "C" int32_t mbir_job_nuisance_update(mbir_job* job)
+{
+  if (!job) return MBIR_ERR_INPUT;
+  if (job->cancel.load()) return MBIR_ERR_CANCELLED;
+  return nuisance_update(job);
+}
+
+extern "C" int32_t mbir_job_cost(mbir_job* job, double* total, double* terms3)
+{
+  if (!job) return MBIR_ERR_INPUT;
+  return job_cost(job, total, terms3);
+}
+
+extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, int32_t cap,
+                                int32_t* n_costs)
+{
+  if (!j) return MBIR_ERR_INPUT;
+  auto t0 = std::chrono::steady_clock::now();
+  int nc = 0, ns = 0, status = 1, outer_done = 0;
+  auto push_cost = [&]() -> int {
+    if (!costs || nc >= cap) return 0;
+    double c = 0;
+    int rc = job_cost(j, &c, nullptr);
+    if (rc < 0) return rc;
+    costs[nc++] = c;
+    return 0;
+  };
+  int rc = push_cost();
+  if (rc < 0) return rc;
+  int num_iter = j->prob.num_inner;
+  for (int outer = 0; outer < j->prob.num_outer; ++outer) {
+    if (outer != 0) num_iter = 1;  // :712-715
+    for (int inner = 0; inner < num_iter; ++inner) {
+      status = run_sweep(j, outer, inner, nullptr);
+      if (status < 0) return status;
+      ns++;
+      if ((rc = push_cost()) < 0) return rc;
+      if (status == 0) break;
+    }
+    if (j->cancel.load()) return MBIR_ERR_CANCELLED;
+    if (j->prob.joint_estimation || j->prob.noise_model) {
+      if (j->prob.joint_estimation) {
+        // joint estimation alone, then the noise update, so the cost after each is recorded
+        const int nm = j->prob.noise_model;
+        j->prob.noise_model = 0;
+        rc = nuisance_update(j);
+        j->prob.noise_model = nm;
+        if (rc < 0) return rc;
+        if ((rc = push_cost()) < 0) return rc;
+      }
+      if (j->prob.noise_model) {
+        const int je = j->prob.joint_estimation;
+        j->prob.joint_estimation = 0;
+        rc = nuisance_update(j);
+        j->prob.joint_estimation = je;
+        if (rc < 0) return rc;
+        if ((rc = push_cost()) < 0) return rc;
+      }
+    }
+    outer_done = outer + 1;
+    if (status == 0 && outer >= 1) break;  // :789-802
+  }
+  if (rs) {
+    rs->sweeps = ns;
+    rs->outer_iterations = outer_done;
+    rs->status = status;
+    rs->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (n_costs) *n_costs = nc;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_forward_project(mbir_job* j, const float* volume, double* out)
+{
+  if (!j || !out) return MBIR_ERR_INPUT;
+  const size_t M = (size_t)j->g.N_theta * j->g.N_r * j->g.N_t;
+  const size_t V = (size_t)j->g.N_z * j->g.N_x * j->g.N_y;
+  hipStream_t s = j->stream;
+  DevBuf<double> yest;
+  DevBuf<float> vol;
+  if (yest.alloc(M) != hipSuccess) return MBIR_ERR_NOMEM;
+  const float* src = j->x.p;
+  if (volume) {
+    if (vol.alloc(V) != hipSuccess) {
+      yest.release();
+      return MBIR_ERR_NOMEM;
+    }
+    if (hipMemcpyAsync(vol.p, volume, V * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) {
+      yest.release();
+      vol.release();
+      return MBIR_ERR_HIP;
+    }
+    src = vol.p;
+  }
+  int rc = MBIR_OK;
+  if (launch_fill_f64(yest.p, 0.0, M, s) != hipSuccess ||
+      launch_forward_project(j->J, src, yest.p, s) != hipSuccess ||
+      hipMemcpyAsync(out, yest.p, M * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    rc = MBIR_ERR_HIP;
+  yest.release();
+  vol.release();
+  return rc;
+}
+
+extern "C" int32_t mbir_job_get_column(mbir_job* j, int32_t z, int32_t x, uint32_t* index,
+                                       float* values, int32_t cap)
+{
+  if (!j || z < 0 || x < 0 || z >= j->g.N_z || x >= j->g.N_x) return MBIR_ERR_INPUT;
+  hipStream_t s = j->stream;
+  const int ns = j->J.nslots;
+  std::vector<float> v(ns);
+  std::vector<int> r(ns);
+  HIP_TRY(launch_export_column(j->J, z, x, j->colv.p, j->colr.p, s));
+  HIP_TRY(hipMemcpyAsync(v.data(), j->colv.p, ns * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(r.data(), j->colr.p, ns * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  int n = 0;
+  for (int k = 0; k < ns; ++k)
+    if (v[k] > 0.f) {
+      if (n < cap) {
+        if (index) index[n] = (uint32_t)r[k];
+        if (values) values[n] = v[k];
+      }
+      n++;
+    }
+  return n;
+}
+
+extern "C" int32_t mbir_job_get_volume(mbir_job* j, float* out)
+{
+  if (!j || !out) return MBIR_ERR_INPUT;
+  const size_t V = (size_t)j->g.N_z * j->g.N_x * j->g.N_y;
+  HIP_TRY(hipMemcpyAsync(out, j->x.p, V * sizeof(float), hipMemcpyDeviceToHost, j->stream));
+  HIP_TRY(hipStreamSynchronize(j->stream));
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_set_volume(mbir_job* j, const float* vol)
+{
+  if (!j || !vol) return MBIR_ERR_INPUT;
+  const size_t V = (size_t)j->g.N_z * j->g.N_x * j->g.N_y;
+  const size_t M = (size_t)j->g.N_theta * j->g.N_r * j->g.N_t;
+  hipStream_t s = j->stream;
+  HIP_TRY(hipMemcpyAsync(j->x.p, vol, V * sizeof(float), hipMemcpyHostToDevice, s));
+  HIP_TRY(j->yest.alloc(M));
+  HIP_TRY(launch_fill_f64(j->yest.p, 0.0, M, s));
+  HIP_TRY(launch_forward_project(j->J, j->x.p, j->yest.p, s));
+  HIP_TRY(launch_init_error(j->J, j->yest.p, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  j->yest.release();
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_get_error_sinogram(mbir_job* j, float* out)
+{
+  if (!j || !out) return MBIR_ERR_INPUT;
+  const size_t M = (size_t)j->g.N_theta * j->g.N_r * j->g.N_t;
+  HIP_TRY(hipMemcpyAsync(out, j->e.p, M * sizeof(float), hipMemcpyDeviceToHost, j->stream));
+  HIP_TRY(hipStreamSynchronize(j->stream));
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_get_nuisance(mbir_job* j, double* I0, double* mu, double* alpha)
+{
+  if (!j) return MBIR_ERR_INPUT;
+  const size_t n = (size_t)j->g.N_theta * sizeof(double);
+  if (I0) HIP_TRY(hipMemcpyAsync(I0, j->I0.p, n, hipMemcpyDeviceToHost, j->stream));
+  if (mu) HIP_TRY(hipMemcpyAsync(mu, j->mu.p, n, hipMemcpyDeviceToHost, j->stream));
+  if (alpha) HIP_TRY(hipMemcpyAsync(alpha, j->alpha.p, n, hipMemcpyDeviceToHost, j->stream));
+  HIP_TRY(hipStreamSynchronize(j->stream));
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_get_geometry(mbir_job* j, mbir_geometry* out)
+{
+  if (!j || !out) return MBIR_ERR_INPUT;
+  *out = j->g;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_get_detector_response(mbir_job* j, double* out)
+{
+  if (!j || !out) return MBIR_ERR_INPUT;
+  std::memcpy(out, j->H.data(), j->H.size() * sizeof(double));
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_cancel(mbir_job* j)
+{
+  if (!j) return MBIR_ERR_INPUT;
+  j->cancel.store(1);
+  return MBIR_OK;
+}
+
+extern "C" int64_t mbir_job_device_bytes(const mbir_job* j) { return j ? j->bytes : -1; }
+
+extern "C" int32_t mbir_job_set_timing(mbir_job* j, int32_t on)
+{
+  if (!j) return MBIR_ERR_INPUT;
+  j->timing = on ? 1 : 0;
+  return MBIR_OK;
+}

@@ -1,0 +1,76 @@
+// mbir_internal.h -- device-side job descriptor and kernel launchers shared by
+// the HIP kernels (mbir_kernels.hip) and the host engine (mbir_capi.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mbir {
+
+constexpr int kResponseBins = 64;     // DETECTOR_RESPONSE_BINS (HAADF_ReconstructionEngine.cpp:249)
+constexpr int kProfileResolution = 1536;
+constexpr int kBeamResolution = 512;
+constexpr int kBlock = 256;           // threads per ICD workgroup (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxNy = 256;           // voxels per line handled by one workgroup
+constexpr int kMaxSlots = 2048;       // N_theta * NB per column
+constexpr int kMaxNB = 8;             // detector bins per view per column
+constexpr int kVlrCap = 4;            // detector rows per voxel slice
+
+// Everything a kernel needs about one job; passed by value.
+struct DevJob {
+  int N_theta, N_r, N_t, N_x, N_y, N_z;
+  int NB;                // slots (candidate detector bins) per view
+  int nslots;            // N_theta * NB
+  double x0, z0, delta_xz, delta_r, R0, RMax, offsetR;
+  const double* cosv;    // [N_theta]
+  const double* sinv;    // [N_theta]
+  const double* H;       // [N_theta][64] detector response
+  const int* vlr_t0;     // [N_y] first detector row of slice y
+  const int* vlr_n;      // [N_y] rows of slice y (<= kVlrCap)
+  const float* vlr_val;  // [N_y][kVlrCap]
+  const uint8_t* mask;   // [N_z][N_x] ROI
+  const int* line_nnz;   // [N_z][N_x] A-column entries per line (0 = empty column)
+  float* x;              // [N_z][N_x][N_y]
+  float* e;              // [N_theta][N_r][N_t] error sinogram
+  const float* y;        // [N_theta][N_r][N_t] measurements
+  const float* wfull;    // [N_theta][N_r][N_t] or nullptr (per-view weights)
+  const float* wview;    // [N_theta] per-view weight (identity noise model: 1/alpha)
+  const double* I0;      // [N_theta]
+  const double* mu;      // [N_theta]
+  // QGGMRF (HAADF/HAADF_QGGMRFPriorModel.cpp:10-19), MRF_P = 2
+  float mrf_q, mrf_c, sigma_p, sigma_pq, alpha, b0;  // b0 = P / (sigma^P c)
+  double mrf_qd, mrf_cd, sigma_pd, sigma_pqd;
+};
+
+struct SweepAccum {  // device accumulators of one sweep
+  double sum_abs_update;
+  double sum_abs_value;
+  unsigned long long zero_skipped;
+  unsigned long long updated;
+  unsigned long long nnz_updated;
+  unsigned long long pad[3];
+};
+
+// ---- launchers (all asynchronous on `s`) ----
+hipError_t launch_line_nnz(const DevJob& J, int* line_nnz, float* x, hipStream_t s);
+hipError_t launch_forward_project(const DevJob& J, const float* vol, double* out, hipStream_t s);
+hipError_t launch_init_error(const DevJob& J, const double* yest, hipStream_t s);
+hipError_t launch_icd_step(const DevJob& J, const int* lines, int nlines, int outer, int inner,
+                           int zero_skipping, SweepAccum* acc, hipStream_t s);
+hipError_t launch_export_column(const DevJob& J, int z, int x, float* vals, int* rows,
+                                hipStream_t s);
+// joint estimation / weights / cost (FP64)
+hipError_t launch_joint_sums(const DevJob& J, double* sums6, hipStream_t s);
+hipError_t launch_joint_solve(int N_theta, const double* sums6, double target_gain,
+                              const double* I0_old, const double* mu_old, double* I0_new,
+                              double* mu_new, hipStream_t s);
+hipError_t launch_joint_apply(const DevJob& J, const double* I0_old, const double* mu_old,
+                              const double* I0_new, const double* mu_new, hipStream_t s);
+hipError_t launch_noise_update(const DevJob& J, double* alpha, float* wview, hipStream_t s);
+hipError_t launch_cost(const DevJob& J, const double* alpha, int noise_model, double* out3,
+                       hipStream_t s);
+hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
+hipError_t launch_fill_f64(double* p, double v, size_t n, hipStream_t s);
+
+}  // namespace mbir

@@ -1,0 +1,143 @@
+"""GPU parity: the HIP path (through the C-ABI) against the FP64 oracle.
+
+Tolerances are those of the engine's contract: A-columns and the forward
+projection within 1e-5 relative of the FP64 oracle, the MBIR cost
+non-increasing on every sweep, and the converged volume / final cost within the
+tolerances written in each test.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from mbirtem import capi
+from mbirtem import phantom as P
+from problems import small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _job(ctx, g, ang, y, **kw):
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, thickness=g.LengthZ / 2)
+    return capi.Job(ctx, geom, ang, y, **kw)
+
+
+def _engine(g, ang, y, **kw):
+    prm = O.params(**kw)
+    return O.Engine(g, ang, y, prm)
+
+
+def test_geometry_matches_oracle(gpu_ctx):
+    g, ang, vol, y = small_problem()
+    job = _job(gpu_ctx, g, ang, y)
+    G = job.geom
+    for f in ("N_x", "N_y", "N_z", "N_r", "N_t", "N_theta"):
+        assert getattr(G, f) == getattr(g, f), f
+    for f in ("x0", "y0", "z0", "R0", "T0", "delta_xz", "LengthZ"):
+        assert getattr(G, f) == pytest.approx(getattr(g, f), abs=0), f
+
+
+@pytest.mark.parametrize("n_r,n_t,views,deg,thick", [(32, 8, 31, 60.0, 8), (128, 16, 61, 60.0, 32),
+                                                     (96, 8, 45, 70.0, 20)])
+def test_detector_response_and_columns(gpu_ctx, n_r, n_t, views, deg, thick):
+    ang = P.tilt_series(views, deg)
+    g = O.geometry(n_r, n_t, ang, thickness=thick)
+    y = np.zeros((views, n_r, n_t))
+    eng = _engine(g, ang, y)
+    job = _job(gpu_ctx, g, ang, y)
+    np.testing.assert_allclose(job.detector_response(), eng.H(), rtol=1e-12, atol=1e-14)
+    rng = np.random.default_rng(0)
+    lines = [(0, 0), (g.N_z - 1, g.N_x - 1), (g.N_z // 2, g.N_x // 2)]
+    lines += [(int(rng.integers(g.N_z)), int(rng.integers(g.N_x))) for _ in range(40)]
+    for z, x in lines:
+        ri, rv = eng.col(z, x)
+        gi, gv = job.column(z, x)
+        dense_r = np.zeros(views * n_r)
+        dense_g = np.zeros(views * n_r)
+        dense_r[ri] = rv
+        dense_g[gi] = gv
+        scale = max(dense_r.max(), 1e-30)
+        # 1e-5 relative (of the column's peak) for every entry
+        assert np.max(np.abs(dense_g - dense_r)) <= 1e-5 * scale, (z, x)
+        # no entry of one is missing from the other above round-off
+        assert set(ri[rv > 1e-5 * scale]) <= set(gi) and set(gi[gv > 1e-5 * scale]) <= set(ri)
+
+
+def test_forward_projection(gpu_ctx):
+    g, ang, vol, y = small_problem(n_r=64, n_t=16, n_views=41, thickness=16)
+    eng = _engine(g, ang, y)
+    job = _job(gpu_ctx, g, ang, y)
+    ref = eng.project(vol)
+    got = job.forward_project(vol.astype(np.float32))
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(got - ref)) <= 1e-5 * scale
+    # initial error sinogram e = y - A x0 - mu with x0 = 0
+    np.testing.assert_allclose(job.error_sinogram(), eng.e(), rtol=0, atol=1e-5 * np.abs(y).max())
+
+
+def test_sweep_cost_monotone_and_error_consistency(gpu_ctx):
+    g, ang, vol, y = small_problem()
+    job = _job(gpu_ctx, g, ang, y)
+    eng = _engine(g, ang, y)
+    c0 = job.cost()
+    assert c0 == pytest.approx(eng.cost(), rel=1e-6)
+    costs = [c0]
+    for it in range(8):
+        status, st = job.sweep(0, it)
+        assert st.updated + st.zero_skipped > 0
+        costs.append(job.cost())
+    # non-increasing cost on every sweep (relative slack for fp32 round-off)
+    for a, b in zip(costs, costs[1:]):
+        assert b <= a * (1 + 1e-6), costs
+    # e stays consistent with the volume: e == y - A x - mu to fp32 accuracy
+    x = job.volume()
+    proj = job.forward_project(x)
+    e_ref = y - proj
+    assert np.max(np.abs(job.error_sinogram() - e_ref)) <= 2e-4 * max(1.0, np.abs(y).max())
+
+
+def test_converged_volume_matches_oracle(gpu_ctx):
+    g, ang, vol, y = small_problem()
+    eng = _engine(g, ang, y)
+    job = _job(gpu_ctx, g, ang, y)
+    for it in range(60):
+        eng.sweep(0, it)
+        job.sweep(0, it)
+    xr, xg = eng.x(), job.volume()
+    rmse = np.sqrt(np.mean((xr - xg) ** 2))
+    scale = np.sqrt(np.mean(xr ** 2))
+    # converged MAP estimates agree to 1% RMSE of the volume's RMS and 1e-4 in cost
+    assert rmse <= 1e-2 * scale, (rmse, scale)
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-4)
+
+
+def test_joint_estimation_and_noise_model(gpu_ctx):
+    rng = np.random.default_rng(7)
+    nv = 31
+    gains = rng.uniform(0.8, 1.2, nv)
+    offs = rng.uniform(-0.05, 0.05, nv)
+    g, ang, vol, y = small_problem(n_views=nv, gains=gains, offsets=offs)
+    # 30 outer iterations: converged enough that the update order no longer matters
+    # (two oracle runs with different seeds then agree to ~2% in alpha)
+    kw = dict(num_outer=30, num_inner=10, joint_estimation=1, noise_model=1, stop_threshold=1e-4)
+    eng = _engine(g, ang, y, **kw)
+    job = _job(gpu_ctx, g, ang, y, **kw)
+    costs_r, nsw_r = eng.run()
+    rs, costs_g = job.run()
+    I0r, mur, alr = eng.nuisance()
+    I0g, mug, alg = job.nuisance()
+    assert np.mean(I0g) == pytest.approx(1.0, abs=1e-9)  # mean-gain constraint
+    np.testing.assert_allclose(I0g, I0r, rtol=1e-2)
+    np.testing.assert_allclose(mug, mur, atol=5e-3)
+    np.testing.assert_allclose(alg, alr, rtol=5e-2)
+    xr, xg = eng.x(), job.volume()
+    assert np.sqrt(np.mean((xr - xg) ** 2)) <= 2e-2 * np.sqrt(np.mean(xr ** 2))
+    assert costs_g[-1] == pytest.approx(costs_r[-1], rel=1e-3)
+
+
+def test_cancel_returns_minus_999(gpu_ctx):
+    g, ang, vol, y = small_problem()
+    job = _job(gpu_ctx, g, ang, y)
+    job.cancel()
+    with pytest.raises(capi.MbirError) as ei:
+        job.sweep(0, 0)
+    assert ei.value.code == capi.MBIR_ERR_CANCELLED
