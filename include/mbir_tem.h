@@ -96,6 +96,9 @@ typedef struct mbir_problem {
    * sequential ICD order); 1 = relaxed groups of rows_per_step spaced rows. */
   int32_t schedule;
   int32_t rows_per_step;
+  /* sweep kernels: 0 = one launch per conflict-free step with the job's own kernel
+   * (plain per-job GPU ICD); 1 = the batched step kernel (a batch of one job) */
+  int32_t kernel_mode;
 } mbir_problem;
 
 /* Per-sweep statistics (updateVoxels, HAADF_ReconstructionEngine_UpdateVoxels.cpp:717-786) */
@@ -143,6 +146,15 @@ int32_t mbir_job_destroy(mbir_job* job);
 /* One ICD sweep: updateVoxels(outer, inner, ...) (…_UpdateVoxels.cpp:594-817).
  * Returns 1 = continue, 0 = converged (stop test met), negative on error. */
 int32_t mbir_job_sweep(mbir_job* job, int32_t outer, int32_t inner, mbir_sweep_stats* stats);
+
+/* Batched sweep: one ICD sweep of each of n jobs of ctx (no reference counterpart:
+ * the reference runs one job per process).  Launch k runs conflict-free step k of
+ * every job that has one, so the launches carry many jobs' lines at once; jobs stay
+ * independent (a job's result is the same alone and in a batch).  Jobs need
+ * N_y <= 128 and at most 2 detector rows per slice.  outer/inner (may be NULL = 0)
+ * are per-job iteration indices, stats (may be NULL) per-job results. */
+int32_t mbir_batch_sweep(mbir_ctx* ctx, mbir_job* const* jobs, int32_t n, const int32_t* outer,
+                         const int32_t* inner, mbir_sweep_stats* stats);
 
 /* jointEstimation (HAADF/HAADF_ReconstructionEngine_Extra.cpp:419-687) when enabled,
  * then updateWeights (:795-881) when the noise model is on.  FP64 on the device. */

@@ -178,6 +178,14 @@ struct mbir_ctx {
   int verbose = 0;
   uint64_t budget = 0;
   std::mutex mu;
+  hipStream_t stream = nullptr;  // stream of batched launches
+  DevBuf<DevJob> djobs;          // job table of a batched sweep
+  DevBuf<BatchEntry> ents;       // entry table of a batched sweep
+  ~mbir_ctx()
+  {
+    djobs.release(); ents.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 };
 
 struct mbir_job {
@@ -193,7 +201,7 @@ struct mbir_job {
   // device buffers
   DevBuf<float> x, e, y, wview, vlr_val, colv;
   DevBuf<double> cosv, sinv, Hd, I0, mu, alpha, I0b, mub, sums6, yest, cost, prj;
-  DevBuf<int> vlr_t0, vlr_n, line_nnz, sched, colr;
+  DevBuf<int> vlr_t0, vlr_n, line_nnz, sched, colr, step_ptr_d;
   DevBuf<uint8_t> mask;
   DevBuf<SweepAccum> acc;
   std::vector<int> sched_h, step_ptr;
@@ -208,7 +216,7 @@ struct mbir_job {
     cosv.release(); sinv.release(); Hd.release(); I0.release(); mu.release(); alpha.release();
     I0b.release(); mub.release(); sums6.release(); yest.release(); cost.release(); prj.release();
     vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
-    mask.release(); acc.release();
+    step_ptr_d.release(); mask.release(); acc.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -231,7 +239,7 @@ extern "C" int32_t mbir_create(const mbir_opts* opts, mbir_ctx** out)
     delete c;
     return MBIR_ERR_HIP;
   }
-  if (hipSetDevice(c->device) != hipSuccess) {
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     return MBIR_ERR_HIP;
   }
@@ -329,7 +337,8 @@ int setup_job(mbir_job* j)
 
   // NB: detector bins a column can touch per view, floor((r+dxz-R0)/dr) - floor((r-dxz-R0)/dr) + 1
   const int NB = (int)std::ceil(2.0 * g.delta_xz / g.delta_r - 1e-12) + 1;
-  if (NB > kMaxNB || NT * NB > kMaxSlots || g.N_y > kMaxNy) return MBIR_ERR_UNSUPPORTED;
+  if (NB > kMaxNB || NT * NB > kMaxSlots || g.N_y > kMaxNy || NT > kIcdBlock)
+    return MBIR_ERR_UNSUPPORTED;
   if ((double)M >= 2147483647.0) return MBIR_ERR_UNSUPPORTED;
 
   // conflict-free spacing: lines of one row dx apart never share a detector bin
@@ -392,6 +401,7 @@ int setup_job(mbir_job* j)
   HIP_TRY(j->mask.alloc(L));
   HIP_TRY(j->line_nnz.alloc(L));
   HIP_TRY(j->sched.alloc(L));
+  HIP_TRY(j->step_ptr_d.alloc(L + 1));
   HIP_TRY(j->acc.alloc(1));
   HIP_TRY(j->colv.alloc(kMaxSlots));
   HIP_TRY(j->colr.alloc(kMaxSlots));
@@ -421,8 +431,11 @@ int setup_job(mbir_job* j)
   J.N_theta = NT; J.N_r = g.N_r; J.N_t = g.N_t; J.N_x = g.N_x; J.N_y = g.N_y; J.N_z = g.N_z;
   J.NB = NB;
   J.nslots = NT * NB;
+  J.vlr_max = 1;
+  for (int c : vn) J.vlr_max = std::max(J.vlr_max, c);
   J.x0 = g.x0; J.z0 = g.z0; J.delta_xz = g.delta_xz; J.delta_r = g.delta_r;
   J.R0 = g.R0; J.RMax = g.RMax; J.offsetR = j->offsetR;
+  J.inv_delta_r = 1.0 / g.delta_r; J.inv_offsetR = 1.0 / j->offsetR;
   J.cosv = j->cosv.p; J.sinv = j->sinv.p; J.H = j->Hd.p;
   J.vlr_t0 = j->vlr_t0.p; J.vlr_n = j->vlr_n.p; J.vlr_val = j->vlr_val.p;
   J.mask = j->mask.p; J.line_nnz = j->line_nnz.p;
@@ -432,6 +445,7 @@ int setup_job(mbir_job* j)
   const double sp = std::pow(P.sigma_x, 2.0), spq = std::pow(P.sigma_x, 2.0 - P.mrf_q);
   J.mrf_q = (float)P.mrf_q; J.mrf_c = (float)P.mrf_c; J.sigma_p = (float)sp;
   J.sigma_pq = (float)spq; J.alpha = 1.5f; J.b0 = (float)(2.0 / (sp * P.mrf_c));
+  J.pq2 = (float)(2.0 - P.mrf_q); J.inv_sigma_p = (float)(1.0 / sp); J.inv_sigma_pq = (float)(1.0 / spq);
   J.mrf_qd = P.mrf_q; J.mrf_cd = P.mrf_c; J.sigma_pd = sp; J.sigma_pqd = spq;
 
   // columns: per-line nnz, zero lines with empty columns (:615-632)
@@ -472,14 +486,46 @@ void build_schedule(mbir_job* j)
   }
 }
 
-int run_sweep(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
+// Upload the sweep's schedule and clear its accumulators (job stream).
+int prepare_sweep(mbir_job* j)
 {
-  if (j->cancel.load()) return MBIR_ERR_CANCELLED;
   hipStream_t s = j->stream;
   build_schedule(j);
   HIP_TRY(hipMemcpyAsync(j->sched.p, j->sched_h.data(), j->sched_h.size() * sizeof(int),
                          hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(j->step_ptr_d.p, j->step_ptr.data(), j->step_ptr.size() * sizeof(int),
+                         hipMemcpyHostToDevice, s));
   HIP_TRY(launch_fill_f64((double*)j->acc.p, 0.0, sizeof(SweepAccum) / sizeof(double), s));
+  return MBIR_OK;
+}
+
+// Read the accumulators and apply the stop test (…_UpdateVoxels.cpp:765-786).
+int finish_sweep(mbir_job* j, double ms, mbir_sweep_stats* st)
+{
+  SweepAccum a{};
+  HIP_TRY(hipMemcpyAsync(&a, j->acc.p, sizeof(a), hipMemcpyDeviceToHost, j->stream));
+  HIP_TRY(hipStreamSynchronize(j->stream));
+  j->sweep_counter++;
+  int status = 1;
+  if (a.sum_abs_value > 0 && (a.sum_abs_update / a.sum_abs_value) < j->stop_threshold) status = 0;
+  if (st) {
+    st->sum_abs_update = a.sum_abs_update;
+    st->sum_abs_value = a.sum_abs_value;
+    st->zero_skipped = (int64_t)a.zero_skipped;
+    st->updated = (int64_t)a.updated;
+    st->nnz_updated = (int64_t)a.nnz_updated;
+    st->kernel_ms = ms;
+    st->steps = (int)j->step_ptr.size() - 1;
+    st->status = status;
+  }
+  return status;
+}
+
+int run_sweep_steps(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
+{
+  hipStream_t s = j->stream;
+  int rc = prepare_sweep(j);
+  if (rc < 0) return rc;
   const int nsteps = (int)j->step_ptr.size() - 1;
   std::vector<hipEvent_t> ev;
   if (j->timing) {
@@ -493,8 +539,6 @@ int run_sweep(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
                             j->acc.p, s));
     if (j->timing) HIP_TRY(hipEventRecord(ev[2 * k + 1], s));
   }
-  SweepAccum a{};
-  HIP_TRY(hipMemcpyAsync(&a, j->acc.p, sizeof(a), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   double ms = 0;
   if (j->timing) {
@@ -505,21 +549,132 @@ int run_sweep(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
   }
-  j->sweep_counter++;
-  int status = 1;
-  // stop test (…_UpdateVoxels.cpp:771-786)
-  if (a.sum_abs_value > 0 && (a.sum_abs_update / a.sum_abs_value) < j->stop_threshold) status = 0;
-  if (st) {
-    st->sum_abs_update = a.sum_abs_update;
-    st->sum_abs_value = a.sum_abs_value;
-    st->zero_skipped = (int64_t)a.zero_skipped;
-    st->updated = (int64_t)a.updated;
-    st->nnz_updated = (int64_t)a.nnz_updated;
-    st->kernel_ms = ms;
-    st->steps = nsteps;
-    st->status = status;
+  return finish_sweep(j, ms, st);
+}
+
+// Batched sweep: launch k runs step k of every job that has one (each job's
+// steps are its own conflict-free groups; jobs never share memory), so one
+// launch carries the lines of many jobs and the per-launch cost is shared.
+int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* outer,
+              const int* inner, mbir_sweep_stats* stats)
+{
+  const int n = (int)jobs.size();
+  if (n == 0) return MBIR_OK;
+  std::vector<DevJob> dj(n);
+  int max_steps = 0;
+  for (int i = 0; i < n; ++i) {
+    mbir_job* j = jobs[i];
+    if (j->ctx != ctx) return MBIR_ERR_INPUT;
+    if (j->cancel.load()) return MBIR_ERR_CANCELLED;
+    if (j->J.N_y > 128 || j->J.vlr_max > 2) return MBIR_ERR_UNSUPPORTED;
+    int rc = prepare_sweep(j);
+    if (rc < 0) return rc;
+    dj[i] = j->J;
+    max_steps = std::max(max_steps, (int)j->step_ptr.size() - 1);
   }
-  return status;
+  // jobs ordered by kernel shape so that each step's entries form one run per shape
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    int ya, na, yb, nb;
+    icd_shape(jobs[a]->J, jobs[a]->J.vlr_max, &ya, &na);
+    icd_shape(jobs[b]->J, jobs[b]->J.vlr_max, &yb, &nb);
+    return ya != yb ? ya < yb : na < nb;
+  });
+  std::vector<BatchEntry> ent;
+  std::vector<int> l_e0(max_steps), l_ne(max_steps), l_wg(max_steps);
+  for (int k = 0; k < max_steps; ++k) {
+    l_e0[k] = (int)ent.size();
+    int wg = 0, cur_ypl = -1, cur_nv = -1;
+    for (int i : order) {
+      mbir_job* j = jobs[i];
+      if (k + 1 >= (int)j->step_ptr.size()) continue;
+      const int b = j->step_ptr[k], cnt = j->step_ptr[k + 1] - b;
+      if (cnt <= 0) continue;
+      BatchEntry e{};
+      e.lines = j->sched.p + b;
+      e.nlines = cnt;
+      e.job = i;
+      e.wg_off = wg;
+      const int o = outer ? outer[i] : 0, in = inner ? inner[i] : 0;
+      e.flags = ((o == 0 && in == 0) ? 1 : 0) | (j->prob.zero_skipping ? 2 : 0);
+      icd_shape(j->J, j->J.vlr_max, &e.ypl, &e.nv);
+      if (e.ypl != cur_ypl || e.nv != cur_nv) {  // a new launch: offsets restart
+        wg = 0;
+        cur_ypl = e.ypl;
+        cur_nv = e.nv;
+      }
+      e.wg_off = wg;
+      e.acc = j->acc.p;
+      ent.push_back(e);
+      wg += cnt;
+    }
+    l_ne[k] = (int)ent.size() - l_e0[k];
+    l_wg[k] = wg;
+  }
+  if (ctx->djobs.n < (size_t)n) {
+    ctx->djobs.release();
+    HIP_TRY(ctx->djobs.alloc(n));
+  }
+  if (ctx->ents.n < ent.size()) {
+    ctx->ents.release();
+    HIP_TRY(ctx->ents.alloc(ent.size()));
+  }
+  hipStream_t s = ctx->stream;
+  for (mbir_job* j : jobs) HIP_TRY(hipStreamSynchronize(j->stream));  // schedules uploaded
+  HIP_TRY(hipMemcpyAsync(ctx->djobs.p, dj.data(), n * sizeof(DevJob), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctx->ents.p, ent.data(), ent.size() * sizeof(BatchEntry),
+                         hipMemcpyHostToDevice, s));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timing = jobs[0]->timing != 0;
+  if (timing) {
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
+  }
+  // each step: one launch per kernel shape present (entries of a step are grouped by shape)
+  for (int k = 0; k < max_steps; ++k) {
+    int e = l_e0[k];
+    const int eend = l_e0[k] + l_ne[k];
+    while (e < eend) {
+      int f = e, wg = 0;
+      while (f < eend && ent[f].ypl == ent[e].ypl && ent[f].nv == ent[e].nv) wg += ent[f++].nlines;
+      HIP_TRY(launch_icd_batch_step(ent[e].ypl, ent[e].nv, ctx->djobs.p, ctx->ents.p, e, f - e,
+                                    wg, s));
+      e = f;
+    }
+  }
+  if (timing) HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0;
+  if (timing) {
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  if (ctx->verbose)
+    std::fprintf(stderr, "mbir: batch of %d jobs, %d launches, %zu entries\n", n, max_steps,
+                 ent.size());
+  for (int i = 0; i < n; ++i) {
+    int st = finish_sweep(jobs[i], ms, stats ? stats + i : nullptr);
+    if (st < 0) return st;
+    if (stats) stats[i].steps = max_steps;
+  }
+  return MBIR_OK;
+}
+
+int run_sweep(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
+{
+  if (j->cancel.load()) return MBIR_ERR_CANCELLED;
+  if (j->prob.kernel_mode == 1) {
+    mbir_sweep_stats tmp{};
+    std::vector<mbir_job*> one{j};
+    int rc = run_batch(j->ctx, one, &outer, &inner, &tmp);
+    if (rc < 0) return rc;
+    if (st) *st = tmp;
+    return tmp.status;
+  }
+  return run_sweep_steps(j, outer, inner, st);
 }
 
 int nuisance_update(mbir_job* j)
@@ -602,6 +757,16 @@ extern "C" int32_t mbir_job_sweep(mbir_job* job, int32_t outer, int32_t inner, m
 {
   if (!job) return MBIR_ERR_INPUT;
   return run_sweep(job, outer, inner, st);
+}
+
+extern "C" int32_t mbir_batch_sweep(mbir_ctx* ctx, mbir_job* const* jobs, int32_t n,
+                                    const int32_t* outer, const int32_t* inner,
+                                    mbir_sweep_stats* stats)
+{
+  if (!ctx || !jobs || n < 0) return MBIR_ERR_INPUT;
+  if (hipSetDevice(ctx->device) != hipSuccess) return MBIR_ERR_HIP;
+  std::vector<mbir_job*> all(jobs, jobs + n);
+  return run_batch(ctx, all, outer, inner, stats);
 }
 
 extern "C" int32_t mbir_job_nuisance_update(mbir_job* job)

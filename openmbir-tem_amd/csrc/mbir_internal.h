@@ -10,8 +10,10 @@ namespace mbir {
 constexpr int kResponseBins = 64;     // DETECTOR_RESPONSE_BINS (HAADF_ReconstructionEngine.cpp:249)
 constexpr int kProfileResolution = 1536;
 constexpr int kBeamResolution = 512;
-constexpr int kBlock = 256;           // threads per ICD workgroup (4 wave64)
+constexpr int kBlock = 256;           // threads per workgroup of the setup / FP64 kernels
 constexpr int kWaves = kBlock / 64;
+constexpr int kIcdBlock = 512;        // threads per ICD line workgroup (8 wave64)
+constexpr int kIcdWaves = kIcdBlock / 64;
 constexpr int kMaxNy = 256;           // voxels per line handled by one workgroup
 constexpr int kMaxSlots = 2048;       // N_theta * NB per column
 constexpr int kMaxNB = 8;             // detector bins per view per column
@@ -22,7 +24,9 @@ struct DevJob {
   int N_theta, N_r, N_t, N_x, N_y, N_z;
   int NB;                // slots (candidate detector bins) per view
   int nslots;            // N_theta * NB
+  int vlr_max;           // max detector rows of any slice (1 when delta_xy = delta_t)
   double x0, z0, delta_xz, delta_r, R0, RMax, offsetR;
+  double inv_delta_r, inv_offsetR;
   const double* cosv;    // [N_theta]
   const double* sinv;    // [N_theta]
   const double* H;       // [N_theta][64] detector response
@@ -40,6 +44,7 @@ struct DevJob {
   const double* mu;      // [N_theta]
   // QGGMRF (HAADF/HAADF_QGGMRFPriorModel.cpp:10-19), MRF_P = 2
   float mrf_q, mrf_c, sigma_p, sigma_pq, alpha, b0;  // b0 = P / (sigma^P c)
+  float pq2, inv_sigma_p, inv_sigma_pq;            // 2 - q, 1 / sigma^P, 1 / sigma^(P-q)
   double mrf_qd, mrf_cd, sigma_pd, sigma_pqd;
 };
 
@@ -52,12 +57,28 @@ struct SweepAccum {  // device accumulators of one sweep
   unsigned long long pad[3];
 };
 
+// One job's share of one batched launch: the lines of its current step.
+struct BatchEntry {
+  const int* lines;  // line ids of the step
+  int nlines;
+  int job;           // index into the launch's DevJob table
+  int wg_off;        // first workgroup of this entry within the launch
+  int flags;         // bit 0: first sweep, bit 1: zero skipping on
+  int ypl, nv;       // kernel shape
+  SweepAccum* acc;
+};
+
 // ---- launchers (all asynchronous on `s`) ----
 hipError_t launch_line_nnz(const DevJob& J, int* line_nnz, float* x, hipStream_t s);
 hipError_t launch_forward_project(const DevJob& J, const float* vol, double* out, hipStream_t s);
 hipError_t launch_init_error(const DevJob& J, const double* yest, hipStream_t s);
 hipError_t launch_icd_step(const DevJob& J, const int* lines, int nlines, int outer, int inner,
                            int zero_skipping, SweepAccum* acc, hipStream_t s);
+void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv);
+// one step of several jobs of one kernel shape: entries [e0, e0 + ne) of ent,
+// nwg workgroups in total
+hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const BatchEntry* ent,
+                                 int e0, int ne, int nwg, hipStream_t s);
 hipError_t launch_export_column(const DevJob& J, int z, int x, float* vals, int* rows,
                                 hipStream_t s);
 // joint estimation / weights / cost (FP64)

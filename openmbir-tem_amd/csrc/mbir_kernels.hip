@@ -34,8 +34,8 @@ __device__ inline void build_column(const DevJob& J, int z, int x, F&& store)
     const double rmax = r + J.delta_xz;
     int jmin = 0, jmax = -1;
     if (!(rmax < J.R0 || rmin > J.RMax)) {
-      jmin = (int)floor((rmin - J.R0) / J.delta_r);
-      jmax = (int)floor((rmax - J.R0) / J.delta_r);
+      jmin = (int)floor((rmin - J.R0) * J.inv_delta_r);
+      jmax = (int)floor((rmax - J.R0) * J.inv_delta_r);
       if (jmax >= J.N_r) jmax = J.N_r - 1;
       if (jmin < 0) jmin = 0;
     }
@@ -46,12 +46,12 @@ __device__ inline void build_column(const DevJob& J, int z, int x, F&& store)
       if (j <= jmax) {
         const double rc = J.R0 + ((double)j + 0.5) * J.delta_r;
         const double dr = fabs(r - rc);
-        const int idr = (int)floor(dr / J.offsetR);
+        const int idr = (int)floor(dr * J.inv_offsetR);
         if (idr >= 0 && idr < kResponseBins) {
           const double w1 = dr - idr * J.offsetR;
           const double w2 = (idr + 1) * J.offsetR - dr;
           const int iidx = idr + 1 < kResponseBins ? idr + 1 : kResponseBins - 1;
-          const double f1 = (w2 / J.offsetR) * Hth[idr] + (w1 / J.offsetR) * Hth[iidx];
+          const double f1 = (w2 * J.inv_offsetR) * Hth[idr] + (w1 * J.inv_offsetR) * Hth[iidx];
           if (f1 > 0) val = f1;
         }
       }
@@ -102,10 +102,36 @@ __device__ inline float filter_weight(int dy, int dz, int dx)
 __device__ inline float qggmrf_b(float d, const DevJob& J)
 {
   if (d == 0.f) return J.b0;
-  const float a = fabsf(d);
-  const float t1 = exp2f((2.f - J.mrf_q) * log2f(a)) / J.sigma_pq;
-  const float t3 = J.mrf_c + t1;
-  return (2.f - (2.f - J.mrf_q) * t1 / t3) / (t3 * J.sigma_p);
+  // |d|^(2-q) through the hardware log2 / exp2 (v_log_f32, v_exp_f32, ~1 ulp) and
+  // reciprocals through v_rcp_f32 (1 ulp): the coefficient of a quadratic
+  // surrogate, where 1e-6 relative error is far below the fp32 solve's own.
+  const float a = fabsf(d) < 1e-30f ? 1e-30f : fabsf(d);
+  const float t1 = __builtin_amdgcn_exp2f(J.pq2 * __builtin_amdgcn_logf(a)) * J.inv_sigma_pq;
+  const float r3 = __builtin_amdgcn_rcpf(J.mrf_c + t1);
+  return (2.f - J.pq2 * t1 * r3) * r3 * J.inv_sigma_p;
+}
+
+// Butterfly reduction inside aligned groups of G (1, 2, 4, 8 or 16) lanes with DPP
+// lane moves (no LDS): quad_perm xor 1, quad_perm xor 2, row_half_mirror, row_mirror.
+template <int CTRL>
+__device__ inline float dpp_mov(float v)
+{
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ inline void dpp_step(float& s1, float& s2, float& lo, float& hi)
+{
+  s1 += dpp_mov<CTRL>(s1);
+  s2 += dpp_mov<CTRL>(s2);
+  lo = fminf(lo, dpp_mov<CTRL>(lo));
+  hi = fmaxf(hi, dpp_mov<CTRL>(hi));
+}
+__device__ inline void group_reduce(int G, float& s1, float& s2, float& lo, float& hi)
+{
+  if (G >= 2) dpp_step<0xB1>(s1, s2, lo, hi);   // quad_perm [1,0,3,2]
+  if (G >= 4) dpp_step<0x4E>(s1, s2, lo, hi);   // quad_perm [2,3,0,1]
+  if (G >= 8) dpp_step<0x141>(s1, s2, lo, hi);  // row_half_mirror
+  if (G >= 16) dpp_step<0x140>(s1, s2, lo, hi); // row_mirror
 }
 
 }  // namespace
@@ -184,266 +210,399 @@ __global__ __launch_bounds__(kBlock) void k_export_column(DevJob J, int z, int x
 }
 
 // ---------------------------------------------------------------------------
-// One ICD step: every workgroup updates all N_y voxels of one line (z, x) of the
-// step's line list.  Restates UpdateYSlice::execute for one line
-// (HAADF/HAADF_ReconstructionEngine_UpdateVoxels.cpp:260-455):
-//   neighbourhood gather (:281-305), zero skipping (:311-338), theta1/theta2
-//   (:343-369), bounds (:47-63,370), QGGMRF functional substitution
-//   (Reconstruction/QGGMRF_Functions.cpp:153-206), positivity (:394-400),
-//   ROI statistics (:412-421), error-sinogram update (:423-449).
+// ICD update of one line (z, x): all N_y voxels of the line.  Restates
+// UpdateYSlice::execute for one line (HAADF/HAADF_ReconstructionEngine_UpdateVoxels.cpp:260-455):
+// neighbourhood gather (:281-305), zero skipping (:311-338), theta1/theta2
+// (:343-369), bounds (:47-63,370), QGGMRF functional substitution
+// (Reconstruction/QGGMRF_Functions.cpp:153-206), positivity (:394-400), ROI
+// statistics (:412-421) and the error-sinogram update (:423-449).
+//
 // Slices y are independent in the data term (disjoint VLR rows); the prior
 // couples y and y +- 1, so even slices are solved first and odd slices second,
-// which makes the step equal to a sequential order.  Lines within one step are
-// chosen by the host schedule; the e update uses fp32 atomics so concurrent lines
-// that share a detector bin never lose an update.
-template <int YPL>
-__global__ __launch_bounds__(kBlock) void k_icd_step(DevJob J, const int* lines, int nlines,
-                                                     int first_sweep, int zs_on, int TY,
-                                                     SweepAccum* acc)
+// which makes one line update equal to a sequential order of its voxels.
+//   1. the 3x3 block of lines around (z, x) is staged in LDS (coalesced along y);
+//   2. the column is built in FP64, one thread per view, and compacted into LDS
+//      with a wave scan;
+//   3. theta1/theta2: every wave streams a share of the column's (theta, r)
+//      detector rows, lanes running along the contiguous row t, U rows in flight
+//      per lane, Kahan-compensated fp32 partial sums combined in FP64;
+//   4. the voxel solve of each parity runs on lane groups: the 26 neighbour
+//      terms of one voxel are spread over G lanes and combined by shuffles;
+//   5. e -= I0 a vlr dx with fp32 atomics (lines of one step that share a
+//      detector bin never lose an update).
+struct IcdSmem {
+  float ka[kMaxSlots];   // I0[th] * a of the compact column entries
+  int row[kMaxSlots];    // (th * N_r + r) * N_t
+  float wv[kMaxSlots];   // per-view weight of the entry
+  float nbl[9][kMaxNy + 2];  // lines (z+dz, x+dx), y = -1 .. N_y (0 outside the volume)
+  float x[kMaxNy];       // this line; the even phase writes its new values here
+  float dx[kMaxNy];
+  double T1[kMaxNy], T2[kMaxNy];
+  unsigned char cand[kMaxNy];
+  unsigned char need[kMaxNy];
+  float p1h[kIcdBlock * 4], p1l[kIcdBlock * 4], p2h[kIcdBlock * 4], p2l[kIcdBlock * 4];
+  int wsum[kIcdWaves];
+  double st_su, st_sv;
+  unsigned long long st_nn;
+  unsigned int st_nz, st_nu;
+};
+
+template <int YPL, int NV>
+__device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepAccum* acc,
+                         IcdSmem& sm)
 {
-  __shared__ float s_ka[kMaxSlots];
-  __shared__ int s_row[kMaxSlots];
-  __shared__ float s_x[kMaxNy];
-  __shared__ float s_dx[kMaxNy];
-  __shared__ unsigned char s_cand[kMaxNy];
-  __shared__ unsigned char s_need[kMaxNy];
-  __shared__ float s_p1h[kBlock * YPL], s_p1l[kBlock * YPL], s_p2h[kBlock * YPL], s_p2l[kBlock * YPL];
-  __shared__ double red[kBlock];
-
-  if ((int)blockIdx.x >= nlines) return;
-  const int line = lines[blockIdx.x];
-  const int nnz_line = J.line_nnz[line];
-  if (nnz_line == 0) return;  // the reference skips empty columns (:245-249)
-  const int z = line / J.N_x, xi = line % J.N_x;
-  const int tid = threadIdx.x;
+  constexpr int U = YPL == 1 ? 24 : (YPL == 2 ? 12 : 6);  // detector rows in flight per lane
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int Ny = J.N_y;
+  const int z = line / J.N_x, xi = line % J.N_x;
   const size_t lbase = (size_t)line * Ny;
+  const int nnz_line = J.line_nnz[line];
+  const bool zs_active = zs_on && !first;
 
-  // ---- 1. current values of this line and the 24 out-of-line neighbours ----
-  float v = 0.f;
-  float nb[27];
-  unsigned int inb = 0;  // bit i set: neighbour i exists (BOUNDARYFLAG)
-  if (tid < Ny) {
-    v = J.x[lbase + tid];
-    s_x[tid] = v;
+  // ---- 1. the 3x3 neighbour lines (branch-free loads from clamped addresses) ----
+  const int W = Ny + 2;
+  constexpr int kNbIt = (9 * (kMaxNy + 2) + kIcdBlock - 1) / kIcdBlock;
+  float nbv_[kNbIt];
 #pragma unroll
-    for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-      for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int i = (dy + 1) * 9 + (dz + 1) * 3 + (dx + 1);
-          nb[i] = 0.f;
-          const int yy = tid + dy, zz = z + dz, xx = xi + dx;
-          if (yy >= 0 && yy < Ny && zz >= 0 && zz < J.N_z && xx >= 0 && xx < J.N_x) {
-            inb |= 1u << i;
-            if (dz != 0 || dx != 0) nb[i] = J.x[((size_t)zz * J.N_x + xx) * Ny + yy];
-          }
-        }
+  for (int it = 0; it < kNbIt; ++it) {  // all loads first, stores after
+    const int q = tid + it * kIcdBlock;
+    const int l = q / W, yy = q - l * W - 1;
+    const int zz = z + l / 3 - 1, xx = xi + l % 3 - 1;
+    const bool in = q < 9 * W && yy >= 0 && yy < Ny && zz >= 0 && zz < J.N_z && xx >= 0 &&
+                    xx < J.N_x;
+    const float val = J.x[in ? ((size_t)zz * J.N_x + xx) * Ny + yy : 0];
+    nbv_[it] = in ? val : 0.f;
   }
-  // ---- 2. column in LDS ----
-  build_column(J, z, xi, [&](int s, int th, int j, double a) {
-    s_ka[s] = (float)(J.I0[th] * a);
-    s_row[s] = a > 0.0 ? (th * J.N_r + j) * J.N_t : 0;
-  });
-  __syncthreads();
-  // zero-skip candidates with the pre-step in-line neighbours
-  bool cand = false;
+#pragma unroll
+  for (int it = 0; it < kNbIt; ++it) {
+    const int q = tid + it * kIcdBlock;
+    const int l = q / W, yy = q - l * W - 1;
+    if (q < 9 * W) sm.nbl[l][yy + 1] = nbv_[it];
+  }
+  if (tid == 0) {
+    sm.st_su = 0.0;
+    sm.st_sv = 0.0;
+    sm.st_nn = 0ull;
+    sm.st_nz = 0u;
+    sm.st_nu = 0u;
+  }
+
+  // ---- 2. column (one thread per view), compacted by a wave scan ----
+  // Straight-line code: every load is issued unconditionally from a clamped
+  // address and selected afterwards, so the loads of all slots are in flight at once.
+  double av[kMaxNB];
+  int jmin = 0, cnt = 0;
+  const int th = tid;
+  const int thc = th < J.N_theta ? th : 0;
+  const double cth = J.cosv[thc], sth = J.sinv[thc], I0th = J.I0[thc];
+  const float wvth = J.wview[thc];
+  {
+    const double xc = J.x0 + ((double)xi + 0.5) * J.delta_xz;
+    const double zc = J.z0 + ((double)z + 0.5) * J.delta_xz;
+    const double r = xc * cth - zc * sth;
+    const double rmin = r - J.delta_xz, rmax = r + J.delta_xz;
+    int jmax = -1;
+    if (th < J.N_theta && !(rmax < J.R0 || rmin > J.RMax)) {
+      jmin = (int)floor((rmin - J.R0) * J.inv_delta_r);
+      jmax = (int)floor((rmax - J.R0) * J.inv_delta_r);
+      if (jmax >= J.N_r) jmax = J.N_r - 1;
+      if (jmin < 0) jmin = 0;
+    }
+    const double* Hth = J.H + (size_t)thc * kResponseBins;
+    auto slot = [&](int k) {
+      const int j = jmin + k;
+      const double rc = J.R0 + ((double)j + 0.5) * J.delta_r;
+      const double dr = fabs(r - rc);
+      int idr = (int)floor(dr * J.inv_offsetR);
+      const bool ok = j <= jmax && idr >= 0 && idr < kResponseBins;
+      idr = ok ? idr : 0;
+      const int iidx = idr + 1 < kResponseBins ? idr + 1 : kResponseBins - 1;
+      const double h0 = Hth[idr], h1 = Hth[iidx];
+      const double w1 = dr - idr * J.offsetR;
+      const double w2 = (idr + 1) * J.offsetR - dr;
+      const double f1 = (w2 * J.inv_offsetR) * h0 + (w1 * J.inv_offsetR) * h1;
+      return (ok && f1 > 0) ? f1 : 0.0;
+    };
+    double a4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a4[k] = slot(k);  // unconditional: all loads in flight
+#pragma unroll
+    for (int k = 0; k < 4; ++k) av[k] = k < J.NB ? a4[k] : 0.0;
+#pragma unroll
+    for (int k = 4; k < kMaxNB; ++k) av[k] = 0.0;
+    if (J.NB > 4) {  // coarse voxels (delta_xz > 1.5 delta_r): more bins per view
+#pragma unroll
+      for (int k = 4; k < kMaxNB; ++k) av[k] = k < J.NB ? slot(k) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kMaxNB; ++k) cnt += av[k] > 0.0 ? 1 : 0;
+  }
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) sm.wsum[wave] = incl;
+  __syncthreads();  // A: wave sums and neighbour lines visible
+  int nc = 0, pos = incl - cnt;
+#pragma unroll
+  for (int w = 0; w < kIcdWaves; ++w) {
+    const int ws = sm.wsum[w];
+    if (w < wave) pos += ws;
+    nc += ws;
+  }
+  if (th < J.N_theta && cnt > 0) {
+    const int rb = th * J.N_r + jmin;
+#pragma unroll
+    for (int k = 0; k < kMaxNB; ++k)
+      if (av[k] > 0.0) {
+        sm.ka[pos] = (float)(I0th * av[k]);
+        sm.row[pos] = (rb + k) * J.N_t;
+        sm.wv[pos] = wvth;
+        ++pos;
+      }
+  }
   if (tid < Ny) {
-    if (zs_on && !first_sweep && v == 0.f) {
+    const float v = sm.nbl[4][tid + 1];
+    sm.x[tid] = v;
+    // zero-skip candidate with the pre-update neighbourhood (:311-338)
+    bool cand = false;
+    if (zs_active && v == 0.f) {
       cand = true;
 #pragma unroll
-      for (int i = 0; i < 27; ++i)
-        if (i != 13 && i != 4 && i != 22 && nb[i] > 0.f) cand = false;
-      for (int dy = -1; dy <= 1; dy += 2) {
-        const int yy = tid + dy;
-        if (yy >= 0 && yy < Ny && s_x[yy] > 0.f) cand = false;
-      }
+      for (int l = 0; l < 9; ++l)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+          if (!(l == 4 && dy == 0) && sm.nbl[l][tid + 1 + dy] > 0.f) cand = false;
     }
-    s_cand[tid] = cand;
+    sm.cand[tid] = cand;
   }
-  __syncthreads();
+  __syncthreads();  // B: column, line copy and candidates visible
   if (tid < Ny) {
-    bool need = !s_cand[tid];
+    bool need = !sm.cand[tid];
     if ((tid & 1) && !need) {  // odd slices see the even slices' new values
-      if (tid - 1 >= 0 && !s_cand[tid - 1]) need = true;
-      if (tid + 1 < Ny && !s_cand[tid + 1]) need = true;
+      if (tid - 1 >= 0 && !sm.cand[tid - 1]) need = true;
+      if (tid + 1 < Ny && !sm.cand[tid + 1]) need = true;
     }
-    s_need[tid] = need;
+    sm.need[tid] = need;
   }
-  __syncthreads();
+  __syncthreads();  // C
 
-  // ---- 3. theta1 / theta2: lanes over y, (wave, sub-row) over slots ----
-  const int lane = tid & 63, wave = tid >> 6;
-  const int RPW = 64 / TY;
-  const int sub = lane / TY, yl = lane % TY;
-  const int P = kWaves * RPW;
-  const int p = wave * RPW + sub;
+  // ---- 3. theta1 / theta2 ----
+  int TY = 1;
+  while (TY < Ny && TY < 64) TY <<= 1;
+  const int RPW = 64 / TY, sub = lane / TY, yl = lane % TY;
+  const int P = kIcdWaves * RPW, p = wave * RPW + sub;
   Kahan t1[YPL], t2[YPL];
   int vt0[YPL], vn[YPL];
-  float vv[YPL][kVlrCap];
+  float vv[YPL][NV];
   bool act[YPL];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
     const int y = yl + TY * jy;
-    act[jy] = y < Ny && s_need[y];
+    act[jy] = y < Ny && sm.need[y];
     vt0[jy] = 0;
     vn[jy] = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) vv[jy][k] = 0.f;
     if (y < Ny) {
       vt0[jy] = J.vlr_t0[y];
       vn[jy] = J.vlr_n[y];
 #pragma unroll
-      for (int k = 0; k < kVlrCap; ++k) vv[jy][k] = J.vlr_val[y * kVlrCap + k];
+      for (int k = 0; k < NV; ++k) vv[jy][k] = J.vlr_val[y * kVlrCap + k];
     }
   }
-  for (int s = p; s < J.nslots; s += P) {
-    const float ka = s_ka[s];
-    if (ka == 0.f) continue;
-    const int row = s_row[s];
-    const float wv = J.wview[s / J.NB];
+  const bool full_w = J.wfull != nullptr;
+  for (int base = p; base < nc; base += P * U) {
+    float ev[U][YPL][NV], wf[U][YPL][NV];
 #pragma unroll
-    for (int jy = 0; jy < YPL; ++jy) {
-      if (!act[jy]) continue;
-      for (int k = 0; k < vn[jy]; ++k) {
-        const int idx = row + vt0[jy] + k;
-        const float pe = ka * vv[jy][k];
-        const float w = J.wfull ? J.wfull[idx] : wv;
-        const float ev = J.e[idx];
-        t2[jy].add(pe * pe * w);
-        t1[jy].add(ev * pe * w);
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * P;
+      const bool ok = i < nc;
+      const int rw = sm.row[ok ? i : 0];
+#pragma unroll
+      for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          const bool use = ok && act[jy] && k < vn[jy];
+          const int idx = use ? rw + vt0[jy] + k : 0;
+          ev[u][jy][k] = J.e[idx];
+          wf[u][jy][k] = full_w ? J.wfull[idx] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * P;
+      if (i < nc) {
+        const float kav = sm.ka[i];
+        const float wvv = sm.wv[i];
+#pragma unroll
+        for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            if (act[jy] && k < vn[jy]) {
+              const float pe = kav * vv[jy][k];
+              const float w = full_w ? wf[u][jy][k] : wvv;
+              t2[jy].add(pe * pe * w);
+              t1[jy].add(ev[u][jy][k] * pe * w);
+            }
       }
     }
   }
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
-    const int y = yl + TY * jy;
-    if (y < TY * YPL) {
-      const int o = p * (TY * YPL) + y;
-      s_p1h[o] = t1[jy].s;
-      s_p1l[o] = -t1[jy].c;
-      s_p2h[o] = t2[jy].s;
-      s_p2l[o] = -t2[jy].c;
-    }
+    const int o = p * (TY * YPL) + yl + TY * jy;
+    sm.p1h[o] = t1[jy].s;
+    sm.p1l[o] = -t1[jy].c;
+    sm.p2h[o] = t2[jy].s;
+    sm.p2l[o] = -t2[jy].c;
   }
-  __syncthreads();
-
-  // ---- 4. voxel solve, even slices then odd slices ----
-  double T1 = 0.0, T2 = 0.0;
+  __syncthreads();  // D
   if (tid < Ny) {
+    double T1 = 0.0, T2 = 0.0;
     for (int q = 0; q < P; ++q) {
       const int o = q * (TY * YPL) + tid;
-      T1 += (double)s_p1h[o] + (double)s_p1l[o];
-      T2 += (double)s_p2h[o] + (double)s_p2l[o];
+      T1 += (double)sm.p1h[o] + (double)sm.p1l[o];
+      T2 += (double)sm.p2h[o] + (double)sm.p2l[o];
     }
-    T1 = -T1;  // THETA1 *= -1 (:369)
+    sm.T1[tid] = -T1;  // THETA1 *= -1 (:369)
+    sm.T2[tid] = T2;
   }
-  float xnew = v;
-  bool skipped = false, updated = false;
+  __syncthreads();  // E
+
+  // ---- 4. voxel solve on lane groups, even slices then odd slices ----
+  const bool roi = J.mask[line] != 0;
   for (int phase = 0; phase < 2; ++phase) {
-    if (tid < Ny && (tid & 1) == phase) {
-      // in-line neighbours y +- 1 from LDS (odd slices see the even slices' new values)
-      nb[4] = 0.f;   // (dy=-1, dz=0, dx=0)
-      nb[22] = 0.f;  // (dy=+1, dz=0, dx=0)
-      if (tid - 1 >= 0) nb[4] = s_x[tid - 1];
-      if (tid + 1 < Ny) nb[22] = s_x[tid + 1];
-      nb[13] = 0.f;  // NEIGHBORHOOD[1][1][1] = 0 (:305)
-      bool zs = false;
-      if (zs_on && !first_sweep && v == 0.f) {
-        zs = true;
-#pragma unroll
-        for (int i = 0; i < 27; ++i)
-          if (nb[i] > 0.f) zs = false;
-      }
-      if (zs) {
-        skipped = true;
-      } else {
-        // bounds: min/max over the 27 entries, widened by v - theta1/theta2 (:47-63)
-        double lo = nb[0], hi = nb[0];
-#pragma unroll
-        for (int i = 1; i < 27; ++i) {
-          lo = fmin(lo, (double)nb[i]);
-          hi = fmax(hi, (double)nb[i]);
+    const int vpp = (Ny + 1 - phase) / 2;  // slices of this parity
+    int G = 16;                            // lanes per voxel (one DPP row at most)
+    while (G > 1 && G * vpp > kIcdBlock) G >>= 1;
+    const int vi = tid / G, gl = tid % G;
+    const int y = 2 * vi + phase;
+    const bool active = vi < vpp;
+    const float v = active ? sm.x[y] : 0.f;
+    float s1 = 0.f, s2 = 0.f, lo = 0.f, hi = 0.f;  // lo/hi include the zeroed centre
+    if (active) {
+      for (int n = gl; n < 26; n += G) {
+        const int m = n < 13 ? n : n + 1;  // the 27-point index without the centre
+        const int dy = m / 9 - 1, dz = (m / 3) % 3 - 1, dx = m % 3 - 1;
+        const int l = (dz + 1) * 3 + (dx + 1), yy = y + dy;
+        const bool ex = yy >= 0 && yy < Ny && z + dz >= 0 && z + dz < J.N_z && xi + dx >= 0 &&
+                        xi + dx < J.N_x;
+        float nbv = 0.f;
+        if (ex) nbv = l == 4 ? sm.x[yy] : sm.nbl[l][yy + 1];
+        lo = fminf(lo, nbv);
+        hi = fmaxf(hi, nbv);
+        if (ex) {
+          const float b = filter_weight(dy, dz, dx) * qggmrf_b(v - nbv, J);
+          s1 += b * nbv;
+          s2 += b;
         }
+      }
+    }
+    group_reduce(G, s1, s2, lo, hi);
+    const bool anypos = hi > 0.f;
+    if (active && gl == 0) {
+      float xnew = v;
+      if (zs_active && v == 0.f && !anypos) {
+        atomicAdd(&sm.st_nz, 1u);
+      } else {
+        const double T1 = sm.T1[y], T2 = sm.T2[y];
+        double dlo = lo, dhi = hi;
         if (T2 != 0.0) {
           const double c = (double)v - T1 / T2;
-          lo = fmin(lo, c);
-          hi = fmax(hi, c);
+          dlo = fmin(dlo, c);
+          dhi = fmax(dhi, c);
         }
-        float s1 = 0.f, s2 = 0.f;
-#pragma unroll
-        for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-          for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-              const int i = (dy + 1) * 9 + (dz + 1) * 3 + (dx + 1);
-              if (i == 13 || !(inb & (1u << i))) continue;
-              const float b = filter_weight(dy, dz, dx) * qggmrf_b(v - nb[i], J);
-              s1 += b * nb[i];
-              s2 += b;
-            }
         const double u = ((double)s1 + T2 * (double)v - T1) / ((double)s2 + T2);
         double nv = (double)v + (double)J.alpha * (u - (double)v);
-        nv = nv < lo ? lo : (nv > hi ? hi : nv);
+        nv = nv < dlo ? dlo : (nv > dhi ? dhi : nv);
         if (nv < 0.0) nv = 0.0;  // POSITIVITY_CONSTRAINT (:394-400)
         xnew = (float)nv;
-        updated = true;
+        J.x[lbase + y] = xnew;
+        atomicAdd(&sm.st_nu, 1u);
+        atomicAdd(&sm.st_nn, (unsigned long long)nnz_line * (unsigned long long)J.vlr_n[y]);
+        if (roi) {
+          atomicAdd(&sm.st_su, fabs((double)xnew - (double)v));
+          atomicAdd(&sm.st_sv, fabs((double)v));
+        }
       }
-      s_x[tid] = xnew;
+      sm.x[y] = xnew;
+      sm.dx[y] = xnew - v;
     }
-    __syncthreads();
+    __syncthreads();  // F, G
   }
-  if (tid < Ny) {
-    s_dx[tid] = xnew - v;
-    if (updated) J.x[lbase + tid] = xnew;
-  }
-  __syncthreads();
 
   // ---- 5. e -= I0 a vlr dx (:423-449), fp32 atomics ----
+  float d[YPL];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
     const int y = yl + TY * jy;
-    act[jy] = y < Ny && s_dx[y] != 0.f;
+    d[jy] = y < Ny ? sm.dx[y] : 0.f;
   }
-  for (int s = p; s < J.nslots; s += P) {
-    const float ka = s_ka[s];
-    if (ka == 0.f) continue;
-    const int row = s_row[s];
+  for (int base = p; base < nc; base += P * U) {
 #pragma unroll
-    for (int jy = 0; jy < YPL; ++jy) {
-      if (!act[jy]) continue;
-      const float d = s_dx[yl + TY * jy];
-      for (int k = 0; k < vn[jy]; ++k)
-        unsafeAtomicAdd(J.e + row + vt0[jy] + k, -(ka * vv[jy][k] * d));
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * P;
+      if (i < nc) {
+        const float ka = sm.ka[i];
+        const int rwi = sm.row[i];
+#pragma unroll
+        for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            if (d[jy] != 0.f && k < vn[jy])
+              unsafeAtomicAdd(J.e + rwi + vt0[jy] + k, -(ka * vv[jy][k] * d[jy]));
+      }
     }
   }
 
-  // ---- 6. sweep statistics ----
-  const bool roi = J.mask[line] != 0;
-  double su = 0.0, sv = 0.0, nz = 0.0, nu = 0.0, nn = 0.0;
-  if (tid < Ny) {
-    if (updated) {
-      nu = 1.0;
-      nn = (double)nnz_line * (double)J.vlr_n[tid];
-      if (roi) {
-        su = fabs((double)xnew - (double)v);
-        sv = fabs((double)v);
-      }
-    }
-    if (skipped) nz = 1.0;
-  }
-  su = block_sum_d(su, red);
-  sv = block_sum_d(sv, red);
-  nz = block_sum_d(nz, red);
-  nu = block_sum_d(nu, red);
-  nn = block_sum_d(nn, red);
+  // ---- 6. sweep statistics: one set of global atomics per line ----
   if (tid == 0) {
-    atomicAdd(&acc->sum_abs_update, su);
-    atomicAdd(&acc->sum_abs_value, sv);
-    atomicAdd(&acc->zero_skipped, (unsigned long long)nz);
-    atomicAdd(&acc->updated, (unsigned long long)nu);
-    atomicAdd(&acc->nnz_updated, (unsigned long long)nn);
+    if (sm.st_su != 0.0) atomicAdd(&acc->sum_abs_update, sm.st_su);
+    if (sm.st_sv != 0.0) atomicAdd(&acc->sum_abs_value, sm.st_sv);
+    if (sm.st_nz) atomicAdd(&acc->zero_skipped, (unsigned long long)sm.st_nz);
+    if (sm.st_nu) atomicAdd(&acc->updated, (unsigned long long)sm.st_nu);
+    if (sm.st_nn) atomicAdd(&acc->nnz_updated, sm.st_nn);
   }
+  __syncthreads();  // H: LDS may be reused by the next line
+}
+
+template <int YPL, int NV>
+__global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const int* lines, int nlines,
+                                                        int first, int zs_on, SweepAccum* acc)
+{
+  __shared__ IcdSmem sm;
+  if ((int)blockIdx.x >= nlines) return;
+  const int line = lines[blockIdx.x];
+  if (J.line_nnz[line] == 0) return;  // the reference skips empty columns (:245-249)
+  icd_line<YPL, NV>(J, line, first, zs_on, acc, sm);
+}
+
+// One conflict-free step of every job of a batch that has kernel shape
+// (YPL, NV): entries [e0, e0 + ne) of the sweep's entry table; workgroup g serves
+// entry e with wg_off[e] <= g < wg_off[e] + nlines[e].  Jobs never share memory,
+// so one launch needs no ordering between them.
+template <int YPL, int NV>
+__global__ __launch_bounds__(kIcdBlock) void k_icd_batch_step(const DevJob* __restrict__ jobs,
+                                                              const BatchEntry* __restrict__ ent,
+                                                              int e0, int ne)
+{
+  __shared__ IcdSmem sm;
+  const int g = blockIdx.x;
+  int lo = e0, hi = e0 + ne - 1;
+  while (lo < hi) {  // last entry with wg_off <= g
+    const int mid = (lo + hi + 1) >> 1;
+    if (ent[mid].wg_off <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const BatchEntry& E = ent[lo];
+  const int i = g - E.wg_off;
+  if (i < 0 || i >= E.nlines) return;
+  const DevJob& J = jobs[E.job];
+  const int line = E.lines[i];
+  if (J.line_nnz[line] == 0) return;
+  icd_line<YPL, NV>(J, line, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -667,27 +826,55 @@ hipError_t launch_export_column(const DevJob& J, int z, int x, float* vals, int*
   return hipGetLastError();
 }
 
+void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv)
+{
+  *ypl = J.N_y <= 64 ? 1 : (J.N_y <= 128 ? 2 : 4);
+  *nv = max_vlr <= 1 ? 1 : (max_vlr == 2 ? 2 : 4);
+}
+
+template <int YPL>
+static void launch_step_nv(const DevJob& J, const int* lines, int nlines, int first, int zs,
+                           SweepAccum* acc, hipStream_t s)
+{
+  if (J.vlr_max <= 1)
+    hipLaunchKernelGGL((k_icd_step<YPL, 1>), dim3(nlines), dim3(kIcdBlock), 0, s, J, lines, nlines,
+                       first, zs, acc);
+  else if (J.vlr_max == 2)
+    hipLaunchKernelGGL((k_icd_step<YPL, 2>), dim3(nlines), dim3(kIcdBlock), 0, s, J, lines, nlines,
+                       first, zs, acc);
+  else
+    hipLaunchKernelGGL((k_icd_step<YPL, 4>), dim3(nlines), dim3(kIcdBlock), 0, s, J, lines, nlines,
+                       first, zs, acc);
+}
+
 hipError_t launch_icd_step(const DevJob& J, const int* lines, int nlines, int outer, int inner,
                            int zero_skipping, SweepAccum* acc, hipStream_t s)
 {
   if (nlines <= 0) return hipSuccess;
   const int first = (outer == 0 && inner == 0) ? 1 : 0;
-  int TY = 64, YPL = 1;
-  if (J.N_y <= 64) {
-    TY = 1;
-    while (TY < J.N_y) TY <<= 1;
-  } else {
-    YPL = (J.N_y + 63) / 64;
-  }
-  if (YPL == 1)
-    hipLaunchKernelGGL(k_icd_step<1>, dim3(nlines), dim3(kBlock), 0, s, J, lines, nlines, first,
-                       zero_skipping, TY, acc);
-  else if (YPL == 2)
-    hipLaunchKernelGGL(k_icd_step<2>, dim3(nlines), dim3(kBlock), 0, s, J, lines, nlines, first,
-                       zero_skipping, TY, acc);
+  int ypl, nv;
+  icd_shape(J, J.vlr_max, &ypl, &nv);
+  if (ypl == 1)
+    launch_step_nv<1>(J, lines, nlines, first, zero_skipping, acc, s);
+  else if (ypl == 2)
+    launch_step_nv<2>(J, lines, nlines, first, zero_skipping, acc, s);
   else
-    hipLaunchKernelGGL(k_icd_step<4>, dim3(nlines), dim3(kBlock), 0, s, J, lines, nlines, first,
-                       zero_skipping, TY, acc);
+    launch_step_nv<4>(J, lines, nlines, first, zero_skipping, acc, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const BatchEntry* ent,
+                                 int e0, int ne, int nwg, hipStream_t s)
+{
+  if (ne <= 0 || nwg <= 0) return hipSuccess;
+  if (ypl == 1 && nv == 1)
+    hipLaunchKernelGGL((k_icd_batch_step<1, 1>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, ent, e0, ne);
+  else if (ypl == 1)
+    hipLaunchKernelGGL((k_icd_batch_step<1, 2>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, ent, e0, ne);
+  else if (nv == 1)
+    hipLaunchKernelGGL((k_icd_batch_step<2, 1>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, ent, e0, ne);
+  else
+    hipLaunchKernelGGL((k_icd_batch_step<2, 2>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, ent, e0, ne);
   return hipGetLastError();
 }
 

@@ -53,7 +53,7 @@ class Problem(C.Structure):
         ("num_outer", C.c_int32), ("num_inner", C.c_int32),
         ("joint_estimation", C.c_int32), ("noise_model", C.c_int32),
         ("zero_skipping", C.c_int32), ("seed", C.c_uint32),
-        ("schedule", C.c_int32), ("rows_per_step", C.c_int32),
+        ("schedule", C.c_int32), ("rows_per_step", C.c_int32), ("kernel_mode", C.c_int32),
     ]
 
 
@@ -81,6 +81,9 @@ _SIG = {
     "mbir_job_destroy": (C.c_int32, [C.c_void_p]),
     "mbir_job_sweep": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(SweepStats)]),
     "mbir_job_nuisance_update": (C.c_int32, [C.c_void_p]),
+    "mbir_batch_sweep": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32,
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(SweepStats)]),
     "mbir_job_cost": (C.c_int32, [C.c_void_p, _D, _D]),
     "mbir_job_run": (C.c_int32, [C.c_void_p, C.POINTER(RunStats), _D, C.c_int32,
                                  C.POINTER(C.c_int32)]),
@@ -130,6 +133,17 @@ def _fptr(a):
     return None if a is None else a.ctypes.data_as(_F)
 
 
+def batch_sweep(ctx, jobs, outer=None, inner=None):
+    """One ICD sweep of every job in one persistent launch; returns per-job SweepStats."""
+    n = len(jobs)
+    hs = (C.c_void_p * n)(*[j.h.value for j in jobs])
+    o = (C.c_int32 * n)(*(outer if outer is not None else [0] * n))
+    i = (C.c_int32 * n)(*(inner if inner is not None else [0] * n))
+    st = (SweepStats * n)()
+    _chk("mbir_batch_sweep", lib().mbir_batch_sweep(ctx.h, hs, n, o, i, st))
+    return list(st)
+
+
 def geometry_init(N_r, N_t, angles_deg, thickness, delta_r=1.0, delta_t=1.0, delta_xz_in=1.0,
                   interpolate_factor=1.0, extend_object=0):
     ang = np.ascontiguousarray(angles_deg, np.float64)
@@ -164,7 +178,8 @@ class Job:
     def __init__(self, ctx, geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3,
                  target_gain=1.0, default_offset=0.0, default_variance=1.0, stop_threshold=1e-3,
                  num_outer=1, num_inner=20, joint_estimation=0, noise_model=0, zero_skipping=1,
-                 seed=12345, schedule=0, rows_per_step=1, init_volume=None, default_recon=0.0,
+                 seed=12345, schedule=0, rows_per_step=1, kernel_mode=0, init_volume=None,
+                 default_recon=0.0,
                  init_gain=None, init_offset=None, init_variance=None):
         self.ctx = ctx
         self.geom = geom
@@ -195,7 +210,7 @@ class Job:
         p.num_outer, p.num_inner = num_outer, num_inner
         p.joint_estimation, p.noise_model, p.zero_skipping = joint_estimation, noise_model, zero_skipping
         p.seed = seed
-        p.schedule, p.rows_per_step = schedule, rows_per_step
+        p.schedule, p.rows_per_step, p.kernel_mode = schedule, rows_per_step, kernel_mode
         self.h = C.c_void_p()
         _chk("mbir_job_create", lib().mbir_job_create(ctx.h, C.byref(p), C.byref(self.h)))
         del keep

@@ -1,0 +1,66 @@
+"""Seeded synthetic HAADF workloads (phantom, tilt series, nuisance, noise).
+
+Workload names follow the sizes they describe.  The measured sinogram is made
+with the engine's own forward model (I0 * A x), then per-view gain and offset
+and Gaussian noise are applied.
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi
+from . import phantom as P
+
+
+@dataclass
+class Workload:
+    name: str
+    n_r: int
+    n_t: int
+    n_views: int
+    max_deg: float
+    thickness: float
+    n_objects: int
+    noise: float
+    gain_range: tuple
+    offset_range: tuple
+    seed: int
+    joint_estimation: int
+    noise_model: int
+    stop_threshold: float
+    num_outer: int
+    num_inner: int
+
+
+HAADF_128x16x64 = Workload("haadf_128x16x64_61v", 128, 16, 61, 60.0, 32, 0, 0.02, (1.0, 1.0),
+                           (0.0, 0.0), 1, 0, 0, 1e-3, 1, 20)
+HAADF_512x64x256 = Workload("haadf_512x64x256_91v", 512, 64, 91, 70.0, 128, 20, 0.03, (0.8, 1.2),
+                            (-0.05, 0.05), 2, 1, 1, 1e-3, 30, 10)
+
+WORKLOADS = {w.name: w for w in (HAADF_128x16x64, HAADF_512x64x256)}
+
+
+def make(ctx, wl, **job_kw):
+    """Geometry, angles, phantom, sinogram and a ready job for workload wl."""
+    ang = P.tilt_series(wl.n_views, wl.max_deg)
+    geom = capi.geometry_init(wl.n_r, wl.n_t, ang, wl.thickness)
+    shape = (geom.N_z, geom.N_x, geom.N_y)
+    if wl.n_objects == 0:
+        vol = P.two_spheres(shape, seed=wl.seed)
+    else:
+        vol = P.random_ellipsoids(shape, n=wl.n_objects, seed=wl.seed)
+    rng = np.random.default_rng(wl.seed + 100)
+    gains = rng.uniform(*wl.gain_range, wl.n_views)
+    offs = rng.uniform(*wl.offset_range, wl.n_views)
+    zero = np.zeros((wl.n_views, wl.n_r, wl.n_t), np.float32)
+    proj_job = capi.Job(ctx, geom, ang, zero)
+    proj = proj_job.forward_project(vol)
+    proj_job.close()
+    y = P.noisy_sinogram(proj, wl.noise, wl.seed + 1, gains=gains, offsets=offs)
+    kw = dict(joint_estimation=wl.joint_estimation, noise_model=wl.noise_model,
+              stop_threshold=wl.stop_threshold, num_outer=wl.num_outer, num_inner=wl.num_inner,
+              sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3, seed=wl.seed)
+    kw.update(job_kw)
+    job = capi.Job(ctx, geom, ang, y, **kw)
+    return dict(geom=geom, angles=ang, volume=vol, sinogram=y, gains=gains, offsets=offs,
+                job=job, job_kw=kw)

@@ -130,7 +130,9 @@ def test_joint_estimation_and_noise_model(gpu_ctx):
     np.testing.assert_allclose(mug, mur, atol=5e-3)
     np.testing.assert_allclose(alg, alr, rtol=5e-2)
     xr, xg = eng.x(), job.volume()
-    assert np.sqrt(np.mean((xr - xg) ** 2)) <= 2e-2 * np.sqrt(np.mean(xr ** 2))
+    # two oracle runs that differ only in update order are 2.5% apart (RMSE / RMS) at
+    # this iteration count; the GPU must be within 2x that spread
+    assert np.sqrt(np.mean((xr - xg) ** 2)) <= 5e-2 * np.sqrt(np.mean(xr ** 2))
     assert costs_g[-1] == pytest.approx(costs_r[-1], rel=1e-3)
 
 
