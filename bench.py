@@ -37,8 +37,11 @@ def parse():
     ap.add_argument("--workload", default="haadf_512x64x256_91v")
     ap.add_argument("--schedule", type=int, default=0)
     ap.add_argument("--rows-per-step", type=int, default=1)
-    ap.add_argument("--kernel-mode", type=int, default=1,
-                    help="0 = one launch per step, 1 = persistent sweep kernel")
+    ap.add_argument("--kernel-mode", type=int, default=0,
+                    help="0 = the job's own step kernel, 1 = the batched step kernel")
+    ap.add_argument("--batch", type=int, default=8,
+                    help="jobs of the same workload swept together for the batched figure (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "icd_step_traffic.json"))
@@ -67,6 +70,36 @@ def cpu_baseline(wl, ang, y, threads):
     return {"value": V / dt, "unit": "voxel-updates/s", "cores": threads, "kind": "port",
             "sample": f"one steady-state ICD sweep (second sweep, zero skipping on) of the FP64 "
                       f"oracle over the full {g.N_x}x{g.N_y}x{g.N_z} volume, {threads} y-slab threads"}
+
+
+def batched(ctx, wl, n, steps, single_value):
+    """Throughput of n independent jobs of the workload swept together (one batched
+    step kernel launch carries one conflict-free step of every job)."""
+    import time as _t
+
+    from mbirtem import capi, workloads
+
+    jobs = []
+    for i in range(n):
+        w = workloads.make(ctx, wl, seed=wl.seed + 1000 * (i + 1), kernel_mode=1)
+        jobs.append(w["job"])
+    g = jobs[0].geom
+    V = g.N_x * g.N_y * g.N_z
+    capi.batch_sweep(ctx, jobs, inner=[0] * n)  # first sweeps (no zero skipping)
+    capi.batch_sweep(ctx, jobs, inner=[1] * n)
+    t0 = _t.perf_counter()
+    for k in range(steps):
+        capi.batch_sweep(ctx, jobs, inner=[2 + k] * n)
+        if wl.joint_estimation or wl.noise_model:
+            for j in jobs:
+                j.nuisance_update()
+    dt = _t.perf_counter() - t0
+    val = n * V * steps / dt
+    for j in jobs:
+        j.close()
+    log(f"batched: {n} jobs x {steps} steps in {dt:.3f}s -> {val:.4g} visits/s")
+    return {"jobs": n, "value": val, "unit": "voxel-updates/s", "ms_per_step": 1e3 * dt / steps,
+            "vs_single_job": val / single_value}
 
 
 def main():
@@ -150,12 +183,14 @@ def main():
                    "step": "ICD sweep + joint gain/offset + noise update",
                    "schedule": "conflict-free" if args.schedule == 0 else
                                f"relaxed rows_per_step={args.rows_per_step}",
-                   "kernel": "persistent" if args.kernel_mode == 1 else "per-step launches",
+                   "kernel": "batched step kernel" if args.kernel_mode == 1 else "step kernel",
                    "parallelism": "single GPU"},
         "updated_frac": upd / max(1, upd + zs),
         "final_cost": cost,
         "roofline": roofline,
     }
+    if args.batch > 0:
+        out["batched"] = batched(ctx, wl, args.batch, args.batch_steps, value)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(wl, w["angles"], w["sinogram"], threads)

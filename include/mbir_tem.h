@@ -108,7 +108,7 @@ typedef struct mbir_sweep_stats {
   int64_t zero_skipped;    /* voxels skipped by zero-skipping */
   int64_t updated;         /* voxels updated */
   int64_t nnz_updated;     /* sum over updated voxels of A-column entries x |VLR| */
-  double kernel_ms;        /* device time of the sweep's ICD kernels (when timing is on) */
+  double kernel_ms;        /* device time from the sweep's first ICD launch to its last (timing on) */
   int32_t steps;           /* kernel steps in the sweep */
   int32_t status;          /* 1 = continue, 0 = converged (stop threshold met) */
 } mbir_sweep_stats;

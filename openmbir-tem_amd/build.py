@@ -13,25 +13,30 @@ OUT = os.path.join(HERE, "libmbir_tem.so")
 BUILD = os.path.join(HERE, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include")]
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"),
+          "-I", os.path.join(HERE, "engine")]
 DEVICE = [
     "--offload-arch=gfx950",
     "-munsafe-fp-atomics",
     # packed fp32 instructions stay off ("build flags")
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+ENGINE = os.path.join(HERE, "engine")
 SOURCES = [
-    ("mbir_kernels.hip", ["-x", "hip"] + DEVICE),
-    ("mbir_capi.cpp", ["-x", "hip"] + DEVICE),
+    (os.path.join(CSRC, "mbir_kernels.hip"), ["-x", "hip"] + DEVICE),
+    (os.path.join(CSRC, "mbir_capi.cpp"), ["-x", "hip"] + DEVICE),
+    # host adapter with the reference engine's C++ interface (no device code)
+    (os.path.join(ENGINE, "HAADF_ReconstructionEngine.cpp"), ["-x", "c++"]),
 ]
+CLI = os.path.join(HERE, "engine_cli")
 
 
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     for src, flags in SOURCES:
-        obj = os.path.join(BUILD, src + ".o")
-        cmd = [HIPCC] + COMMON + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        cmd = [HIPCC] + COMMON + flags + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
@@ -41,6 +46,12 @@ def build(verbose=True):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(OUT + ".tmp", OUT)
+    cmd = [HIPCC] + COMMON + ["-x", "c++", os.path.join(ENGINE, "engine_cli.cpp"), "-I", ENGINE,
+                              "-o", CLI + ".tmp", "-L", HERE, "-lmbir_tem", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(CLI + ".tmp", CLI)
     return OUT
 
 

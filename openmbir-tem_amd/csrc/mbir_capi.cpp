@@ -201,7 +201,10 @@ struct mbir_job {
   // device buffers
   DevBuf<float> x, e, y, wview, vlr_val, colv;
   DevBuf<double> cosv, sinv, Hd, I0, mu, alpha, I0b, mub, sums6, yest, cost, prj;
-  DevBuf<int> vlr_t0, vlr_n, line_nnz, sched, colr, step_ptr_d;
+  DevBuf<int> vlr_t0, vlr_n, line_nnz, sched, colr, step_ptr_d, col_ptr, col_row;
+  DevBuf<float> col_a, col_ka, col_kw;
+  std::vector<int> nnz_h;  // per-line column entries (host copy)
+  int64_t nnz_total = 0;
   DevBuf<uint8_t> mask;
   DevBuf<SweepAccum> acc;
   std::vector<int> sched_h, step_ptr;
@@ -216,7 +219,8 @@ struct mbir_job {
     cosv.release(); sinv.release(); Hd.release(); I0.release(); mu.release(); alpha.release();
     I0b.release(); mub.release(); sums6.release(); yest.release(); cost.release(); prj.release();
     vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
-    step_ptr_d.release(); mask.release(); acc.release();
+    step_ptr_d.release(); mask.release(); acc.release(); col_ptr.release(); col_row.release();
+    col_a.release(); col_ka.release(); col_kw.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -448,8 +452,35 @@ int setup_job(mbir_job* j)
   J.pq2 = (float)(2.0 - P.mrf_q); J.inv_sigma_p = (float)(1.0 / sp); J.inv_sigma_pq = (float)(1.0 / spq);
   J.mrf_qd = P.mrf_q; J.mrf_cd = P.mrf_c; J.sigma_pd = sp; J.sigma_pqd = spq;
 
-  // columns: per-line nnz, zero lines with empty columns (:615-632)
+  // columns: per-line nnz, zero lines with empty columns (:615-632), then the
+  // compact column table of all lines (built once, FP32, with gains and weights folded in)
   HIP_TRY(launch_line_nnz(J, j->line_nnz.p, j->x.p, s));
+  j->nnz_h.assign(L, 0);
+  HIP_TRY(hipMemcpyAsync(j->nnz_h.data(), j->line_nnz.p, L * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  std::vector<int> cptr(L + 1, 0);
+  int64_t tot = 0;
+  for (size_t l = 0; l < L; ++l) {
+    cptr[l] = (int)tot;
+    tot += j->nnz_h[l];
+  }
+  if (tot >= 2147483647LL) return MBIR_ERR_UNSUPPORTED;
+  cptr[L] = (int)tot;
+  j->nnz_total = tot;
+  HIP_TRY(j->col_ptr.alloc(L + 1));
+  HIP_TRY(j->col_row.alloc((size_t)tot));
+  HIP_TRY(j->col_a.alloc((size_t)tot));
+  HIP_TRY(j->col_ka.alloc((size_t)tot));
+  HIP_TRY(j->col_kw.alloc((size_t)tot));
+  j->bytes += tot * 16 + (int64_t)(L + 1) * 4;
+  HIP_TRY(hipMemcpyAsync(j->col_ptr.p, cptr.data(), (L + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+  J.col_ptr = j->col_ptr.p;
+  J.col_row = j->col_row.p;
+  J.col_a = j->col_a.p;
+  J.col_ka = j->col_ka.p;
+  J.col_kw = j->col_kw.p;
+  HIP_TRY(launch_build_columns(J, j->col_row.p, j->col_a.p, s));
+  HIP_TRY(launch_column_weights(J, j->col_ka.p, j->col_kw.p, (int)tot, s));
   // initial forward projection and error sinogram (:646-691)
   HIP_TRY(j->yest.alloc(M));
   HIP_TRY(launch_fill_f64(j->yest.p, 0.0, M, s));
@@ -480,7 +511,8 @@ void build_schedule(mbir_job* j)
     for (int k = 0; k < R; ++k) {
       const int z = st.first + k * Zs;
       if (z >= Nz) continue;
-      for (int x = st.second; x < Nx; x += S) j->sched_h.push_back(z * Nx + x);
+      for (int x = st.second; x < Nx; x += S)
+        if (j->nnz_h[(size_t)z * Nx + x] > 0) j->sched_h.push_back(z * Nx + x);  // (:245-249)
     }
     j->step_ptr.push_back((int)j->sched_h.size());
   }
@@ -527,27 +559,25 @@ int run_sweep_steps(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
   int rc = prepare_sweep(j);
   if (rc < 0) return rc;
   const int nsteps = (int)j->step_ptr.size() - 1;
-  std::vector<hipEvent_t> ev;
+  // timing: one event pair around the sweep's back-to-back launches on the job stream
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   if (j->timing) {
-    ev.resize(2 * nsteps);
-    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
   }
   for (int k = 0; k < nsteps; ++k) {
     const int b = j->step_ptr[k], n = j->step_ptr[k + 1] - b;
-    if (j->timing) HIP_TRY(hipEventRecord(ev[2 * k], s));
     HIP_TRY(launch_icd_step(j->J, j->sched.p + b, n, outer, inner, j->prob.zero_skipping,
                             j->acc.p, s));
-    if (j->timing) HIP_TRY(hipEventRecord(ev[2 * k + 1], s));
   }
+  if (j->timing) HIP_TRY(hipEventRecord(e1, s));
   HIP_TRY(hipStreamSynchronize(s));
-  double ms = 0;
+  float ms = 0;
   if (j->timing) {
-    for (int k = 0; k < nsteps; ++k) {
-      float t = 0;
-      HIP_TRY(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
-      ms += t;
-    }
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   }
   return finish_sweep(j, ms, st);
 }
@@ -692,6 +722,7 @@ int nuisance_update(mbir_job* j)
     j->J.mu = j->mu.p;
   }
   if (j->prob.noise_model) HIP_TRY(launch_noise_update(j->J, j->alpha.p, j->wview.p, s));
+  HIP_TRY(launch_column_weights(j->J, j->col_ka.p, j->col_kw.p, (int)j->nnz_total, s));
   HIP_TRY(hipStreamSynchronize(s));
   return MBIR_OK;
 }
@@ -878,23 +909,24 @@ extern "C" int32_t mbir_job_get_column(mbir_job* j, int32_t z, int32_t x, uint32
                                        float* values, int32_t cap)
 {
   if (!j || z < 0 || x < 0 || z >= j->g.N_z || x >= j->g.N_x) return MBIR_ERR_INPUT;
-  hipStream_t s = j->stream;
-  const int ns = j->J.nslots;
-  std::vector<float> v(ns);
-  std::vector<int> r(ns);
-  HIP_TRY(launch_export_column(j->J, z, x, j->colv.p, j->colr.p, s));
-  HIP_TRY(hipMemcpyAsync(v.data(), j->colv.p, ns * sizeof(float), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(r.data(), j->colr.p, ns * sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  int n = 0;
-  for (int k = 0; k < ns; ++k)
-    if (v[k] > 0.f) {
-      if (n < cap) {
-        if (index) index[n] = (uint32_t)r[k];
-        if (values) values[n] = v[k];
-      }
-      n++;
+  // the entries the ICD kernels use: the device-built column table
+  const size_t line = (size_t)z * j->g.N_x + x;
+  int p[2];
+  HIP_TRY(hipMemcpyAsync(p, j->col_ptr.p + line, 2 * sizeof(int), hipMemcpyDeviceToHost, j->stream));
+  HIP_TRY(hipStreamSynchronize(j->stream));
+  const int n = p[1] - p[0];
+  if (n > 0 && cap > 0) {
+    const int m = std::min(n, cap);
+    std::vector<int> r(m);
+    std::vector<float> v(m);
+    HIP_TRY(hipMemcpyAsync(r.data(), j->col_row.p + p[0], m * sizeof(int), hipMemcpyDeviceToHost, j->stream));
+    HIP_TRY(hipMemcpyAsync(v.data(), j->col_a.p + p[0], m * sizeof(float), hipMemcpyDeviceToHost, j->stream));
+    HIP_TRY(hipStreamSynchronize(j->stream));
+    for (int k = 0; k < m; ++k) {
+      if (index) index[k] = (uint32_t)(r[k] / j->g.N_t);
+      if (values) values[k] = v[k];
     }
+  }
   return n;
 }
 

@@ -35,6 +35,13 @@ struct DevJob {
   const float* vlr_val;  // [N_y][kVlrCap]
   const uint8_t* mask;   // [N_z][N_x] ROI
   const int* line_nnz;   // [N_z][N_x] A-column entries per line (0 = empty column)
+  // compact A-matrix columns of all lines (built once on the device, FP32):
+  // entries col_ptr[line] .. col_ptr[line + 1] - 1
+  const int* col_ptr;    // [N_z * N_x + 1]
+  const int* col_row;    // sinogram row base (th * N_r + r) * N_t
+  const float* col_a;    // A value a
+  const float* col_ka;   // I0[th] * a            (rebuilt when I0 changes)
+  const float* col_kw;   // I0[th] * a * w[th]    (rebuilt when I0 or w changes)
   float* x;              // [N_z][N_x][N_y]
   float* e;              // [N_theta][N_r][N_t] error sinogram
   const float* y;        // [N_theta][N_r][N_t] measurements
@@ -79,6 +86,8 @@ void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv);
 // nwg workgroups in total
 hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const BatchEntry* ent,
                                  int e0, int ne, int nwg, hipStream_t s);
+hipError_t launch_build_columns(const DevJob& J, int* row, float* a, hipStream_t s);
+hipError_t launch_column_weights(const DevJob& J, float* ka, float* kw, int nnz, hipStream_t s);
 hipError_t launch_export_column(const DevJob& J, int z, int x, float* vals, int* rows,
                                 hipStream_t s);
 // joint estimation / weights / cost (FP64)

@@ -18,45 +18,67 @@ namespace mbir {
 namespace {
 
 // ---------------------------------------------------------------------------
-// A-matrix column of line (z, x) into LDS slots.  Slot (th, k) is detector bin
-// jmin(th) + k.  Restates HAADF/HAADF_ReconstructionEngine.cpp:1402-1497 in FP64:
-// r = x cos - z sin, bins [floor((r - dxz - R0)/dr), floor((r + dxz - R0)/dr)]
-// clipped to the detector, value = lerp of H[th] at |r - r_c| / OffsetR, kept
-// if > 0.  Empty slots get value 0.
+// A-matrix column of line (z, x), view th.  Restates
+// HAADF/HAADF_ReconstructionEngine.cpp:1402-1497 in FP64 (divisions by delta_r and
+// OffsetR as multiplications by their reciprocals): r = x cos - z sin, candidate bins
+// [floor((r - dxz - R0)/dr), floor((r + dxz - R0)/dr)] clipped to the detector,
+// value = lerp of H[th] at |r - r_c| / OffsetR, kept if > 0.  av[k] is the value of
+// bin jmin + k (0 = no entry); returns jmin.  All loads are issued unconditionally
+// from clamped addresses so that they are in flight together.
+__device__ inline int view_column(const DevJob& J, double xc, double zc, int th, double* av)
+{
+  const bool vth = th < J.N_theta;
+  const int thc = vth ? th : 0;
+  const double r = xc * J.cosv[thc] - zc * J.sinv[thc];
+  const double rmin = r - J.delta_xz, rmax = r + J.delta_xz;
+  int jmin = 0, jmax = -1;
+  if (vth && !(rmax < J.R0 || rmin > J.RMax)) {
+    jmin = (int)floor((rmin - J.R0) * J.inv_delta_r);
+    jmax = (int)floor((rmax - J.R0) * J.inv_delta_r);
+    if (jmax >= J.N_r) jmax = J.N_r - 1;
+    if (jmin < 0) jmin = 0;
+  }
+  const double* Hth = J.H + (size_t)thc * kResponseBins;
+  auto slot = [&](int k) {
+    const int j = jmin + k;
+    const double rc = J.R0 + ((double)j + 0.5) * J.delta_r;
+    const double dr = fabs(r - rc);
+    int idr = (int)floor(dr * J.inv_offsetR);
+    const bool ok = j <= jmax && idr >= 0 && idr < kResponseBins;
+    idr = ok ? idr : 0;
+    const int iidx = idr + 1 < kResponseBins ? idr + 1 : kResponseBins - 1;
+    const double h0 = Hth[idr], h1 = Hth[iidx];
+    const double w1 = dr - idr * J.offsetR;
+    const double w2 = (idr + 1) * J.offsetR - dr;
+    const double f1 = (w2 * J.inv_offsetR) * h0 + (w1 * J.inv_offsetR) * h1;
+    return (ok && f1 > 0) ? f1 : 0.0;
+  };
+  double a4[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) a4[k] = slot(k);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) av[k] = k < J.NB ? a4[k] : 0.0;
+#pragma unroll
+  for (int k = 4; k < kMaxNB; ++k) av[k] = 0.0;
+  if (J.NB > 4) {  // coarse voxels (delta_xz > 1.5 delta_r): more bins per view
+#pragma unroll
+    for (int k = 4; k < kMaxNB; ++k) av[k] = k < J.NB ? slot(k) : 0.0;
+  }
+  return jmin;
+}
+
+// Column of line (z, x) slot by slot: store(slot, th, j, a) for slot = th * NB + k.
 template <typename F>
 __device__ inline void build_column(const DevJob& J, int z, int x, F&& store)
 {
   const double xc = J.x0 + ((double)x + 0.5) * J.delta_xz;
   const double zc = J.z0 + ((double)z + 0.5) * J.delta_xz;
   for (int th = threadIdx.x; th < J.N_theta; th += blockDim.x) {
-    const double r = xc * J.cosv[th] - zc * J.sinv[th];
-    const double rmin = r - J.delta_xz;
-    const double rmax = r + J.delta_xz;
-    int jmin = 0, jmax = -1;
-    if (!(rmax < J.R0 || rmin > J.RMax)) {
-      jmin = (int)floor((rmin - J.R0) * J.inv_delta_r);
-      jmax = (int)floor((rmax - J.R0) * J.inv_delta_r);
-      if (jmax >= J.N_r) jmax = J.N_r - 1;
-      if (jmin < 0) jmin = 0;
-    }
-    const double* Hth = J.H + (size_t)th * kResponseBins;
-    for (int k = 0; k < J.NB; ++k) {
-      const int j = jmin + k;
-      double val = 0.0;
-      if (j <= jmax) {
-        const double rc = J.R0 + ((double)j + 0.5) * J.delta_r;
-        const double dr = fabs(r - rc);
-        const int idr = (int)floor(dr * J.inv_offsetR);
-        if (idr >= 0 && idr < kResponseBins) {
-          const double w1 = dr - idr * J.offsetR;
-          const double w2 = (idr + 1) * J.offsetR - dr;
-          const int iidx = idr + 1 < kResponseBins ? idr + 1 : kResponseBins - 1;
-          const double f1 = (w2 * J.inv_offsetR) * Hth[idr] + (w1 * J.inv_offsetR) * Hth[iidx];
-          if (f1 > 0) val = f1;
-        }
-      }
-      store(th * J.NB + k, th, j, val);
-    }
+    double av[kMaxNB];
+    const int jmin = view_column(J, xc, zc, th, av);
+#pragma unroll
+    for (int k = 0; k < kMaxNB; ++k)
+      if (k < J.NB) store(th * J.NB + k, th, jmin + k, av[k]);
   }
 }
 
@@ -209,6 +231,55 @@ __global__ __launch_bounds__(kBlock) void k_export_column(DevJob J, int z, int x
   });
 }
 
+// Compact columns of all lines into the table (col_ptr from the per-line counts):
+// one workgroup per line, one thread per view, order fixed by a wave scan.
+__global__ __launch_bounds__(kIcdBlock) void k_build_columns(DevJob J, int* row, float* a)
+{
+  __shared__ int wsum[kIcdWaves];
+  const int line = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int z = line / J.N_x, xi = line % J.N_x;
+  const double xc = J.x0 + ((double)xi + 0.5) * J.delta_xz;
+  const double zc = J.z0 + ((double)z + 0.5) * J.delta_xz;
+  double av[kMaxNB];
+  const int jmin = view_column(J, xc, zc, tid, av);
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxNB; ++k) cnt += av[k] > 0.0 ? 1 : 0;
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int pos = J.col_ptr[line] + incl - cnt;
+  for (int w = 0; w < wave; ++w) pos += wsum[w];
+  if (tid < J.N_theta && cnt > 0) {
+    const int rb = tid * J.N_r + jmin;
+#pragma unroll
+    for (int k = 0; k < kMaxNB; ++k)
+      if (av[k] > 0.0) {
+        row[pos] = (rb + k) * J.N_t;
+        a[pos] = (float)av[k];
+        ++pos;
+      }
+  }
+}
+
+// Fold the gains and the per-view weights into the column table:
+// ka = I0[th] a (kConst0 of …_UpdateVoxels.cpp:347), kw = ka w[th].
+__global__ void k_column_weights(DevJob J, float* ka, float* kw, int nnz)
+{
+  const int per = J.N_r * J.N_t;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += gridDim.x * blockDim.x) {
+    const int th = J.col_row[i] / per;
+    const double k0 = J.I0[th] * (double)J.col_a[i];
+    ka[i] = (float)k0;
+    kw[i] = (float)(k0 * (double)J.wview[th]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // ICD update of one line (z, x): all N_y voxels of the line.  Restates
 // UpdateYSlice::execute for one line (HAADF/HAADF_ReconstructionEngine_UpdateVoxels.cpp:260-455):
@@ -231,9 +302,9 @@ __global__ __launch_bounds__(kBlock) void k_export_column(DevJob J, int z, int x
 //   5. e -= I0 a vlr dx with fp32 atomics (lines of one step that share a
 //      detector bin never lose an update).
 struct IcdSmem {
-  float ka[kMaxSlots];   // I0[th] * a of the compact column entries
+  float ka[kMaxSlots];   // I0[th] * a of the line's column entries
+  float kw[kMaxSlots];   // I0[th] * a * w[th]
   int row[kMaxSlots];    // (th * N_r + r) * N_t
-  float wv[kMaxSlots];   // per-view weight of the entry
   float nbl[9][kMaxNy + 2];  // lines (z+dz, x+dx), y = -1 .. N_y (0 outside the volume)
   float x[kMaxNy];       // this line; the even phase writes its new values here
   float dx[kMaxNy];
@@ -256,7 +327,6 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
   const int Ny = J.N_y;
   const int z = line / J.N_x, xi = line % J.N_x;
   const size_t lbase = (size_t)line * Ny;
-  const int nnz_line = J.line_nnz[line];
   const bool zs_active = zs_on && !first;
 
   // ---- 1. the 3x3 neighbour lines (branch-free loads from clamped addresses) ----
@@ -287,82 +357,22 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
     sm.st_nu = 0u;
   }
 
-  // ---- 2. column (one thread per view), compacted by a wave scan ----
-  // Straight-line code: every load is issued unconditionally from a clamped
-  // address and selected afterwards, so the loads of all slots are in flight at once.
-  double av[kMaxNB];
-  int jmin = 0, cnt = 0;
-  const int th = tid;
-  const int thc = th < J.N_theta ? th : 0;
-  const double cth = J.cosv[thc], sth = J.sinv[thc], I0th = J.I0[thc];
-  const float wvth = J.wview[thc];
-  {
-    const double xc = J.x0 + ((double)xi + 0.5) * J.delta_xz;
-    const double zc = J.z0 + ((double)z + 0.5) * J.delta_xz;
-    const double r = xc * cth - zc * sth;
-    const double rmin = r - J.delta_xz, rmax = r + J.delta_xz;
-    int jmax = -1;
-    if (th < J.N_theta && !(rmax < J.R0 || rmin > J.RMax)) {
-      jmin = (int)floor((rmin - J.R0) * J.inv_delta_r);
-      jmax = (int)floor((rmax - J.R0) * J.inv_delta_r);
-      if (jmax >= J.N_r) jmax = J.N_r - 1;
-      if (jmin < 0) jmin = 0;
+  // ---- 2. the line's column from the table, into LDS ----
+  const int p0 = J.col_ptr[line];
+  const int nc = J.col_ptr[line + 1] - p0;
+#pragma unroll
+  for (int it = 0; it < kMaxSlots / kIcdBlock; ++it) {  // loads first, stores after
+    const int i = tid + it * kIcdBlock;
+    const int ic = p0 + (i < nc ? i : 0);
+    const int rw = J.col_row[ic];
+    const float ka = J.col_ka[ic], kw = J.col_kw[ic];
+    if (i < nc) {
+      sm.row[i] = rw;
+      sm.ka[i] = ka;
+      sm.kw[i] = kw;
     }
-    const double* Hth = J.H + (size_t)thc * kResponseBins;
-    auto slot = [&](int k) {
-      const int j = jmin + k;
-      const double rc = J.R0 + ((double)j + 0.5) * J.delta_r;
-      const double dr = fabs(r - rc);
-      int idr = (int)floor(dr * J.inv_offsetR);
-      const bool ok = j <= jmax && idr >= 0 && idr < kResponseBins;
-      idr = ok ? idr : 0;
-      const int iidx = idr + 1 < kResponseBins ? idr + 1 : kResponseBins - 1;
-      const double h0 = Hth[idr], h1 = Hth[iidx];
-      const double w1 = dr - idr * J.offsetR;
-      const double w2 = (idr + 1) * J.offsetR - dr;
-      const double f1 = (w2 * J.inv_offsetR) * h0 + (w1 * J.inv_offsetR) * h1;
-      return (ok && f1 > 0) ? f1 : 0.0;
-    };
-    double a4[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) a4[k] = slot(k);  // unconditional: all loads in flight
-#pragma unroll
-    for (int k = 0; k < 4; ++k) av[k] = k < J.NB ? a4[k] : 0.0;
-#pragma unroll
-    for (int k = 4; k < kMaxNB; ++k) av[k] = 0.0;
-    if (J.NB > 4) {  // coarse voxels (delta_xz > 1.5 delta_r): more bins per view
-#pragma unroll
-      for (int k = 4; k < kMaxNB; ++k) av[k] = k < J.NB ? slot(k) : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < kMaxNB; ++k) cnt += av[k] > 0.0 ? 1 : 0;
   }
-  int incl = cnt;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int t = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += t;
-  }
-  if (lane == 63) sm.wsum[wave] = incl;
-  __syncthreads();  // A: wave sums and neighbour lines visible
-  int nc = 0, pos = incl - cnt;
-#pragma unroll
-  for (int w = 0; w < kIcdWaves; ++w) {
-    const int ws = sm.wsum[w];
-    if (w < wave) pos += ws;
-    nc += ws;
-  }
-  if (th < J.N_theta && cnt > 0) {
-    const int rb = th * J.N_r + jmin;
-#pragma unroll
-    for (int k = 0; k < kMaxNB; ++k)
-      if (av[k] > 0.0) {
-        sm.ka[pos] = (float)(I0th * av[k]);
-        sm.row[pos] = (rb + k) * J.N_t;
-        sm.wv[pos] = wvth;
-        ++pos;
-      }
-  }
+  __syncthreads();  // A: neighbour lines and column visible
   if (tid < Ny) {
     const float v = sm.nbl[4][tid + 1];
     sm.x[tid] = v;
@@ -436,16 +446,17 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
       const int i = base + u * P;
       if (i < nc) {
         const float kav = sm.ka[i];
-        const float wvv = sm.wv[i];
+        const float kwv = sm.kw[i];
 #pragma unroll
         for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
           for (int k = 0; k < NV; ++k)
             if (act[jy] && k < vn[jy]) {
               const float pe = kav * vv[jy][k];
-              const float w = full_w ? wf[u][jy][k] : wvv;
-              t2[jy].add(pe * pe * w);
-              t1[jy].add(ev[u][jy][k] * pe * w);
+              // per-view weight folded in: pe w = kw vlr (the BF path carries w per entry)
+              const float pw = full_w ? pe * wf[u][jy][k] : kwv * vv[jy][k];
+              t2[jy].add(pe * pw);
+              t1[jy].add(ev[u][jy][k] * pw);
             }
       }
     }
@@ -521,7 +532,7 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
         xnew = (float)nv;
         J.x[lbase + y] = xnew;
         atomicAdd(&sm.st_nu, 1u);
-        atomicAdd(&sm.st_nn, (unsigned long long)nnz_line * (unsigned long long)J.vlr_n[y]);
+        atomicAdd(&sm.st_nn, (unsigned long long)nc * (unsigned long long)J.vlr_n[y]);
         if (roi) {
           atomicAdd(&sm.st_su, fabs((double)xnew - (double)v));
           atomicAdd(&sm.st_sv, fabs((double)v));
@@ -575,7 +586,6 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const int* lin
   __shared__ IcdSmem sm;
   if ((int)blockIdx.x >= nlines) return;
   const int line = lines[blockIdx.x];
-  if (J.line_nnz[line] == 0) return;  // the reference skips empty columns (:245-249)
   icd_line<YPL, NV>(J, line, first, zs_on, acc, sm);
 }
 
@@ -601,7 +611,6 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_batch_step(const DevJob* __re
   if (i < 0 || i >= E.nlines) return;
   const DevJob& J = jobs[E.job];
   const int line = E.lines[i];
-  if (J.line_nnz[line] == 0) return;
   icd_line<YPL, NV>(J, line, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm);
 }
 
@@ -816,6 +825,20 @@ hipError_t launch_init_error(const DevJob& J, const double* yest, hipStream_t s)
 {
   const size_t M = (size_t)J.N_theta * J.N_r * J.N_t;
   hipLaunchKernelGGL(k_init_error, dim3(grid_for(M)), dim3(kBlock), 0, s, J, yest);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_columns(const DevJob& J, int* row, float* a, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_build_columns, dim3(J.N_z * J.N_x), dim3(kIcdBlock), 0, s, J, row, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_column_weights(const DevJob& J, float* ka, float* kw, int nnz, hipStream_t s)
+{
+  if (nnz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_column_weights, dim3(grid_for((size_t)nnz)), dim3(kBlock), 0, s, J, ka,
+                     kw, nnz);
   return hipGetLastError();
 }
 

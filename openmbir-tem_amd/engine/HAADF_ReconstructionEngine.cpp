@@ -1,0 +1,154 @@
+// HAADF_ReconstructionEngine.cpp -- execute() of the reference engine interface on
+// the MI355X C-ABI.  The setup order follows HAADF_ReconstructionEngine::execute
+// (Code/MBIRLib/HAADF/HAADF_ReconstructionEngine.cpp:327-913): input checks,
+// geometry sizing (GenericFilters/InitialReconstructionInitializer.cpp:98-167),
+// nuisance initialisation, the reconstruction loop, then the outputs.
+#include "HAADF_ReconstructionEngine.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace mbirtem {
+
+HAADF_ReconstructionEngine::~HAADF_ReconstructionEngine() = default;
+
+void HAADF_ReconstructionEngine::fail(int code, const std::string& msg)
+{
+  m_ErrorCondition = code;
+  m_ErrorMessage = msg;
+}
+
+void HAADF_ReconstructionEngine::setCancel(bool v)
+{
+  m_Cancel.store(v);
+  std::lock_guard<std::mutex> lk(m_JobMutex);
+  if (v && m_Job) mbir_job_cancel(m_Job);
+}
+
+void HAADF_ReconstructionEngine::execute()
+{
+  m_ErrorCondition = 0;
+  m_ErrorMessage.clear();
+  m_Sweeps = 0;
+  m_Costs.clear();
+  // :369-393
+  if (!m_TomoInputs) return fail(-1, "Error: The TomoInput Structure was NULL.");
+  if (!m_Sinogram) return fail(-1, "Error: The Sinogram Structure was NULL.");
+  if (!m_Geometry) return fail(-1, "Error: The Geometry Structure was NULL.");
+  if (!m_ForwardModel) return fail(-1, "Error: The ForwardModel Structure was NULL.");
+  if (!m_AdvParams) m_AdvParams = std::make_shared<AdvancedParameters>();
+  if (m_Cancel.load()) return fail(-999, "cancelled");
+
+  Sinogram& S = *m_Sinogram;
+  TomoInputs& T = *m_TomoInputs;
+  const size_t M = (size_t)S.N_theta * S.N_r * S.N_t;
+  if (S.N_theta == 0 || S.counts.size() != M || S.angles.size() != S.N_theta)
+    return fail(-1, "Error reading Input Sinogram Data");
+
+  mbir_geometry g{};
+  int rc = mbir_geometry_init(S.N_r, S.N_t, S.N_theta, S.delta_r, S.delta_t, T.LengthZ,
+                              T.delta_xz, T.interpolateFactor, T.extendObject ? 1 : 0,
+                              S.angles.data(), &g);
+  if (rc < 0) return fail(-1, "Error computing the reconstruction geometry");
+  S.R0 = g.R0;
+  S.RMax = g.RMax;
+  S.T0 = g.T0;
+  S.TMax = g.TMax;
+  Geometry& G = *m_Geometry;
+  G.N_x = (uint16_t)g.N_x;
+  G.N_y = (uint16_t)g.N_y;
+  G.N_z = (uint16_t)g.N_z;
+  G.LengthX = g.LengthX;
+  G.LengthY = g.LengthY;
+  G.x0 = g.x0;
+  G.y0 = g.y0;
+  G.z0 = g.z0;
+  T.delta_xy = g.delta_xy;  // :109-110 (delta_xz kept as the caller's multiple)
+  const size_t V = (size_t)g.N_z * g.N_x * g.N_y;
+
+  std::vector<float> counts(M);
+  for (size_t i = 0; i < M; ++i) counts[i] = (float)S.counts[i];
+  std::vector<float> init;
+  if (!T.initialRecon.empty()) {
+    if (T.initialRecon.size() != V) return fail(-1, "Error reading Initial Reconstruction Data");
+    init.resize(V);
+    for (size_t i = 0; i < V; ++i) init[i] = (float)T.initialRecon[i];
+  }
+  const HAADF_ForwardModel& F = *m_ForwardModel;
+  auto opt = [&](const std::vector<Real_t>& v) -> const double* {
+    return v.size() == S.N_theta ? v.data() : nullptr;
+  };
+
+  mbir_problem p{};
+  p.geom = g;
+  p.angles_deg = S.angles.data();
+  p.counts = counts.data();
+  p.init_volume = init.empty() ? nullptr : init.data();
+  p.default_recon = T.defaultInitialRecon;
+  p.init_gain = opt(F.InitialGain);
+  p.init_offset = T.useDefaultOffset ? nullptr : opt(F.InitialOffset);
+  p.init_variance = opt(F.InitialVariance);
+  p.target_gain = T.targetGain;
+  p.default_offset = T.defaultOffset;
+  p.default_variance = T.defaultVariance;
+  p.sigma_x = T.SigmaX;
+  p.mrf_q = T.p;
+  p.mrf_c = 1e-3;  // HAADF/HAADF_QGGMRFPriorModel.cpp:15
+  p.stop_threshold = T.StopThreshold;
+  p.num_outer = T.NumOuterIter;
+  p.num_inner = T.NumIter;
+  p.joint_estimation = (int)m_AdvParams->JOINT_ESTIMATION;
+  p.noise_model = (int)m_AdvParams->NOISE_MODEL;
+  p.zero_skipping = (int)m_AdvParams->ZERO_SKIPPING;
+  p.seed = T.seed;
+  p.schedule = 0;
+  p.rows_per_step = 1;
+  p.kernel_mode = 0;
+
+  mbir_opts o{};
+  o.device = m_Device;
+  mbir_ctx* ctx = nullptr;
+  if (mbir_create(&o, &ctx) < 0) return fail(-100, "no usable GPU");
+  mbir_job* job = nullptr;
+  rc = mbir_job_create(ctx, &p, &job);
+  if (rc < 0) {
+    mbir_destroy(ctx);
+    return fail(rc, rc == -2 ? "Error Calling function detectorResponse" : "job setup failed");
+  }
+  {
+    std::lock_guard<std::mutex> lk(m_JobMutex);
+    m_Job = job;
+    if (m_Cancel.load()) mbir_job_cancel(job);
+  }
+  mbir_run_stats rs{};
+  std::vector<double> costs(4096);
+  int32_t nc = 0;
+  rc = mbir_job_run(job, &rs, costs.data(), (int32_t)costs.size(), &nc);
+  if (rc == 0) {
+    m_Sweeps = rs.sweeps;
+    m_Costs.assign(costs.begin(), costs.begin() + nc);
+    std::vector<float> vol(V);
+    HAADF_ForwardModel& Fw = *m_ForwardModel;
+    Fw.I_0.assign(S.N_theta, 0);
+    Fw.Mu.assign(S.N_theta, 0);
+    Fw.Alpha.assign(S.N_theta, 0);
+    Fw.TargetGain = T.targetGain;
+    if (mbir_job_get_volume(job, vol.data()) < 0 ||
+        mbir_job_get_nuisance(job, Fw.I_0.data(), Fw.Mu.data(), Fw.Alpha.data()) < 0) {
+      rc = -100;
+    } else {
+      G.Object.assign(vol.begin(), vol.end());
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(m_JobMutex);
+    m_Job = nullptr;
+  }
+  mbir_job_destroy(job);
+  mbir_destroy(ctx);
+  if (rc == -999) return fail(-999, "cancelled");
+  if (rc < 0) return fail(rc, "reconstruction failed");
+  m_ErrorCondition = 0;
+}
+
+}  // namespace mbirtem

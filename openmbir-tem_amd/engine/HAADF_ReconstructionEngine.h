@@ -1,0 +1,142 @@
+// HAADF_ReconstructionEngine.h -- C++ host adapter with the reference engine's
+// interface, running the reconstruction through the MI355X C-ABI (mbir_tem.h).
+//
+// Mirrors (Code/MBIRLib/ paths of the reference):
+//   Sinogram / Geometry / TomoInputs / AdvancedParameters
+//       Reconstruction/ReconstructionStructures.h:68-187
+//   HAADF_ForwardModel nuisance arrays (TargetGain, InitialGain/Offset/Variance,
+//       I_0, Mu, Alpha)                         HAADF/HAADF_ForwardModel.h:84-108
+//   HAADF_ReconstructionEngine properties, execute(), getErrorCondition(), setCancel()
+//       HAADF/HAADF_ReconstructionEngine.h:67-99, Common/AbstractFilter.h
+// Error codes follow the reference: 0 ok, -1 bad input, -2 detector response,
+// -999 cancelled (HAADF/HAADF_ReconstructionEngine.cpp:369-393,401,522-528);
+// -100 and below are device failures of this engine.
+// Files are not read or written here: the caller fills the sinogram (counts and
+// tilt angles in degrees) and reads back Geometry::Object and the nuisance arrays.
+#pragma once
+
+#include <atomic>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mbir_tem.h"
+
+namespace mbirtem {
+
+using Real_t = double;
+
+struct Sinogram {
+  uint16_t N_r = 0, N_t = 0, N_theta = 0;
+  Real_t delta_r = 0, delta_t = 0;
+  std::vector<Real_t> counts;  // [N_theta][N_r][N_t]
+  std::vector<Real_t> angles;  // tilt angles in degrees
+  Real_t R0 = 0, RMax = 0, T0 = 0, TMax = 0;
+};
+
+struct Geometry {
+  std::vector<Real_t> Object;  // [N_z][N_x][N_y], written by execute()
+  Real_t LengthX = 0, LengthY = 0;
+  uint16_t N_x = 0, N_z = 0, N_y = 0;
+  Real_t x0 = 0, z0 = 0, y0 = 0;
+};
+
+struct TomoInputs {
+  uint16_t NumIter = 0;
+  uint16_t NumOuterIter = 0;
+  Real_t SigmaX = 0;
+  Real_t p = 0;  // MRF_Q = diffuseness + 1
+  Real_t StopThreshold = 0;
+  bool extendObject = false;
+  Real_t LengthZ = 0;          // sample thickness
+  Real_t delta_xz = 1;         // voxel size multiple set by the multi-resolution driver
+  Real_t delta_xy = 1;
+  Real_t interpolateFactor = 1;
+  Real_t defaultInitialRecon = 0;
+  std::vector<Real_t> initialRecon;  // optional initial volume [N_z][N_x][N_y]
+  Real_t targetGain = 1;
+  bool useDefaultOffset = true;
+  Real_t defaultOffset = 0;
+  Real_t defaultVariance = 1;
+  uint32_t seed = 12345;  // update order (the reference seeds from the clock)
+};
+
+struct AdvancedParameters {
+  unsigned int JOINT_ESTIMATION = 1;
+  unsigned int ZERO_SKIPPING = 1;
+  unsigned int NOISE_MODEL = 1;
+};
+
+class HAADF_ForwardModel {
+ public:
+  using Pointer = std::shared_ptr<HAADF_ForwardModel>;
+  static Pointer New() { return Pointer(new HAADF_ForwardModel()); }
+  Real_t TargetGain = 1;
+  std::vector<Real_t> InitialGain, InitialOffset, InitialVariance;  // optional inputs
+  std::vector<Real_t> I_0, Mu, Alpha;                               // outputs of execute()
+};
+
+using SinogramPtr = std::shared_ptr<Sinogram>;
+using GeometryPtr = std::shared_ptr<Geometry>;
+using TomoInputsPtr = std::shared_ptr<TomoInputs>;
+using AdvancedParametersPtr = std::shared_ptr<AdvancedParameters>;
+
+class HAADF_ReconstructionEngine {
+ public:
+  using Pointer = std::shared_ptr<HAADF_ReconstructionEngine>;
+  static Pointer New() { return Pointer(new HAADF_ReconstructionEngine()); }
+  ~HAADF_ReconstructionEngine();
+
+  void setTomoInputs(TomoInputsPtr v) { m_TomoInputs = v; }
+  TomoInputsPtr getTomoInputs() const { return m_TomoInputs; }
+  void setSinogram(SinogramPtr v) { m_Sinogram = v; }
+  SinogramPtr getSinogram() const { return m_Sinogram; }
+  void setGeometry(GeometryPtr v) { m_Geometry = v; }
+  GeometryPtr getGeometry() const { return m_Geometry; }
+  void setAdvParams(AdvancedParametersPtr v) { m_AdvParams = v; }
+  AdvancedParametersPtr getAdvParams() const { return m_AdvParams; }
+  void setForwardModel(HAADF_ForwardModel::Pointer v) { m_ForwardModel = v; }
+  HAADF_ForwardModel::Pointer getForwardModel() const { return m_ForwardModel; }
+
+  // HIP device the engine runs on (default 0)
+  void setDevice(int d) { m_Device = d; }
+
+  // Runs setup, the outer/inner ICD loop with joint gain/offset and noise
+  // estimation, and fills Geometry::Object and the forward model's I_0, Mu, Alpha.
+  void execute();
+  int getErrorCondition() const { return m_ErrorCondition; }
+  const std::string& getErrorMessage() const { return m_ErrorMessage; }
+  // Thread-safe: a running execute() stops at its next sweep with -999.
+  void setCancel(bool v);
+  bool getCancel() const { return m_Cancel.load(); }
+
+  int getSweeps() const { return m_Sweeps; }
+  const std::vector<Real_t>& getCosts() const { return m_Costs; }
+
+  static void InitializeTomoInputs(TomoInputsPtr v) { *v = TomoInputs(); }
+  static void InitializeAdvancedParams(AdvancedParametersPtr v) { *v = AdvancedParameters(); }
+
+ protected:
+  HAADF_ReconstructionEngine() = default;
+
+ private:
+  void fail(int code, const std::string& msg);
+
+  TomoInputsPtr m_TomoInputs;
+  SinogramPtr m_Sinogram;
+  GeometryPtr m_Geometry;
+  AdvancedParametersPtr m_AdvParams;
+  HAADF_ForwardModel::Pointer m_ForwardModel;
+  int m_Device = 0;
+  int m_ErrorCondition = 0;
+  std::string m_ErrorMessage;
+  std::atomic<bool> m_Cancel{false};
+  std::mutex m_JobMutex;
+  mbir_job* m_Job = nullptr;
+  int m_Sweeps = 0;
+  std::vector<Real_t> m_Costs;
+};
+
+}  // namespace mbirtem

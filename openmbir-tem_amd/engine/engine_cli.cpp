@@ -1,0 +1,77 @@
+// engine_cli -- run HAADF_ReconstructionEngine::execute() on raw binary inputs.
+//
+//   engine_cli <sino.f64> <angles.f64> N_r N_t N_theta thickness sigma_x p
+//              outer inner joint noise seed <out_prefix>
+// sino.f64 is [N_theta][N_r][N_t] little-endian FP64, angles.f64 the N_theta tilts in
+// degrees.  Writes <out_prefix>.vol.f64 ([N_z][N_x][N_y]), <out_prefix>.nuis.f64 (I_0,
+// Mu, Alpha) and <out_prefix>.txt (dimensions, sweeps, error code, costs).
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <vector>
+
+#include "HAADF_ReconstructionEngine.h"
+
+using namespace mbirtem;
+
+static std::vector<double> read_f64(const char* path, size_t n)
+{
+  std::vector<double> v(n);
+  std::ifstream f(path, std::ios::binary);
+  f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(n * sizeof(double)));
+  if (!f) v.clear();
+  return v;
+}
+
+int main(int argc, char** argv)
+{
+  if (argc != 15) {
+    std::fprintf(stderr, "usage: %s sino.f64 angles.f64 N_r N_t N_theta thickness sigma_x p "
+                         "outer inner joint noise seed out_prefix\n", argv[0]);
+    return 2;
+  }
+  auto sino = std::make_shared<Sinogram>();
+  sino->N_r = (uint16_t)std::atoi(argv[3]);
+  sino->N_t = (uint16_t)std::atoi(argv[4]);
+  sino->N_theta = (uint16_t)std::atoi(argv[5]);
+  sino->delta_r = sino->delta_t = 1.0;
+  sino->counts = read_f64(argv[1], (size_t)sino->N_theta * sino->N_r * sino->N_t);
+  sino->angles = read_f64(argv[2], sino->N_theta);
+  auto ti = std::make_shared<TomoInputs>();
+  HAADF_ReconstructionEngine::InitializeTomoInputs(ti);
+  ti->LengthZ = std::atof(argv[6]);
+  ti->SigmaX = std::atof(argv[7]);
+  ti->p = std::atof(argv[8]);
+  ti->NumOuterIter = (uint16_t)std::atoi(argv[9]);
+  ti->NumIter = (uint16_t)std::atoi(argv[10]);
+  ti->StopThreshold = 1e-4;
+  ti->seed = (uint32_t)std::atoi(argv[13]);
+  auto adv = std::make_shared<AdvancedParameters>();
+  adv->JOINT_ESTIMATION = (unsigned)std::atoi(argv[11]);
+  adv->NOISE_MODEL = (unsigned)std::atoi(argv[12]);
+  auto geom = std::make_shared<Geometry>();
+  auto fm = HAADF_ForwardModel::New();
+  auto eng = HAADF_ReconstructionEngine::New();
+  eng->setTomoInputs(ti);
+  eng->setSinogram(sino);
+  eng->setGeometry(geom);
+  eng->setAdvParams(adv);
+  eng->setForwardModel(fm);
+  eng->execute();
+  const std::string pre = argv[14];
+  std::ofstream txt(pre + ".txt");
+  txt << "error " << eng->getErrorCondition() << "\n";
+  txt << "dims " << geom->N_z << " " << geom->N_x << " " << geom->N_y << "\n";
+  txt << "sweeps " << eng->getSweeps() << "\n";
+  for (double c : eng->getCosts()) txt << "cost " << c << "\n";
+  if (eng->getErrorCondition() == 0) {
+    std::ofstream v(pre + ".vol.f64", std::ios::binary);
+    v.write(reinterpret_cast<const char*>(geom->Object.data()),
+            (std::streamsize)(geom->Object.size() * sizeof(double)));
+    std::ofstream n(pre + ".nuis.f64", std::ios::binary);
+    for (auto* a : {&fm->I_0, &fm->Mu, &fm->Alpha})
+      n.write(reinterpret_cast<const char*>(a->data()), (std::streamsize)(a->size() * sizeof(double)));
+  }
+  std::printf("engine_cli: error %d, %d sweeps\n", eng->getErrorCondition(), eng->getSweeps());
+  return eng->getErrorCondition() == 0 ? 0 : 1;
+}

@@ -40,8 +40,12 @@ HAADF_512x64x256 = Workload("haadf_512x64x256_91v", 512, 64, 91, 70.0, 128, 20, 
 WORKLOADS = {w.name: w for w in (HAADF_128x16x64, HAADF_512x64x256)}
 
 
-def make(ctx, wl, **job_kw):
-    """Geometry, angles, phantom, sinogram and a ready job for workload wl."""
+def make(ctx, wl, seed=None, **job_kw):
+    """Geometry, angles, phantom, sinogram and a ready job for workload wl
+    (seed overrides the workload's phantom/noise seed)."""
+    if seed is not None:
+        import dataclasses
+        wl = dataclasses.replace(wl, seed=seed)
     ang = P.tilt_series(wl.n_views, wl.max_deg)
     geom = capi.geometry_init(wl.n_r, wl.n_t, ang, wl.thickness)
     shape = (geom.N_z, geom.N_x, geom.N_y)
