@@ -134,6 +134,9 @@ def test_joint_estimation_and_noise_model(gpu_ctx):
     # this iteration count; the GPU must be within 2x that spread
     assert np.sqrt(np.mean((xr - xg) ** 2)) <= 5e-2 * np.sqrt(np.mean(xr ** 2))
     assert costs_g[-1] == pytest.approx(costs_r[-1], rel=1e-3)
+    # the GPU's FP64 cost of its own end state equals the oracle's cost of that state
+    chk = O.Engine(g, ang, y, O.params(**kw), gains=I0g, offsets=mug, variances=alg, x0=xg)
+    assert job.cost() == pytest.approx(chk.cost(), rel=1e-6)
 
 
 def test_cancel_returns_minus_999(gpu_ctx):

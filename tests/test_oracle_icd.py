@@ -235,6 +235,18 @@ def test_joint_estimation_is_the_constrained_least_squares():
     _, _, alpha = eng.nuisance()
     np.testing.assert_allclose(alpha, np.mean(eng.e() ** 2, axis=(1, 2)), rtol=1e-12)
     np.testing.assert_allclose(eng.w(), 1.0 / alpha[:, None, None] * np.ones_like(y), rtol=1e-12)
+    # ... which is the maximum-likelihood variance: the cost's noise terms
+    # 1/2 sum e^2/alpha + 1/2 sum log(2 pi alpha) grow when any view's alpha moves
+    prm = O.params(joint_estimation=1, noise_model=1)
+    x = eng.x()
+    c0 = O.Engine(g, ang, y, prm, gains=I_new, offsets=mu_new, variances=alpha, x0=x).cost()
+    assert c0 == pytest.approx(eng.cost(), rel=1e-12)
+    for k in (0, 7, nv - 1):
+        for f in (0.97, 1.03):
+            a = alpha.copy()
+            a[k] *= f
+            c = O.Engine(g, ang, y, prm, gains=I_new, offsets=mu_new, variances=a, x0=x).cost()
+            assert c > c0
 
 
 def test_run_costs_never_increase():
