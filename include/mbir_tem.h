@@ -195,6 +195,72 @@ int64_t mbir_job_device_bytes(const mbir_job* job);
 /* 1 = time every ICD kernel step with HIP events on the job's stream */
 int32_t mbir_job_set_timing(mbir_job* job, int32_t on);
 
+/* ---- request server ----
+ * No reference counterpart: the reference runs one reconstruction per process
+ * (HAADF/HAADF_MultiResolutionReconstruction.cpp:410-436 drives one engine).  The
+ * server takes jobs as they arrive, admits them in arrival order when their
+ * device footprint fits the HBM budget and a slot is free, and runs every
+ * admitted job's execute() loop (the one mbir_job_run runs) concurrently:
+ * each launch carries the next conflict-free ICD step of every resident job, so
+ * a job joins at the next launch after its setup and leaves when it converges.
+ * A job's result is the same as mbir_job_run's for the same problem.  Jobs need
+ * N_y <= 128 and at most 2 detector rows per slice (else they fail with -3). */
+typedef struct mbir_server mbir_server;
+
+typedef struct mbir_server_opts {
+  int32_t max_jobs;          /* jobs resident on the device at once (0 = 64, at most 256) */
+  int32_t setup_threads;     /* host threads that set admitted jobs up (0 = 2, at most 8) */
+  uint64_t hbm_budget_bytes; /* device bytes for resident jobs (0 = the context's budget,
+                                or 80% of the free device memory) */
+} mbir_server_opts;
+
+#define MBIR_JOB_QUEUED 0
+#define MBIR_JOB_SETUP 1
+#define MBIR_JOB_RUNNING 2
+#define MBIR_JOB_DONE 3
+#define MBIR_JOB_FAILED 4
+
+typedef struct mbir_job_report {
+  int32_t state;            /* MBIR_JOB_* */
+  int32_t error;            /* 0, or the failure code (-999 cancelled, -4 over budget, ...) */
+  int32_t sweeps;
+  int32_t outer_iterations;
+  int32_t status;           /* last sweep status (0 = stop threshold met) */
+  int32_t pad;
+  int64_t voxel_visits;     /* updated + zero-skipped voxels over all sweeps */
+  int64_t updated;
+  int64_t device_bytes;
+  double t_submit, t_admit, t_start, t_done; /* seconds since the server was created */
+} mbir_job_report;
+
+typedef struct mbir_server_stats {
+  int64_t ticks, launches, entries, workgroups;
+  int32_t jobs_submitted, jobs_done, jobs_failed, peak_resident;
+  uint64_t budget_bytes, reserved_bytes, peak_reserved_bytes;
+  int32_t fatal_error, pad;
+} mbir_server_stats;
+
+/* Upper bound of a job's device footprint, used for admission. */
+int64_t mbir_problem_device_bytes(const mbir_problem* prob);
+int32_t mbir_server_create(mbir_ctx* ctx, const mbir_server_opts* opts, mbir_server** out);
+/* Copies the problem's host arrays; returns at once with a ticket. */
+int32_t mbir_server_submit(mbir_server* srv, const mbir_problem* prob, uint64_t* ticket);
+/* Waits for one job (timeout_s < 0: no limit).  0 = finished (rep->state DONE or
+ * FAILED), 1 = timed out (rep holds the current state). */
+int32_t mbir_server_wait(mbir_server* srv, uint64_t ticket, double timeout_s,
+                         mbir_job_report* rep);
+/* Waits for the next finished job not yet returned here: 0 and *ticket, or 1 on timeout. */
+int32_t mbir_server_wait_any(mbir_server* srv, double timeout_s, uint64_t* ticket);
+/* The finished job, for mbir_job_get_volume / _get_nuisance / ... (valid until release). */
+int32_t mbir_server_job(mbir_server* srv, uint64_t ticket, mbir_job** job);
+/* Frees a finished job's device state and returns its memory to the budget. */
+int32_t mbir_server_release(mbir_server* srv, uint64_t ticket);
+/* Thread-safe: a queued job fails at once, a running one at its next sweep end (-999). */
+int32_t mbir_server_cancel(mbir_server* srv, uint64_t ticket);
+int32_t mbir_server_get_stats(mbir_server* srv, mbir_server_stats* out);
+/* Cancels what is still queued or running, then frees everything. */
+int32_t mbir_server_destroy(mbir_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

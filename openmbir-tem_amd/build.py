@@ -18,13 +18,15 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "inclu
 DEVICE = [
     "--offload-arch=gfx950",
     "-munsafe-fp-atomics",
-    # packed fp32 instructions stay off ("build flags")
+    # no packed fp32 instructions: some of their forms have given wrong results on
+    # this hardware without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 ENGINE = os.path.join(HERE, "engine")
 SOURCES = [
     (os.path.join(CSRC, "mbir_kernels.hip"), ["-x", "hip"] + DEVICE),
     (os.path.join(CSRC, "mbir_capi.cpp"), ["-x", "hip"] + DEVICE),
+    (os.path.join(CSRC, "mbir_serve.cpp"), ["-x", "hip"] + DEVICE),
     # host adapter with the reference engine's C++ interface (no device code)
     (os.path.join(ENGINE, "HAADF_ReconstructionEngine.cpp"), ["-x", "c++"]),
 ]

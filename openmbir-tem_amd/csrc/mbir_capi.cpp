@@ -15,8 +15,7 @@
 #include <random>
 #include <vector>
 
-#include "../../include/mbir_tem.h"
-#include "mbir_internal.h"
+#include "mbir_host.h"
 
 using namespace mbir;
 
@@ -25,16 +24,6 @@ using namespace mbir;
 namespace {
 
 const double kPi = 3.14159265358979323846;
-
-#define HIP_TRY(expr)                                                                    \
-  do {                                                                                   \
-    hipError_t _e = (expr);                                                              \
-    if (_e != hipSuccess) {                                                              \
-      std::fprintf(stderr, "mbir: HIP error %s at %s:%d: %s\n", hipGetErrorName(_e),    \
-                   __FILE__, __LINE__, #expr);                                           \
-      return MBIR_ERR_HIP;                                                               \
-    }                                                                                    \
-  } while (0)
 
 // ---------------------------------------------------------------------------
 // Host FP64 detector model (once per job).
@@ -155,75 +144,9 @@ int voxel_line_responses(const mbir_geometry& g, double offsetT, std::vector<int
   return 0;
 }
 
-template <typename T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  hipError_t alloc(size_t count)
-  {
-    n = count;
-    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
-  }
-  void release()
-  {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-  }
-};
 
 }  // namespace
 
-struct mbir_ctx {
-  int device = 0;
-  int verbose = 0;
-  uint64_t budget = 0;
-  std::mutex mu;
-  hipStream_t stream = nullptr;  // stream of batched launches
-  DevBuf<DevJob> djobs;          // job table of a batched sweep
-  DevBuf<BatchEntry> ents;       // entry table of a batched sweep
-  ~mbir_ctx()
-  {
-    djobs.release(); ents.release();
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
-
-struct mbir_job {
-  mbir_ctx* ctx = nullptr;
-  mbir_problem prob{};
-  mbir_geometry g{};
-  hipStream_t stream = nullptr;
-  DevJob J{};
-  double offsetR = 0, offsetT = 0;
-  int spacing = 1;  // conflict-free x spacing of lines in one step
-  std::vector<double> angles_rad, H;
-  std::vector<double> I0h, muh, alphah;
-  // device buffers
-  DevBuf<float> x, e, y, wview, vlr_val, colv;
-  DevBuf<double> cosv, sinv, Hd, I0, mu, alpha, I0b, mub, sums6, yest, cost, prj;
-  DevBuf<int> vlr_t0, vlr_n, line_nnz, sched, colr, step_ptr_d, col_ptr, col_row;
-  DevBuf<float> col_a, col_ka, col_kw;
-  std::vector<int> nnz_h;  // per-line column entries (host copy)
-  int64_t nnz_total = 0;
-  DevBuf<uint8_t> mask;
-  DevBuf<SweepAccum> acc;
-  std::vector<int> sched_h, step_ptr;
-  std::atomic<int> cancel{0};
-  int timing = 0;
-  int sweep_counter = 0;
-  double stop_threshold = 0;
-  int64_t bytes = 0;
-  ~mbir_job()
-  {
-    x.release(); e.release(); y.release(); wview.release(); vlr_val.release(); colv.release();
-    cosv.release(); sinv.release(); Hd.release(); I0.release(); mu.release(); alpha.release();
-    I0b.release(); mub.release(); sums6.release(); yest.release(); cost.release(); prj.release();
-    vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
-    step_ptr_d.release(); mask.release(); acc.release(); col_ptr.release(); col_row.release();
-    col_a.release(); col_ka.release(); col_kw.release();
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
 
 // ---------------------------------------------------------------------------
 extern "C" int32_t mbir_version(void) { return MBIR_VERSION; }
@@ -445,6 +368,7 @@ int setup_job(mbir_job* j)
   J.mask = j->mask.p; J.line_nnz = j->line_nnz.p;
   J.x = j->x.p; J.e = j->e.p; J.y = j->y.p; J.wfull = nullptr; J.wview = j->wview.p;
   J.I0 = j->I0.p; J.mu = j->mu.p;
+  J.sched = j->sched.p; J.acc = j->acc.p;
   // QGGMRF constants (HAADF/HAADF_QGGMRFPriorModel.cpp:10-19), MRF_P = 2
   const double sp = std::pow(P.sigma_x, 2.0), spq = std::pow(P.sigma_x, 2.0 - P.mrf_q);
   J.mrf_q = (float)P.mrf_q; J.mrf_c = (float)P.mrf_c; J.sigma_p = (float)sp;
@@ -492,10 +416,16 @@ int setup_job(mbir_job* j)
   return MBIR_OK;
 }
 
+}  // namespace
+
+namespace mbir {
+
 // Step schedule of one sweep.  schedule 0: step (z, c) = lines {(z, c + S m)},
 // S the conflict-free spacing, steps in a seeded random order.  schedule 1:
-// rows_per_step rows spaced N_z / rows_per_step apart share a step.
-void build_schedule(mbir_job* j)
+// rows_per_step rows spaced N_z / rows_per_step apart share a step.  Steps whose
+// lines all have empty columns are dropped when drop_empty_steps is set (they
+// update nothing).
+void build_schedule(mbir_job* j, bool drop_empty_steps)
 {
   const int Nz = j->g.N_z, Nx = j->g.N_x, S = j->spacing;
   const int R = j->prob.schedule == 1 ? std::max(1, std::min(j->prob.rows_per_step, Nz)) : 1;
@@ -514,29 +444,26 @@ void build_schedule(mbir_job* j)
       for (int x = st.second; x < Nx; x += S)
         if (j->nnz_h[(size_t)z * Nx + x] > 0) j->sched_h.push_back(z * Nx + x);  // (:245-249)
     }
+    if (drop_empty_steps && (int)j->sched_h.size() == j->step_ptr.back()) continue;
     j->step_ptr.push_back((int)j->sched_h.size());
   }
 }
 
-// Upload the sweep's schedule and clear its accumulators (job stream).
-int prepare_sweep(mbir_job* j)
+// Upload the sweep's schedule (from sched_src, default the host schedule) and
+// clear its accumulators, on stream s.
+int prepare_sweep_on(mbir_job* j, hipStream_t s, const int* sched_src)
 {
-  hipStream_t s = j->stream;
-  build_schedule(j);
-  HIP_TRY(hipMemcpyAsync(j->sched.p, j->sched_h.data(), j->sched_h.size() * sizeof(int),
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(j->step_ptr_d.p, j->step_ptr.data(), j->step_ptr.size() * sizeof(int),
-                         hipMemcpyHostToDevice, s));
+  if (!sched_src) sched_src = j->sched_h.data();
+  if (!j->sched_h.empty())
+    HIP_TRY(hipMemcpyAsync(j->sched.p, sched_src, j->sched_h.size() * sizeof(int),
+                           hipMemcpyHostToDevice, s));
   HIP_TRY(launch_fill_f64((double*)j->acc.p, 0.0, sizeof(SweepAccum) / sizeof(double), s));
   return MBIR_OK;
 }
 
-// Read the accumulators and apply the stop test (…_UpdateVoxels.cpp:765-786).
-int finish_sweep(mbir_job* j, double ms, mbir_sweep_stats* st)
+// Stop test on a sweep's accumulators (…_UpdateVoxels.cpp:765-786); counts the sweep.
+int sweep_status(mbir_job* j, const SweepAccum& a, double ms, mbir_sweep_stats* st)
 {
-  SweepAccum a{};
-  HIP_TRY(hipMemcpyAsync(&a, j->acc.p, sizeof(a), hipMemcpyDeviceToHost, j->stream));
-  HIP_TRY(hipStreamSynchronize(j->stream));
   j->sweep_counter++;
   int status = 1;
   if (a.sum_abs_value > 0 && (a.sum_abs_update / a.sum_abs_value) < j->stop_threshold) status = 0;
@@ -551,6 +478,51 @@ int finish_sweep(mbir_job* j, double ms, mbir_sweep_stats* st)
     st->status = status;
   }
   return status;
+}
+
+// jointEstimation then updateWeights (HAADF/HAADF_ReconstructionEngine.cpp:768-786)
+// and the refresh of the gain- and weight-folded column values, enqueued on s.
+int nuisance_update_on(mbir_job* j, hipStream_t s)
+{
+  const int NT = j->g.N_theta;
+  if (j->prob.joint_estimation) {
+    HIP_TRY(launch_joint_sums(j->J, j->sums6.p, s));
+    HIP_TRY(launch_joint_solve(NT, j->sums6.p, j->prob.target_gain, j->I0.p, j->mu.p, j->I0b.p,
+                               j->mub.p, s));
+    HIP_TRY(launch_joint_apply(j->J, j->I0.p, j->mu.p, j->I0b.p, j->mub.p, s));
+    std::swap(j->I0.p, j->I0b.p);
+    std::swap(j->mu.p, j->mub.p);
+    j->J.I0 = j->I0.p;
+    j->J.mu = j->mu.p;
+  }
+  if (j->prob.noise_model) HIP_TRY(launch_noise_update(j->J, j->alpha.p, j->wview.p, s));
+  HIP_TRY(launch_column_weights(j->J, j->col_ka.p, j->col_kw.p, (int)j->nnz_total, s));
+  return MBIR_OK;
+}
+
+}  // namespace mbir
+
+namespace {
+
+// Schedule of the job's next sweep, uploaded on the job stream with its
+// accumulators cleared.
+int prepare_sweep(mbir_job* j)
+{
+  build_schedule(j, false);
+  int rc = prepare_sweep_on(j, j->stream, nullptr);
+  if (rc < 0) return rc;
+  HIP_TRY(hipMemcpyAsync(j->step_ptr_d.p, j->step_ptr.data(), j->step_ptr.size() * sizeof(int),
+                         hipMemcpyHostToDevice, j->stream));
+  return MBIR_OK;
+}
+
+// Read the accumulators and apply the stop test.
+int finish_sweep(mbir_job* j, double ms, mbir_sweep_stats* st)
+{
+  SweepAccum a{};
+  HIP_TRY(hipMemcpyAsync(&a, j->acc.p, sizeof(a), hipMemcpyDeviceToHost, j->stream));
+  HIP_TRY(hipStreamSynchronize(j->stream));
+  return sweep_status(j, a, ms, st);
 }
 
 int run_sweep_steps(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
@@ -709,21 +681,9 @@ int run_sweep(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
 
 int nuisance_update(mbir_job* j)
 {
-  hipStream_t s = j->stream;
-  const int NT = j->g.N_theta;
-  if (j->prob.joint_estimation) {
-    HIP_TRY(launch_joint_sums(j->J, j->sums6.p, s));
-    HIP_TRY(launch_joint_solve(NT, j->sums6.p, j->prob.target_gain, j->I0.p, j->mu.p, j->I0b.p,
-                               j->mub.p, s));
-    HIP_TRY(launch_joint_apply(j->J, j->I0.p, j->mu.p, j->I0b.p, j->mub.p, s));
-    std::swap(j->I0.p, j->I0b.p);
-    std::swap(j->mu.p, j->mub.p);
-    j->J.I0 = j->I0.p;
-    j->J.mu = j->mu.p;
-  }
-  if (j->prob.noise_model) HIP_TRY(launch_noise_update(j->J, j->alpha.p, j->wview.p, s));
-  HIP_TRY(launch_column_weights(j->J, j->col_ka.p, j->col_kw.p, (int)j->nnz_total, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  int rc = nuisance_update_on(j, j->stream);
+  if (rc < 0) return rc;
+  HIP_TRY(hipStreamSynchronize(j->stream));
   return MBIR_OK;
 }
 

@@ -1,0 +1,111 @@
+// mbir_host.h -- host-side state of the engine shared by the C-ABI (mbir_capi.cpp)
+// and the request server (mbir_serve.cpp): the context, one job's device buffers
+// and the sweep helpers both drive.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+
+#include "../../include/mbir_tem.h"
+#include "mbir_internal.h"
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess) {                                                              \
+      std::fprintf(stderr, "mbir: HIP error %s at %s:%d: %s\n", hipGetErrorName(_e),    \
+                   __FILE__, __LINE__, #expr);                                           \
+      return MBIR_ERR_HIP;                                                               \
+    }                                                                                    \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count)
+  {
+    n = count;
+    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+};
+
+using mbir::DevJob;
+using mbir::SweepAccum;
+using mbir::BatchEntry;
+
+struct mbir_ctx {
+  int device = 0;
+  int verbose = 0;
+  uint64_t budget = 0;
+  std::mutex mu;
+  hipStream_t stream = nullptr;  // stream of batched launches
+  DevBuf<DevJob> djobs;          // job table of a batched sweep
+  DevBuf<BatchEntry> ents;       // entry table of a batched sweep
+  ~mbir_ctx()
+  {
+    djobs.release(); ents.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct mbir_job {
+  mbir_ctx* ctx = nullptr;
+  mbir_problem prob{};
+  mbir_geometry g{};
+  hipStream_t stream = nullptr;
+  DevJob J{};
+  double offsetR = 0, offsetT = 0;
+  int spacing = 1;  // conflict-free x spacing of lines in one step
+  std::vector<double> angles_rad, H;
+  std::vector<double> I0h, muh, alphah;
+  // device buffers
+  DevBuf<float> x, e, y, wview, vlr_val, colv;
+  DevBuf<double> cosv, sinv, Hd, I0, mu, alpha, I0b, mub, sums6, yest, cost, prj;
+  DevBuf<int> vlr_t0, vlr_n, line_nnz, sched, colr, step_ptr_d, col_ptr, col_row;
+  DevBuf<float> col_a, col_ka, col_kw;
+  std::vector<int> nnz_h;  // per-line column entries (host copy)
+  int64_t nnz_total = 0;
+  DevBuf<uint8_t> mask;
+  DevBuf<SweepAccum> acc;
+  std::vector<int> sched_h, step_ptr;
+  std::atomic<int> cancel{0};
+  int timing = 0;
+  int sweep_counter = 0;
+  double stop_threshold = 0;
+  int64_t bytes = 0;
+  ~mbir_job()
+  {
+    x.release(); e.release(); y.release(); wview.release(); vlr_val.release(); colv.release();
+    cosv.release(); sinv.release(); Hd.release(); I0.release(); mu.release(); alpha.release();
+    I0b.release(); mub.release(); sums6.release(); yest.release(); cost.release(); prj.release();
+    vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
+    step_ptr_d.release(); mask.release(); acc.release(); col_ptr.release(); col_row.release();
+    col_a.release(); col_ka.release(); col_kw.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace mbir {
+
+// Sweep helpers (mbir_capi.cpp).  prepare_sweep builds the seeded step schedule
+// of the job's next sweep, uploads it and clears the sweep accumulators on stream s;
+// sweep_status applies the stop test to the accumulators (1 continue, 0 converged)
+// and counts the sweep; nuisance_update_on enqueues joint estimation, the noise
+// update and the column-weight refresh on s without waiting.
+void build_schedule(mbir_job* j, bool drop_empty_steps);
+int prepare_sweep_on(mbir_job* j, hipStream_t s, const int* sched_src);
+int sweep_status(mbir_job* j, const SweepAccum& a, double ms, mbir_sweep_stats* st);
+int nuisance_update_on(mbir_job* j, hipStream_t s);
+
+}  // namespace mbir

@@ -19,6 +19,15 @@ constexpr int kMaxSlots = 2048;       // N_theta * NB per column
 constexpr int kMaxNB = 8;             // detector bins per view per column
 constexpr int kVlrCap = 4;            // detector rows per voxel slice
 
+struct SweepAccum {  // device accumulators of one sweep
+  double sum_abs_update;
+  double sum_abs_value;
+  unsigned long long zero_skipped;
+  unsigned long long updated;
+  unsigned long long nnz_updated;
+  unsigned long long pad[3];
+};
+
 // Everything a kernel needs about one job; passed by value.
 struct DevJob {
   int N_theta, N_r, N_t, N_x, N_y, N_z;
@@ -53,16 +62,10 @@ struct DevJob {
   float mrf_q, mrf_c, sigma_p, sigma_pq, alpha, b0;  // b0 = P / (sigma^P c)
   float pq2, inv_sigma_p, inv_sigma_pq;            // 2 - q, 1 / sigma^P, 1 / sigma^(P-q)
   double mrf_qd, mrf_cd, sigma_pd, sigma_pqd;
+  const int* sched;      // [N_z * N_x] line ids of the current sweep, step by step
+  SweepAccum* acc;       // the current sweep's accumulators
 };
 
-struct SweepAccum {  // device accumulators of one sweep
-  double sum_abs_update;
-  double sum_abs_value;
-  unsigned long long zero_skipped;
-  unsigned long long updated;
-  unsigned long long nnz_updated;
-  unsigned long long pad[3];
-};
 
 // One job's share of one batched launch: the lines of its current step.
 struct BatchEntry {
@@ -73,6 +76,21 @@ struct BatchEntry {
   int flags;         // bit 0: first sweep, bit 1: zero skipping on
   int ypl, nv;       // kernel shape
   SweepAccum* acc;
+};
+
+// One job's share of one request-server launch: lines sched[line0 .. line0 + nlines)
+// of job slot `job`, served by workgroups wg_off .. wg_off + nlines - 1.
+struct TickEntry {
+  int job;
+  int line0;
+  int nlines;
+  int wg_off;
+  int flags;  // bit 0: first sweep, bit 1: zero skipping on
+};
+constexpr int kTickMax = 96;  // entries per launch, passed by value as a kernel argument
+struct TickArgs {
+  int n;
+  TickEntry e[kTickMax];
 };
 
 // ---- launchers (all asynchronous on `s`) ----
@@ -100,6 +118,11 @@ hipError_t launch_joint_apply(const DevJob& J, const double* I0_old, const doubl
 hipError_t launch_noise_update(const DevJob& J, double* alpha, float* wview, hipStream_t s);
 hipError_t launch_cost(const DevJob& J, const double* alpha, int noise_model, double* out3,
                        hipStream_t s);
+// one launch of the request server: kernel shape (ypl, nv), nwg workgroups
+hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& args, int nwg,
+                           hipStream_t s);
+// jobs[slot] = J, ordered on s (the descriptor travels as a kernel argument)
+hipError_t launch_set_devjob(DevJob* jobs, int slot, const DevJob& J, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 hipError_t launch_fill_f64(double* p, double v, size_t n, hipStream_t s);
 

@@ -616,6 +616,35 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_batch_step(const DevJob* __re
   icd_line<YPL, NV>(J, line, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm);
 }
 
+// One launch of the request server: entries travel by value in the kernel
+// arguments (no host-to-device copy per launch), each naming a job slot and a
+// contiguous run of its current sweep's schedule.
+template <int YPL, int NV>
+__global__ __launch_bounds__(kIcdBlock) void k_icd_tick(const DevJob* __restrict__ jobs,
+                                                        const TickArgs args)
+{
+  __shared__ IcdSmem sm;
+  const int g = blockIdx.x;
+  int lo = 0, hi = args.n - 1;
+  while (lo < hi) {  // last entry with wg_off <= g
+    const int mid = (lo + hi + 1) >> 1;
+    if (args.e[mid].wg_off <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const TickEntry& E = args.e[lo];
+  const int i = g - E.wg_off;
+  if (i < 0 || i >= E.nlines) return;
+  const DevJob& J = jobs[E.job];
+  const int line = J.sched[E.line0 + i];
+  icd_line<YPL, NV>(J, line, E.flags & 1, (E.flags >> 1) & 1, J.acc, sm);
+}
+
+// jobs[slot] = J: a job descriptor written in stream order.
+__global__ void k_set_devjob(DevJob* jobs, int slot, const DevJob J)
+{
+  if (threadIdx.x == 0) jobs[slot] = J;
+}
+
 // ---------------------------------------------------------------------------
 // Joint gain/offset estimation, FP64 per-view sums
 // (HAADF/HAADF_ReconstructionEngine_Extra.cpp:443-498): with Ŷ = (y - e - mu)/I0,
@@ -900,6 +929,27 @@ hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const Batc
     hipLaunchKernelGGL((k_icd_batch_step<2, 1>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, ent, e0, ne);
   else
     hipLaunchKernelGGL((k_icd_batch_step<2, 2>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, ent, e0, ne);
+  return hipGetLastError();
+}
+
+hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& args, int nwg,
+                           hipStream_t s)
+{
+  if (args.n <= 0 || args.n > kTickMax || nwg <= 0) return hipSuccess;
+  if (ypl == 1 && nv == 1)
+    hipLaunchKernelGGL((k_icd_tick<1, 1>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args);
+  else if (ypl == 1)
+    hipLaunchKernelGGL((k_icd_tick<1, 2>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args);
+  else if (nv == 1)
+    hipLaunchKernelGGL((k_icd_tick<2, 1>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args);
+  else
+    hipLaunchKernelGGL((k_icd_tick<2, 2>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_devjob(DevJob* jobs, int slot, const DevJob& J, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_set_devjob, dim3(1), dim3(64), 0, s, jobs, slot, J);
   return hipGetLastError();
 }
 

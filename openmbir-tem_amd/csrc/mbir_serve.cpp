@@ -1,0 +1,618 @@
+// mbir_serve.cpp -- request server: reconstruction jobs arrive at any time, are
+// admitted against an HBM budget, and run their execute() loops concurrently on
+// one GPU with continuous batching at ICD-step granularity.
+//
+// Each "tick" is one launch per kernel shape that carries the next conflict-free
+// step of every resident job, so jobs of any size share launches, a job that
+// arrives joins at the next tick, and a job that converges leaves at once.  Every
+// job keeps its volume, error sinogram, column table and nuisance parameters on
+// the device between sweeps.  A job's steps, sweeps and nuisance updates are the
+// same operations in the same order as mbir_job_run, so its result does not
+// depend on what else is running.
+//
+// Threads: setup workers admit queued jobs in arrival order (FIFO, no overtaking)
+// when the job fits the budget and a slot is free, and run the job's setup
+// (mbir_job_create) on the job's own stream; one tick thread owns the launch
+// stream, reads each sweep's accumulators back asynchronously (pinned memory +
+// event), applies the stop test and drives the outer/inner loop of
+// HAADF/HAADF_ReconstructionEngine.cpp:706-815.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+
+#include "mbir_host.h"
+
+using namespace mbir;
+
+namespace {
+
+constexpr int kMaxServerJobs = 256;
+constexpr int kTicksInFlight = 16;  // launches the host may run ahead of the device
+
+struct Req {
+  uint64_t ticket = 0;
+  mbir_problem prob{};  // host pointers into the copies below until setup is done
+  std::vector<double> angles, gain, offset, variance;
+  std::vector<float> counts, init;
+  int state = MBIR_JOB_QUEUED;
+  int error = 0;
+  mbir_job* job = nullptr;
+  int slot = -1;
+  int64_t est = 0, reserved = 0;
+  // position in execute()'s loop
+  int outer = 0, inner = 0, num_iter = 0, status = 1, sweeps = 0, outer_done = 0;
+  int step = 0, nsteps = 0;
+  bool waiting = false;    // the sweep's last step is launched; accumulators in flight
+  bool finishing = false;  // final nuisance update in flight
+  int ypl = 1, nv = 1;
+  SweepAccum* hacc = nullptr;  // pinned
+  int* hsched = nullptr;       // pinned copy of the current sweep's schedule
+  hipEvent_t ev = nullptr;
+  int64_t visits = 0, updated = 0;
+  double t_submit = 0, t_admit = 0, t_start = 0, t_done = 0;
+  std::atomic<bool> cancel{false};
+};
+
+int64_t estimate_bytes(const mbir_problem& p)
+{
+  const mbir_geometry& g = p.geom;
+  const int64_t V = (int64_t)g.N_z * g.N_x * g.N_y;
+  const int64_t M = (int64_t)g.N_theta * g.N_r * g.N_t;
+  const int64_t L = (int64_t)g.N_z * g.N_x;
+  const int64_t NB = (int64_t)std::ceil(2.0 * g.delta_xz / g.delta_r - 1e-12) + 1;
+  // x; e, y; FP64 projection during setup; per-line arrays; column table bound
+  return V * 4 + M * 8 + M * 8 + L * 21 + L * g.N_theta * NB * 16 + (1 << 20);
+}
+
+}  // namespace
+
+struct mbir_server {
+  mbir_ctx* ctx = nullptr;
+  int max_jobs = 64;
+  uint64_t budget = 0, reserved = 0, peak_reserved = 0;
+  int resident = 0;  // jobs holding a slot (in setup, running or finished but unreleased)
+  hipStream_t stream = nullptr;
+  DevBuf<DevJob> djobs;
+  std::vector<int> free_slots;
+  std::mutex mu;
+  std::condition_variable cv_setup, cv_run, cv_done;
+  std::deque<Req*> queue;
+  std::vector<Req*> ready;
+  std::map<uint64_t, Req*> reqs;
+  std::deque<uint64_t> done_fifo;
+  uint64_t next_ticket = 1;
+  bool stopping = false;
+  int setups_running = 0;
+  int fatal = 0;
+  std::vector<std::thread> setup_thr;
+  std::thread run_thr;
+  std::chrono::steady_clock::time_point t0;
+  mbir_server_stats st{};
+
+  double now() const
+  {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  void setup_loop();
+  void run_loop();
+  int start_sweep(Req* r);
+  int end_sweep(Req* r);
+  void finish(Req* r, int state, int error);  // with mu held
+};
+
+// ---------------------------------------------------------------------------
+void mbir_server::setup_loop()
+{
+  (void)hipSetDevice(ctx->device);
+  std::unique_lock<std::mutex> lk(mu);
+  for (;;) {
+    cv_setup.wait(lk, [&] {
+      if (stopping) return true;
+      if (queue.empty()) return false;
+      Req* h = queue.front();
+      if ((uint64_t)h->est > budget) return true;  // can never fit: fail it
+      return resident < max_jobs && reserved + (uint64_t)h->est <= budget;
+    });
+    if (stopping) return;
+    Req* r = queue.front();
+    queue.pop_front();
+    if ((uint64_t)r->est > budget) {
+      finish(r, MBIR_JOB_FAILED, MBIR_ERR_BUDGET);
+      continue;
+    }
+    r->reserved = r->est;
+    reserved += r->est;
+    peak_reserved = std::max(peak_reserved, reserved);
+    resident++;
+    setups_running++;
+    r->state = MBIR_JOB_SETUP;
+    r->t_admit = now();
+    lk.unlock();
+
+    mbir_job* job = nullptr;
+    int rc = mbir_job_create(ctx, &r->prob, &job);
+    const size_t L = rc == 0 ? (size_t)job->g.N_z * job->g.N_x : 0;
+    if (rc == 0 && (hipHostMalloc((void**)&r->hacc, sizeof(SweepAccum)) != hipSuccess ||
+                    hipHostMalloc((void**)&r->hsched, std::max<size_t>(L, 1) * sizeof(int)) !=
+                        hipSuccess ||
+                    hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess))
+      rc = MBIR_ERR_HIP;
+    if (rc == 0 && (job->J.N_y > 128 || job->J.vlr_max > 2)) rc = MBIR_ERR_UNSUPPORTED;
+
+    lk.lock();
+    setups_running--;
+    if (rc != 0) {
+      if (job) mbir_job_destroy(job);
+      reserved -= (uint64_t)r->reserved;
+      r->reserved = 0;
+      resident--;
+      finish(r, MBIR_JOB_FAILED, rc);
+      cv_setup.notify_all();
+      continue;
+    }
+    r->job = job;
+    // the measured footprint replaces the estimate
+    reserved = reserved - (uint64_t)r->reserved + (uint64_t)job->bytes;
+    r->reserved = job->bytes;
+    peak_reserved = std::max(peak_reserved, reserved);
+    std::vector<double>().swap(r->angles);
+    std::vector<float>().swap(r->counts);
+    std::vector<float>().swap(r->init);
+    ready.push_back(r);
+    cv_run.notify_all();
+    cv_setup.notify_all();
+  }
+}
+
+// Leaves the tick loop: records the outcome and wakes waiters (mu held).
+void mbir_server::finish(Req* r, int state, int error)
+{
+  r->state = state;
+  r->error = error;
+  r->t_done = now();
+  if (r->slot >= 0) {
+    free_slots.push_back(r->slot);
+    r->slot = -1;
+  }
+  if (state == MBIR_JOB_DONE) st.jobs_done++;
+  else st.jobs_failed++;
+  done_fifo.push_back(r->ticket);
+  cv_done.notify_all();
+}
+
+// Schedule of the job's next sweep: seeded step order, uploaded from pinned
+// memory on the launch stream with the accumulators cleared.
+int mbir_server::start_sweep(Req* r)
+{
+  mbir_job* j = r->job;
+  build_schedule(j, true);
+  if (!j->sched_h.empty()) std::memcpy(r->hsched, j->sched_h.data(), j->sched_h.size() * sizeof(int));
+  int rc = prepare_sweep_on(j, stream, r->hsched);
+  if (rc < 0) return rc;
+  r->step = 0;
+  r->nsteps = (int)j->step_ptr.size() - 1;
+  r->waiting = false;
+  if (r->nsteps == 0) {  // nothing to update: the sweep ends at once
+    HIP_TRY(hipMemcpyAsync(r->hacc, j->acc.p, sizeof(SweepAccum), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipEventRecord(r->ev, stream));
+    r->waiting = true;
+  }
+  return MBIR_OK;
+}
+
+// A sweep's accumulators are back: stop test, then the loop of
+// HAADF/HAADF_ReconstructionEngine.cpp:706-815 (mbir_job_run is the same loop).
+// Returns 1 when the job has left the tick loop.
+int mbir_server::end_sweep(Req* r)
+{
+  mbir_job* j = r->job;
+  mbir_sweep_stats ss{};
+  const int status = sweep_status(j, *r->hacc, 0.0, &ss);
+  r->waiting = false;
+  r->sweeps++;
+  r->status = status;
+  r->visits += ss.updated + ss.zero_skipped;
+  r->updated += ss.updated;
+  if (r->cancel.load() || j->cancel.load()) {
+    std::lock_guard<std::mutex> lk(mu);
+    finish(r, MBIR_JOB_FAILED, MBIR_ERR_CANCELLED);
+    return 1;
+  }
+  r->inner++;
+  if (status != 0 && r->inner < r->num_iter) return start_sweep(r) < 0 ? -1 : 0;
+  // end of this outer iteration's inner loop
+  const int outer = r->outer;
+  if (j->prob.joint_estimation || j->prob.noise_model) {
+    if (nuisance_update_on(j, stream) < 0) return -1;
+    if (launch_set_devjob(djobs.p, r->slot, j->J, stream) != hipSuccess) return -1;
+  }
+  r->outer++;
+  r->outer_done = r->outer;
+  if ((status == 0 && outer >= 1) || r->outer >= j->prob.num_outer) {
+    // done once the stream has passed the last nuisance update
+    if (hipEventRecord(r->ev, stream) != hipSuccess) return -1;
+    r->finishing = true;
+    return 0;
+  }
+  r->num_iter = 1;  // :712-715
+  r->inner = 0;
+  return start_sweep(r) < 0 ? -1 : 0;
+}
+
+void mbir_server::run_loop()
+{
+  (void)hipSetDevice(ctx->device);
+  std::vector<Req*> act;
+  std::vector<hipEvent_t> tick_ev(kTicksInFlight, nullptr);
+  for (auto& e : tick_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+  int64_t tick = 0, n_launch = 0, n_entries = 0, n_wg = 0;
+  TickArgs args;
+  auto fail_all = [&](int code) {
+    std::lock_guard<std::mutex> lk(mu);
+    fatal = code;
+    for (Req* r : act) finish(r, MBIR_JOB_FAILED, code);
+    act.clear();
+  };
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      if (act.empty())
+        cv_run.wait(lk, [&] {
+          return !ready.empty() || (stopping && queue.empty() && setups_running == 0);
+        });
+      if (act.empty() && ready.empty() && stopping && queue.empty() && setups_running == 0) break;
+      for (Req* r : ready) {
+        r->slot = free_slots.back();
+        free_slots.pop_back();
+        r->state = MBIR_JOB_RUNNING;
+        r->t_start = now();
+        r->num_iter = r->job->prob.num_inner;
+        icd_shape(r->job->J, r->job->J.vlr_max, &r->ypl, &r->nv);
+        act.push_back(r);
+      }
+      st.peak_resident = std::max<int32_t>(st.peak_resident, (int32_t)act.size());
+      std::vector<Req*> fresh;
+      fresh.swap(ready);
+      lk.unlock();
+      bool bad = false;
+      for (Req* r : fresh) {
+        if (launch_set_devjob(djobs.p, r->slot, r->job->J, stream) != hipSuccess ||
+            start_sweep(r) < 0)
+          bad = true;
+      }
+      if (bad) {
+        fail_all(MBIR_ERR_HIP);
+        continue;
+      }
+    }
+    // sweeps whose accumulators (or final nuisance updates) have arrived
+    bool bad = false;
+    for (Req* r : act) {
+      if (!r->waiting && !r->finishing) continue;
+      const hipError_t q = hipEventQuery(r->ev);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) {
+        bad = true;
+        break;
+      }
+      if (r->finishing) {
+        r->finishing = false;
+        std::lock_guard<std::mutex> lk(mu);
+        finish(r, MBIR_JOB_DONE, 0);
+        continue;
+      }
+      if (end_sweep(r) < 0) {
+        bad = true;
+        break;
+      }
+    }
+    if (bad) {
+      fail_all(MBIR_ERR_HIP);
+      continue;
+    }
+    act.erase(std::remove_if(act.begin(), act.end(),
+                             [](Req* r) {
+                               return r->state == MBIR_JOB_DONE || r->state == MBIR_JOB_FAILED;
+                             }),
+              act.end());
+    // one tick: the next step of every job that has one, one launch per kernel shape
+    bool launched = false;
+    for (int shape = 0; shape < 4 && !bad; ++shape) {
+      const int ypl = 1 + (shape >> 1), nv = 1 + (shape & 1);
+      args.n = 0;
+      int wg = 0;
+      std::vector<Req*> last;  // jobs whose sweep ends with this launch
+      auto flush = [&]() -> bool {
+        if (args.n == 0) return true;
+        if (launch_icd_tick(ypl, nv, djobs.p, args, wg, stream) != hipSuccess) return false;
+        n_launch++;
+        n_entries += args.n;
+        n_wg += wg;
+        launched = true;
+        for (Req* r : last) {
+          if (hipMemcpyAsync(r->hacc, r->job->acc.p, sizeof(SweepAccum), hipMemcpyDeviceToHost,
+                             stream) != hipSuccess ||
+              hipEventRecord(r->ev, stream) != hipSuccess)
+            return false;
+          r->waiting = true;
+        }
+        last.clear();
+        args.n = 0;
+        wg = 0;
+        return true;
+      };
+      for (Req* r : act) {
+        if (r->ypl != ypl || r->nv != nv || r->waiting || r->finishing || r->step >= r->nsteps)
+          continue;
+        mbir_job* j = r->job;
+        TickEntry& E = args.e[args.n++];
+        E.job = r->slot;
+        E.line0 = j->step_ptr[r->step];
+        E.nlines = j->step_ptr[r->step + 1] - E.line0;
+        E.wg_off = wg;
+        E.flags = ((r->outer == 0 && r->inner == 0) ? 1 : 0) | (j->prob.zero_skipping ? 2 : 0);
+        wg += E.nlines;
+        if (++r->step == r->nsteps) last.push_back(r);
+        if (args.n == kTickMax && !flush()) bad = true;
+      }
+      if (!bad && !flush()) bad = true;
+    }
+    if (bad) {
+      fail_all(MBIR_ERR_HIP);
+      continue;
+    }
+    if (launched) {
+      // bound how far the host runs ahead, so arrivals and sweep ends are seen promptly
+      hipEvent_t& e = tick_ev[tick % kTicksInFlight];
+      if (tick >= kTicksInFlight) (void)hipEventSynchronize(e);
+      (void)hipEventRecord(e, stream);
+      tick++;
+      std::lock_guard<std::mutex> lk(mu);
+      st.ticks = tick;
+      st.launches = n_launch;
+      st.entries = n_entries;
+      st.workgroups = n_wg;
+    } else {
+      // nothing to launch: wait for the oldest outstanding sweep end
+      for (Req* r : act)
+        if (r->waiting || r->finishing) {
+          (void)hipEventSynchronize(r->ev);
+          break;
+        }
+    }
+  }
+  for (auto& e : tick_ev) (void)hipEventDestroy(e);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int64_t mbir_problem_device_bytes(const mbir_problem* p)
+{
+  if (!p) return MBIR_ERR_INPUT;
+  return estimate_bytes(*p);
+}
+
+extern "C" int32_t mbir_server_create(mbir_ctx* ctx, const mbir_server_opts* o, mbir_server** out)
+{
+  if (!ctx || !out) return MBIR_ERR_INPUT;
+  *out = nullptr;
+  if (hipSetDevice(ctx->device) != hipSuccess) return MBIR_ERR_HIP;
+  auto* s = new mbir_server();
+  s->ctx = ctx;
+  int threads = 2;
+  if (o) {
+    if (o->max_jobs > 0) s->max_jobs = std::min(o->max_jobs, kMaxServerJobs);
+    if (o->setup_threads > 0) threads = std::min(o->setup_threads, 8);
+    s->budget = o->hbm_budget_bytes;
+  }
+  if (s->budget == 0) s->budget = ctx->budget;
+  if (s->budget == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      delete s;
+      return MBIR_ERR_HIP;
+    }
+    s->budget = (uint64_t)(0.8 * (double)fr);
+  }
+  if (hipStreamCreate(&s->stream) != hipSuccess || s->djobs.alloc(s->max_jobs) != hipSuccess) {
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->djobs.release();
+    delete s;
+    return MBIR_ERR_HIP;
+  }
+  for (int i = s->max_jobs - 1; i >= 0; --i) s->free_slots.push_back(i);
+  s->t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < threads; ++i) s->setup_thr.emplace_back([s] { s->setup_loop(); });
+  s->run_thr = std::thread([s] { s->run_loop(); });
+  *out = s;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_server_submit(mbir_server* s, const mbir_problem* p, uint64_t* ticket)
+{
+  if (!s || !p || !ticket || !p->angles_deg || !p->counts) return MBIR_ERR_INPUT;
+  const mbir_geometry& g = p->geom;
+  if (g.N_theta <= 0 || g.N_r <= 0 || g.N_t <= 0 || g.N_x <= 0 || g.N_y <= 0 || g.N_z <= 0)
+    return MBIR_ERR_INPUT;
+  auto* r = new Req();
+  const size_t NT = g.N_theta, M = NT * g.N_r * g.N_t, V = (size_t)g.N_z * g.N_x * g.N_y;
+  r->prob = *p;
+  r->angles.assign(p->angles_deg, p->angles_deg + NT);
+  r->counts.assign(p->counts, p->counts + M);
+  r->prob.angles_deg = r->angles.data();
+  r->prob.counts = r->counts.data();
+  if (p->init_volume) {
+    r->init.assign(p->init_volume, p->init_volume + V);
+    r->prob.init_volume = r->init.data();
+  }
+  auto copy = [&](const double* src, std::vector<double>& dst, const double** field) {
+    if (src) {
+      dst.assign(src, src + NT);
+      *field = dst.data();
+    }
+  };
+  copy(p->init_gain, r->gain, &r->prob.init_gain);
+  copy(p->init_offset, r->offset, &r->prob.init_offset);
+  copy(p->init_variance, r->variance, &r->prob.init_variance);
+  r->est = estimate_bytes(*p);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stopping) {
+    delete r;
+    return MBIR_ERR_INPUT;
+  }
+  r->ticket = s->next_ticket++;
+  r->t_submit = s->now();
+  s->reqs[r->ticket] = r;
+  s->queue.push_back(r);
+  s->st.jobs_submitted++;
+  *ticket = r->ticket;
+  s->cv_setup.notify_all();
+  return MBIR_OK;
+}
+
+static void fill_report(const Req* r, mbir_job_report* rep)
+{
+  if (!rep) return;
+  std::memset(rep, 0, sizeof(*rep));
+  rep->state = r->state;
+  rep->error = r->error;
+  rep->sweeps = r->sweeps;
+  rep->outer_iterations = r->outer_done;
+  rep->status = r->status;
+  rep->voxel_visits = r->visits;
+  rep->updated = r->updated;
+  rep->device_bytes = r->job ? r->job->bytes : 0;
+  rep->t_submit = r->t_submit;
+  rep->t_admit = r->t_admit;
+  rep->t_start = r->t_start;
+  rep->t_done = r->t_done;
+}
+
+extern "C" int32_t mbir_server_wait(mbir_server* s, uint64_t ticket, double timeout_s,
+                                    mbir_job_report* rep)
+{
+  if (!s) return MBIR_ERR_INPUT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return MBIR_ERR_INPUT;
+  Req* r = it->second;
+  auto fin = [&] { return r->state == MBIR_JOB_DONE || r->state == MBIR_JOB_FAILED; };
+  if (timeout_s < 0) s->cv_done.wait(lk, fin);
+  else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), fin)) {
+    fill_report(r, rep);
+    return 1;
+  }
+  fill_report(r, rep);
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_server_wait_any(mbir_server* s, double timeout_s, uint64_t* ticket)
+{
+  if (!s || !ticket) return MBIR_ERR_INPUT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto any = [&] { return !s->done_fifo.empty(); };
+  if (timeout_s < 0) s->cv_done.wait(lk, any);
+  else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), any)) return 1;
+  *ticket = s->done_fifo.front();
+  s->done_fifo.pop_front();
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_server_job(mbir_server* s, uint64_t ticket, mbir_job** job)
+{
+  if (!s || !job) return MBIR_ERR_INPUT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end() || it->second->state != MBIR_JOB_DONE) return MBIR_ERR_INPUT;
+  *job = it->second->job;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_server_release(mbir_server* s, uint64_t ticket)
+{
+  if (!s) return MBIR_ERR_INPUT;
+  Req* r = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return MBIR_ERR_INPUT;
+    r = it->second;
+    if (r->state != MBIR_JOB_DONE && r->state != MBIR_JOB_FAILED) return MBIR_ERR_INPUT;
+    s->reqs.erase(it);
+  }
+  if (r->job) mbir_job_destroy(r->job);
+  if (r->hacc) (void)hipHostFree(r->hacc);
+  if (r->hsched) (void)hipHostFree(r->hsched);
+  if (r->ev) (void)hipEventDestroy(r->ev);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (r->reserved > 0) {
+      s->reserved -= (uint64_t)r->reserved;
+      s->resident--;
+    }
+    s->cv_setup.notify_all();
+  }
+  delete r;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_server_cancel(mbir_server* s, uint64_t ticket)
+{
+  if (!s) return MBIR_ERR_INPUT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return MBIR_ERR_INPUT;
+  Req* r = it->second;
+  if (r->state == MBIR_JOB_QUEUED) {
+    s->queue.erase(std::remove(s->queue.begin(), s->queue.end(), r), s->queue.end());
+    s->finish(r, MBIR_JOB_FAILED, MBIR_ERR_CANCELLED);
+  } else {
+    r->cancel.store(true);
+  }
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_server_get_stats(mbir_server* s, mbir_server_stats* out)
+{
+  if (!s || !out) return MBIR_ERR_INPUT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->st;
+  out->reserved_bytes = s->reserved;
+  out->peak_reserved_bytes = s->peak_reserved;
+  out->budget_bytes = s->budget;
+  out->fatal_error = s->fatal;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_server_destroy(mbir_server* s)
+{
+  if (!s) return MBIR_ERR_INPUT;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+    while (!s->queue.empty()) {
+      Req* r = s->queue.front();
+      s->queue.pop_front();
+      s->finish(r, MBIR_JOB_FAILED, MBIR_ERR_CANCELLED);
+    }
+    for (auto& kv : s->reqs) kv.second->cancel.store(true);
+    s->cv_setup.notify_all();
+    s->cv_run.notify_all();
+  }
+  for (auto& t : s->setup_thr) t.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cv_run.notify_all();
+  }
+  s->run_thr.join();
+  (void)hipStreamSynchronize(s->stream);
+  std::vector<uint64_t> tickets;
+  for (auto& kv : s->reqs) tickets.push_back(kv.first);
+  for (uint64_t t : tickets) mbir_server_release(s, t);
+  s->djobs.release();
+  (void)hipStreamDestroy(s->stream);
+  delete s;
+  return MBIR_OK;
+}

@@ -70,6 +70,30 @@ class RunStats(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("max_jobs", C.c_int32), ("setup_threads", C.c_int32),
+                ("hbm_budget_bytes", C.c_uint64)]
+
+
+class JobReport(C.Structure):
+    _fields_ = [("state", C.c_int32), ("error", C.c_int32), ("sweeps", C.c_int32),
+                ("outer_iterations", C.c_int32), ("status", C.c_int32), ("pad", C.c_int32),
+                ("voxel_visits", C.c_int64), ("updated", C.c_int64), ("device_bytes", C.c_int64),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_start", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("ticks", C.c_int64), ("launches", C.c_int64), ("entries", C.c_int64),
+                ("workgroups", C.c_int64), ("jobs_submitted", C.c_int32),
+                ("jobs_done", C.c_int32), ("jobs_failed", C.c_int32),
+                ("peak_resident", C.c_int32), ("budget_bytes", C.c_uint64),
+                ("reserved_bytes", C.c_uint64), ("peak_reserved_bytes", C.c_uint64),
+                ("fatal_error", C.c_int32), ("pad", C.c_int32)]
+
+
+JOB_QUEUED, JOB_SETUP, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(5)
+
 _SIG = {
     "mbir_version": (C.c_int32, []),
     "mbir_create": (C.c_int32, [C.POINTER(Opts), C.POINTER(C.c_void_p)]),
@@ -98,6 +122,16 @@ _SIG = {
     "mbir_job_cancel": (C.c_int32, [C.c_void_p]),
     "mbir_job_device_bytes": (C.c_int64, [C.c_void_p]),
     "mbir_job_set_timing": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "mbir_problem_device_bytes": (C.c_int64, [C.POINTER(Problem)]),
+    "mbir_server_create": (C.c_int32, [C.c_void_p, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
+    "mbir_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(Problem), C.POINTER(C.c_uint64)]),
+    "mbir_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(JobReport)]),
+    "mbir_server_wait_any": (C.c_int32, [C.c_void_p, C.c_double, C.POINTER(C.c_uint64)]),
+    "mbir_server_job": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "mbir_server_release": (C.c_int32, [C.c_void_p, C.c_uint64]),
+    "mbir_server_cancel": (C.c_int32, [C.c_void_p, C.c_uint64]),
+    "mbir_server_get_stats": (C.c_int32, [C.c_void_p, C.POINTER(ServerStats)]),
+    "mbir_server_destroy": (C.c_int32, [C.c_void_p]),
 }
 
 _LIB = None
@@ -134,7 +168,8 @@ def _fptr(a):
 
 
 def batch_sweep(ctx, jobs, outer=None, inner=None):
-    """One ICD sweep of every job in one persistent launch; returns per-job SweepStats."""
+    """One ICD sweep of every job, each launch carrying one step of every job;
+    returns per-job SweepStats."""
     n = len(jobs)
     hs = (C.c_void_p * n)(*[j.h.value for j in jobs])
     o = (C.c_int32 * n)(*(outer if outer is not None else [0] * n))
@@ -172,48 +207,64 @@ class Context:
             pass
 
 
-class Job:
-    """One reconstruction job on the device (mbir_job_*)."""
-
-    def __init__(self, ctx, geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3,
+def make_problem(geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3,
                  target_gain=1.0, default_offset=0.0, default_variance=1.0, stop_threshold=1e-3,
                  num_outer=1, num_inner=20, joint_estimation=0, noise_model=0, zero_skipping=1,
                  seed=12345, schedule=0, rows_per_step=1, kernel_mode=0, init_volume=None,
-                 default_recon=0.0,
-                 init_gain=None, init_offset=None, init_variance=None):
+                 default_recon=0.0, init_gain=None, init_offset=None, init_variance=None):
+    """mbir_problem for the arrays given; returns (Problem, arrays to keep alive)."""
+    ang = np.ascontiguousarray(angles_deg, np.float64)
+    cnt = np.ascontiguousarray(counts, np.float32)
+    assert cnt.size == geom.N_theta * geom.N_r * geom.N_t
+    keep = [ang, cnt]
+    p = Problem()
+    p.geom = geom
+    p.angles_deg = _dptr(ang)
+    p.counts = _fptr(cnt)
+    if init_volume is not None:
+        iv = np.ascontiguousarray(init_volume, np.float32)
+        keep.append(iv)
+        p.init_volume = _fptr(iv)
+    p.default_recon = default_recon
+    for name, arr in (("init_gain", init_gain), ("init_offset", init_offset),
+                      ("init_variance", init_variance)):
+        if arr is not None:
+            a = np.ascontiguousarray(arr, np.float64)
+            keep.append(a)
+            setattr(p, name, _dptr(a))
+    p.target_gain = target_gain
+    p.default_offset = default_offset
+    p.default_variance = default_variance
+    p.sigma_x, p.mrf_q, p.mrf_c = sigma_x, mrf_q, mrf_c
+    p.stop_threshold = stop_threshold
+    p.num_outer, p.num_inner = num_outer, num_inner
+    p.joint_estimation, p.noise_model, p.zero_skipping = joint_estimation, noise_model, zero_skipping
+    p.seed = seed
+    p.schedule, p.rows_per_step, p.kernel_mode = schedule, rows_per_step, kernel_mode
+    return p, keep
+
+
+class Job:
+    """One reconstruction job on the device (mbir_job_*)."""
+
+    def __init__(self, ctx, geom, angles_deg, counts, **kw):
         self.ctx = ctx
         self.geom = geom
-        self._angles = np.ascontiguousarray(angles_deg, np.float64)
-        self._counts = np.ascontiguousarray(counts, np.float32)
-        assert self._counts.size == geom.N_theta * geom.N_r * geom.N_t
-        keep = [self._angles, self._counts]
-        p = Problem()
-        p.geom = geom
-        p.angles_deg = _dptr(self._angles)
-        p.counts = _fptr(self._counts)
-        if init_volume is not None:
-            iv = np.ascontiguousarray(init_volume, np.float32)
-            keep.append(iv)
-            p.init_volume = _fptr(iv)
-        p.default_recon = default_recon
-        for name, arr in (("init_gain", init_gain), ("init_offset", init_offset),
-                          ("init_variance", init_variance)):
-            if arr is not None:
-                a = np.ascontiguousarray(arr, np.float64)
-                keep.append(a)
-                setattr(p, name, _dptr(a))
-        p.target_gain = target_gain
-        p.default_offset = default_offset
-        p.default_variance = default_variance
-        p.sigma_x, p.mrf_q, p.mrf_c = sigma_x, mrf_q, mrf_c
-        p.stop_threshold = stop_threshold
-        p.num_outer, p.num_inner = num_outer, num_inner
-        p.joint_estimation, p.noise_model, p.zero_skipping = joint_estimation, noise_model, zero_skipping
-        p.seed = seed
-        p.schedule, p.rows_per_step, p.kernel_mode = schedule, rows_per_step, kernel_mode
+        self._owned = True
+        p, keep = make_problem(geom, angles_deg, counts, **kw)
         self.h = C.c_void_p()
         _chk("mbir_job_create", lib().mbir_job_create(ctx.h, C.byref(p), C.byref(self.h)))
         del keep
+
+    @classmethod
+    def borrowed(cls, handle, geom):
+        """View of a job owned elsewhere (a server's finished job): close() keeps it."""
+        j = cls.__new__(cls)
+        j.ctx = None
+        j.geom = geom
+        j._owned = False
+        j.h = C.c_void_p(handle)
+        return j
 
     @property
     def vshape(self):
@@ -226,9 +277,9 @@ class Job:
         return (g.N_theta, g.N_r, g.N_t)
 
     def close(self):
-        if self.h:
+        if self.h and self._owned:
             lib().mbir_job_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -304,3 +355,67 @@ class Job:
 
     def set_timing(self, on=True):
         _chk("mbir_job_set_timing", lib().mbir_job_set_timing(self.h, 1 if on else 0))
+
+
+def problem_device_bytes(geom, angles_deg, counts, **kw):
+    p, keep = make_problem(geom, angles_deg, counts, **kw)
+    return lib().mbir_problem_device_bytes(C.byref(p))
+
+
+class Server:
+    """Request server (mbir_server_*): jobs run concurrently with continuous batching."""
+
+    def __init__(self, ctx, max_jobs=0, setup_threads=0, hbm_budget_bytes=0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        self._geom = {}
+        o = ServerOpts(max_jobs, setup_threads, hbm_budget_bytes)
+        _chk("mbir_server_create", lib().mbir_server_create(ctx.h, C.byref(o), C.byref(self.h)))
+
+    def submit(self, geom, angles_deg, counts, **kw):
+        p, keep = make_problem(geom, angles_deg, counts, **kw)
+        t = C.c_uint64()
+        _chk("mbir_server_submit", lib().mbir_server_submit(self.h, C.byref(p), C.byref(t)))
+        del keep
+        self._geom[t.value] = geom
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        """JobReport once the job has finished, None on timeout."""
+        rep = JobReport()
+        rc = _chk("mbir_server_wait", lib().mbir_server_wait(self.h, ticket, timeout, C.byref(rep)))
+        return None if rc == 1 else rep
+
+    def wait_any(self, timeout=-1.0):
+        t = C.c_uint64()
+        rc = _chk("mbir_server_wait_any", lib().mbir_server_wait_any(self.h, timeout, C.byref(t)))
+        return None if rc == 1 else t.value
+
+    def job(self, ticket):
+        """The finished job (results through the Job getters) until release()."""
+        h = C.c_void_p()
+        _chk("mbir_server_job", lib().mbir_server_job(self.h, ticket, C.byref(h)))
+        return Job.borrowed(h.value, self._geom[ticket])
+
+    def release(self, ticket):
+        _chk("mbir_server_release", lib().mbir_server_release(self.h, ticket))
+        self._geom.pop(ticket, None)
+
+    def cancel(self, ticket):
+        _chk("mbir_server_cancel", lib().mbir_server_cancel(self.h, ticket))
+
+    def stats(self):
+        st = ServerStats()
+        _chk("mbir_server_get_stats", lib().mbir_server_get_stats(self.h, C.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            lib().mbir_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
