@@ -8,9 +8,14 @@ gain/offset estimation and the noise-variance update
 volume and nuisance state are resident on the GPU before timing starts.
 
 value = voxel visits per second (N_x*N_y*N_z per sweep; zero-skipped visits are
-reported separately).  roofline: the ICD line-update kernel (k_icd_step), its
-algorithmic bytes per launch over its average launch duration measured with HIP
-events on the engine's stream.  cpu_baseline: the FP64 C oracle's sweep with the
+reported separately).  batched_slabs: 48 independent slab problems of uneven size
+reconstructed to the stop threshold together in the request server, against the
+same problems run one after another with the plain per-job path (mbir_job_run with
+the job's own step kernel).  served: the same mix plus 5% 1024x128x512 jobs arriving
+as a Poisson stream (4 jobs/s) into the server; time-to-converged p50/p99 against a
+first-come-first-served queue of plain per-job runs (their measured service times).
+roofline: the ICD line-update kernel (k_icd_step), its algorithmic bytes per launch
+over its average launch duration measured with HIP events on the engine's stream.  cpu_baseline: the FP64 C oracle's sweep with the
 reference's y-slab thread decomposition on this host's cores.
 """
 import argparse
@@ -42,6 +47,10 @@ def parse():
     ap.add_argument("--batch", type=int, default=8,
                     help="jobs of the same workload swept together for the batched figure (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--slabs", type=int, default=48,
+                    help="independent slab problems for the batched and served figures (0 = skip)")
+    ap.add_argument("--arrival-rate", type=float, default=4.0)
+    ap.add_argument("--big-frac", type=float, default=0.05)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "icd_step_traffic.json"))
@@ -100,6 +109,165 @@ def batched(ctx, wl, n, steps, single_value):
     log(f"batched: {n} jobs x {steps} steps in {dt:.3f}s -> {val:.4g} visits/s")
     return {"jobs": n, "value": val, "unit": "voxel-updates/s", "ms_per_step": 1e3 * dt / steps,
             "vs_single_job": val / single_value}
+
+
+def _pct(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v, np.float64), q))
+
+
+def _rmse(a, b):
+    import numpy as np
+    d = np.asarray(a, np.float64) - np.asarray(b, np.float64)
+    return float(np.sqrt(np.mean(d * d)))
+
+
+def plain_runs(ctx, inputs):
+    """Each problem alone with the plain per-job path: setup + mbir_job_run (own step
+    kernel, one launch per conflict-free step).  Returns service time, sweeps, RMSE."""
+    from mbirtem import capi
+    out = []
+    for w in inputs:
+        t0 = time.perf_counter()
+        job = capi.Job(ctx, w["geom"], w["angles"], w["sinogram"], **w["job_kw"])
+        rs, _ = job.run(cap=0)
+        t1 = time.perf_counter()
+        vol = job.volume()
+        job.close()
+        g = w["geom"]
+        out.append({"service": t1 - t0, "sweeps": rs.sweeps, "rmse": _rmse(vol, w["volume"]),
+                    "visits": g.N_x * g.N_y * g.N_z * rs.sweeps})
+    return out
+
+
+def server_runs(ctx, inputs, arrivals):
+    """The problems submitted to the request server at the given arrival times (s)."""
+    import threading
+
+    from mbirtem import capi
+    srv = capi.Server(ctx, max_jobs=min(256, len(inputs)), setup_threads=4)
+    order = sorted(range(len(inputs)), key=lambda i: arrivals[i])
+    tickets = {}
+    base = time.perf_counter()
+
+    def feeder():
+        for i in order:
+            dt = base + arrivals[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            w = inputs[i]
+            tickets[srv.submit(w["geom"], w["angles"], w["sinogram"], **w["job_kw"])] = i
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    res = [None] * len(inputs)
+    for _ in range(len(inputs)):
+        t = srv.wait_any(timeout=600.0)
+        if t is None:
+            raise RuntimeError("request server: no job finished within 600 s")
+        while t not in tickets:
+            time.sleep(0.001)
+        i = tickets[t]
+        rep = srv.wait(t, timeout=0.0)
+        if rep.state != capi.JOB_DONE:
+            raise RuntimeError(f"request server: job {i} failed with {rep.error}")
+        vol = srv.job(t).volume()
+        srv.release(t)
+        g = inputs[i]["geom"]
+        res[i] = {"latency": rep.t_done - rep.t_submit, "t_submit": rep.t_submit,
+                  "t_done": rep.t_done, "sweeps": rep.sweeps, "rmse": _rmse(vol, inputs[i]["volume"]),
+                  "visits": g.N_x * g.N_y * g.N_z * rep.sweeps,
+                  "queue": rep.t_admit - rep.t_submit, "setup": rep.t_start - rep.t_admit}
+    th.join()
+    st = srv.stats()
+    srv.close()
+    return res, st
+
+
+def slab_figures(ctx, n, rate, big_frac):
+    """batched_slabs (all n problems at once) and served (Poisson arrivals) figures."""
+    import numpy as np
+
+    from mbirtem import workloads
+    specs = workloads.slab_mix(n, seed=7)
+    rng = np.random.default_rng(11)
+    served_specs = [workloads.slab_mix(1, seed=1000 + i, big_frac=1.0)[0]
+                    if rng.uniform() < big_frac else s for i, s in enumerate(specs)]
+    t0 = time.time()
+    cache = {}
+    for s in specs + served_specs:
+        if s.name not in cache:
+            cache[s.name] = workloads.make_slab(ctx, s)
+            if len(cache) % 8 == 0:
+                log(f"slabs: {len(cache)} problems prepared ({time.time() - t0:.1f}s)")
+    log(f"slabs: {len(cache)} problems prepared in {time.time() - t0:.1f}s")
+    plain = {}
+    t0 = time.time()
+    for k, (name, w) in enumerate(cache.items()):
+        plain[name] = plain_runs(ctx, [w])[0]
+        if k % 8 == 7:
+            log(f"slabs: {k + 1} plain runs done ({time.time() - t0:.1f}s)")
+    log(f"slabs: plain per-job runs {time.time() - t0:.1f}s")
+
+    # batched: every problem submitted at t = 0
+    inputs = [cache[s.name] for s in specs]
+    res, st = server_runs(ctx, inputs, [0.0] * n)
+    p = [plain[s.name] for s in specs]
+    for a, b in zip(res, p):
+        if a["sweeps"] != b["sweeps"] or a["rmse"] != b["rmse"]:
+            raise RuntimeError("request server result differs from the plain per-job run")
+    make = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    visits = sum(r["visits"] for r in res)
+    plain_total = sum(q["service"] for q in p)
+    plain_lat = np.cumsum([q["service"] for q in p])  # one after another, submitted at once
+    lat = [r["latency"] for r in res]
+    batched = {
+        "jobs": n, "value": visits / make, "unit": "voxel-updates/s",
+        "plain_value": visits / plain_total, "vs_plain": plain_total / make,
+        "makespan_s": make, "plain_total_s": plain_total,
+        "time_to_converged_p50_s": _pct(lat, 50), "time_to_converged_p99_s": _pct(lat, 99),
+        "plain_time_to_converged_p50_s": _pct(plain_lat, 50),
+        "plain_time_to_converged_p99_s": _pct(plain_lat, 99),
+        "p99_vs_plain": _pct(lat, 99) / _pct(plain_lat, 99),
+        "rmse_vs_phantom_mean": float(np.mean([r["rmse"] for r in res])),
+        "final_rmse_equal_to_plain": True, "sweeps_total": int(sum(r["sweeps"] for r in res)),
+        "launches": int(st.launches), "entries_per_launch": st.entries / max(1, st.launches),
+        "peak_resident": int(st.peak_resident), "peak_device_gb": st.peak_reserved_bytes / 1e9,
+    }
+    log(f"batched slabs: {batched['value']:.4g} visits/s vs plain {batched['plain_value']:.4g} "
+        f"({batched['vs_plain']:.2f}x), p99 {batched['time_to_converged_p99_s']:.2f}s vs "
+        f"{batched['plain_time_to_converged_p99_s']:.2f}s")
+
+    # served: Poisson arrivals
+    inputs = [cache[s.name] for s in served_specs]
+    arr = np.cumsum(rng.exponential(1.0 / rate, len(inputs)))
+    arr -= arr[0]
+    res, st = server_runs(ctx, inputs, list(arr))
+    p = [plain[s.name] for s in served_specs]
+    for a, b in zip(res, p):
+        if a["sweeps"] != b["sweeps"] or a["rmse"] != b["rmse"]:
+            raise RuntimeError("request server result differs from the plain per-job run")
+    fifo, done = [], 0.0
+    for a, q in zip(arr, p):  # first come, first served, one job at a time
+        done = max(done, a) + q["service"]
+        fifo.append(done - a)
+    lat = [r["latency"] for r in res]
+    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    served = {
+        "jobs": len(inputs), "arrival_rate_per_s": rate,
+        "big_jobs": sum(1 for s in served_specs if s.n_r == 1024 and s.n_t == 128),
+        "time_to_converged_p50_s": _pct(lat, 50), "time_to_converged_p99_s": _pct(lat, 99),
+        "plain_fifo_p50_s": _pct(fifo, 50), "plain_fifo_p99_s": _pct(fifo, 99),
+        "p99_vs_plain": _pct(lat, 99) / _pct(fifo, 99),
+        "value": sum(r["visits"] for r in res) / span, "unit": "voxel-updates/s",
+        "queue_wait_p99_s": _pct([r["queue"] for r in res], 99),
+        "setup_p50_s": _pct([r["setup"] for r in res], 50),
+        "final_rmse_equal_to_plain": True,
+        "peak_resident": int(st.peak_resident), "peak_device_gb": st.peak_reserved_bytes / 1e9,
+    }
+    log(f"served: p50 {served['time_to_converged_p50_s']:.2f}s p99 "
+        f"{served['time_to_converged_p99_s']:.2f}s vs plain FIFO p99 {served['plain_fifo_p99_s']:.2f}s")
+    return batched, served
 
 
 def main():
@@ -191,6 +359,9 @@ def main():
     }
     if args.batch > 0:
         out["batched"] = batched(ctx, wl, args.batch, args.batch_steps, value)
+    if args.slabs > 0:
+        out["batched_slabs"], out["served"] = slab_figures(ctx, args.slabs, args.arrival_rate,
+                                                           args.big_frac)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(wl, w["angles"], w["sinogram"], threads)

@@ -369,6 +369,7 @@ int setup_job(mbir_job* j)
   J.x = j->x.p; J.e = j->e.p; J.y = j->y.p; J.wfull = nullptr; J.wview = j->wview.p;
   J.I0 = j->I0.p; J.mu = j->mu.p;
   J.sched = j->sched.p; J.acc = j->acc.p;
+  J.e_atomic = P.schedule == 1 ? 1 : 0;
   // QGGMRF constants (HAADF/HAADF_QGGMRFPriorModel.cpp:10-19), MRF_P = 2
   const double sp = std::pow(P.sigma_x, 2.0), spq = std::pow(P.sigma_x, 2.0 - P.mrf_q);
   J.mrf_q = (float)P.mrf_q; J.mrf_c = (float)P.mrf_c; J.sigma_p = (float)sp;

@@ -62,6 +62,7 @@ struct DevJob {
   float mrf_q, mrf_c, sigma_p, sigma_pq, alpha, b0;  // b0 = P / (sigma^P c)
   float pq2, inv_sigma_p, inv_sigma_pq;            // 2 - q, 1 / sigma^P, 1 / sigma^(P-q)
   double mrf_qd, mrf_cd, sigma_pd, sigma_pqd;
+  int e_atomic;          // 1: lines of a step may share detector bins (relaxed schedule)
   const int* sched;      // [N_z * N_x] line ids of the current sweep, step by step
   SweepAccum* acc;       // the current sweep's accumulators
 };

@@ -301,8 +301,10 @@ __global__ void k_column_weights(DevJob J, float* ka, float* kw, int nnz)
 //      per lane, Kahan-compensated fp32 partial sums combined in FP64;
 //   4. the voxel solve of each parity runs on lane groups: the 26 neighbour
 //      terms of one voxel are spread over G lanes and combined by shuffles;
-//   5. e -= I0 a vlr dx with fp32 atomics (lines of one step that share a
-//      detector bin never lose an update).
+//   5. e -= I0 a vlr dx: plain read-modify-write, since no two lines of a
+//      conflict-free step share a detector bin (fp32 atomics for the relaxed
+//      schedule, whose lines may share bins).
+template <int YPL>
 struct IcdSmem {
   float ka[kMaxSlots];   // I0[th] * a of the line's column entries
   float kw[kMaxSlots];   // I0[th] * a * w[th]
@@ -313,16 +315,16 @@ struct IcdSmem {
   double T1[kMaxNy], T2[kMaxNy];
   unsigned char cand[kMaxNy];
   unsigned char need[kMaxNy];
-  float p1h[kIcdBlock * 4], p1l[kIcdBlock * 4], p2h[kIcdBlock * 4], p2l[kIcdBlock * 4];
-  int wsum[kIcdWaves];
+  // theta partial sums: (8 waves x 64/TY rows) x (TY x YPL slices) = kIcdBlock x YPL
+  float p1h[kIcdBlock * YPL], p1l[kIcdBlock * YPL], p2h[kIcdBlock * YPL], p2l[kIcdBlock * YPL];
   double st_su, st_sv;
   unsigned long long st_nn;
   unsigned int st_nz, st_nu;
 };
 
-template <int YPL, int NV>
+template <int YPL, int NV, bool FW>
 __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepAccum* acc,
-                         IcdSmem& sm)
+                         IcdSmem<YPL>& sm)
 {
   constexpr int U = YPL == 1 ? 24 : (YPL == 2 ? 12 : 6);  // detector rows in flight per lane
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -425,9 +427,8 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
       for (int k = 0; k < NV; ++k) vv[jy][k] = J.vlr_val[y * kVlrCap + k];
     }
   }
-  const bool full_w = J.wfull != nullptr;
   for (int base = p; base < nc; base += P * U) {
-    float ev[U][YPL][NV], wf[U][YPL][NV];
+    float ev[U][YPL][NV], wf[FW ? U : 1][YPL][NV];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int i = base + u * P;
@@ -440,7 +441,7 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
           const bool use = ok && act[jy] && k < vn[jy];
           const int idx = use ? rw + vt0[jy] + k : 0;
           ev[u][jy][k] = J.e[idx];
-          wf[u][jy][k] = full_w ? J.wfull[idx] : 0.f;
+          if constexpr (FW) wf[u][jy][k] = J.wfull[idx];
         }
     }
 #pragma unroll
@@ -456,7 +457,9 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
             if (act[jy] && k < vn[jy]) {
               const float pe = kav * vv[jy][k];
               // per-view weight folded in: pe w = kw vlr (the BF path carries w per entry)
-              const float pw = full_w ? pe * wf[u][jy][k] : kwv * vv[jy][k];
+              float pw;
+              if constexpr (FW) pw = pe * wf[u][jy][k];
+              else pw = kwv * vv[jy][k];
               t2[jy].add(pe * pw);
               t1[jy].add(ev[u][jy][k] * pw);
             }
@@ -546,26 +549,60 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
     __syncthreads();  // F, G
   }
 
-  // ---- 5. e -= I0 a vlr dx (:423-449), fp32 atomics ----
+  // ---- 5. e -= I0 a vlr dx (:423-449) ----
   float d[YPL];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
     const int y = yl + TY * jy;
     d[jy] = y < Ny ? sm.dx[y] : 0.f;
   }
-  for (int base = p; base < nc; base += P * U) {
+  if (J.e_atomic) {
+    for (int base = p; base < nc; base += P * U) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = base + u * P;
-      if (i < nc) {
-        const float ka = sm.ka[i];
-        const int rwi = sm.row[i];
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * P;
+        if (i < nc) {
+          const float ka = sm.ka[i];
+          const int rwi = sm.row[i];
+#pragma unroll
+          for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+              if (d[jy] != 0.f && k < vn[jy])
+                unsafeAtomicAdd(J.e + rwi + vt0[jy] + k, -(ka * vv[jy][k] * d[jy]));
+        }
+      }
+    }
+  } else {
+    // every load of a batch in flight before its stores; e - v equals e + (-v)
+    for (int base = p; base < nc; base += P * U) {
+      float ev[U][YPL][NV];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * P;
+        const bool ok = i < nc;
+        const int rw = sm.row[ok ? i : 0];
 #pragma unroll
         for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
-          for (int k = 0; k < NV; ++k)
-            if (d[jy] != 0.f && k < vn[jy])
-              unsafeAtomicAdd(J.e + rwi + vt0[jy] + k, -(ka * vv[jy][k] * d[jy]));
+          for (int k = 0; k < NV; ++k) {
+            const bool use = ok && d[jy] != 0.f && k < vn[jy];
+            ev[u][jy][k] = J.e[use ? rw + vt0[jy] + k : 0];
+          }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * P;
+        if (i < nc) {
+          const float ka = sm.ka[i];
+          const int rwi = sm.row[i];
+#pragma unroll
+          for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+              if (d[jy] != 0.f && k < vn[jy])
+                J.e[rwi + vt0[jy] + k] = ev[u][jy][k] - ka * vv[jy][k] * d[jy];
+        }
       }
     }
   }
@@ -585,10 +622,10 @@ template <int YPL, int NV>
 __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const int* lines, int nlines,
                                                         int first, int zs_on, SweepAccum* acc)
 {
-  __shared__ IcdSmem sm;
+  __shared__ IcdSmem<YPL> sm;
   if ((int)blockIdx.x >= nlines) return;
   const int line = lines[blockIdx.x];
-  icd_line<YPL, NV>(J, line, first, zs_on, acc, sm);
+  icd_line<YPL, NV, false>(J, line, first, zs_on, acc, sm);
 }
 
 // One conflict-free step of every job of a batch that has kernel shape
@@ -600,7 +637,7 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_batch_step(const DevJob* __re
                                                               const BatchEntry* __restrict__ ent,
                                                               int e0, int ne)
 {
-  __shared__ IcdSmem sm;
+  __shared__ IcdSmem<YPL> sm;
   const int g = blockIdx.x;
   int lo = e0, hi = e0 + ne - 1;
   while (lo < hi) {  // last entry with wg_off <= g
@@ -608,12 +645,13 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_batch_step(const DevJob* __re
     if (ent[mid].wg_off <= g) lo = mid;
     else hi = mid - 1;
   }
+  lo = __builtin_amdgcn_readfirstlane(lo);
   const BatchEntry& E = ent[lo];
   const int i = g - E.wg_off;
   if (i < 0 || i >= E.nlines) return;
-  const DevJob& J = jobs[E.job];
-  const int line = E.lines[i];
-  icd_line<YPL, NV>(J, line, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm);
+  const DevJob J = jobs[__builtin_amdgcn_readfirstlane(E.job)];
+  const int line = __builtin_amdgcn_readfirstlane(E.lines[i]);
+  icd_line<YPL, NV, false>(J, line, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm);
 }
 
 // One launch of the request server: entries travel by value in the kernel
@@ -623,7 +661,7 @@ template <int YPL, int NV>
 __global__ __launch_bounds__(kIcdBlock) void k_icd_tick(const DevJob* __restrict__ jobs,
                                                         const TickArgs args)
 {
-  __shared__ IcdSmem sm;
+  __shared__ IcdSmem<YPL> sm;
   const int g = blockIdx.x;
   int lo = 0, hi = args.n - 1;
   while (lo < hi) {  // last entry with wg_off <= g
@@ -631,12 +669,18 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_tick(const DevJob* __restrict
     if (args.e[mid].wg_off <= g) lo = mid;
     else hi = mid - 1;
   }
-  const TickEntry& E = args.e[lo];
-  const int i = g - E.wg_off;
-  if (i < 0 || i >= E.nlines) return;
-  const DevJob& J = jobs[E.job];
-  const int line = J.sched[E.line0 + i];
-  icd_line<YPL, NV>(J, line, E.flags & 1, (E.flags >> 1) & 1, J.acc, sm);
+  // the entry is uniform over the workgroup: keep its fields and the job
+  // descriptor's address in scalar registers
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const int job = __builtin_amdgcn_readfirstlane(args.e[lo].job);
+  const int i = g - __builtin_amdgcn_readfirstlane(args.e[lo].wg_off);
+  const int nl = __builtin_amdgcn_readfirstlane(args.e[lo].nlines);
+  const int l0 = __builtin_amdgcn_readfirstlane(args.e[lo].line0);
+  const int flags = __builtin_amdgcn_readfirstlane(args.e[lo].flags);
+  if (i < 0 || i >= nl) return;
+  const DevJob J = jobs[job];
+  const int line = __builtin_amdgcn_readfirstlane(J.sched[l0 + i]);
+  icd_line<YPL, NV, false>(J, line, flags & 1, (flags >> 1) & 1, J.acc, sm);
 }
 
 // jobs[slot] = J: a job descriptor written in stream order.

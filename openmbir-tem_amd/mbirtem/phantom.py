@@ -59,6 +59,28 @@ def random_ellipsoids(shape, n=20, seed=2, dens=(0.2, 1.0), dtype=np.float32):
     return ellipsoids(shape, items, dtype)
 
 
+def random_spheres(shape, n=12, seed=3, dens=(0.2, 1.0), dtype=np.float32):
+    """n random spheres inside the volume (radius up to 0.2 of the smallest of N_z, N_x and
+    twice N_y), densities U[dens]; later spheres overwrite earlier ones.  Each sphere is
+    drawn inside its bounding box only, so large volumes stay cheap."""
+    nz, nx, ny = shape
+    rng = np.random.default_rng(seed)
+    vol = np.zeros(shape, dtype)
+    rmax = max(2.0, 0.2 * min(nz, nx, 2 * ny))
+    for _ in range(n):
+        r = rng.uniform(0.3 * rmax, rmax)
+        c = [rng.uniform(r, nz - r), rng.uniform(r, nx - r), rng.uniform(0, ny)]
+        lo = [max(0, int(np.floor(c[k] - r))) for k in range(3)]
+        hi = [min(d, int(np.ceil(c[k] + r)) + 1) for k, d in enumerate(shape)]
+        z = np.arange(lo[0], hi[0])[:, None, None] + 0.5
+        x = np.arange(lo[1], hi[1])[None, :, None] + 0.5
+        y = np.arange(lo[2], hi[2])[None, None, :] + 0.5
+        inside = (z - c[0]) ** 2 + (x - c[1]) ** 2 + (y - c[2]) ** 2 <= r * r
+        box = vol[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        box[inside] = rng.uniform(*dens)
+    return vol
+
+
 def noisy_sinogram(proj, sigma, seed, gains=None, offsets=None):
     """y = gain * proj + offset + N(0, sigma^2), proj laid out [N_theta][N_r][N_t]."""
     rng = np.random.default_rng(seed)

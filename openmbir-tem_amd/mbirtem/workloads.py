@@ -68,3 +68,55 @@ def make(ctx, wl, seed=None, **job_kw):
     job = capi.Job(ctx, geom, ang, y, **kw)
     return dict(geom=geom, angles=ang, volume=vol, sinogram=y, gains=gains, offsets=offs,
                 job=job, job_kw=kw)
+
+
+# Many independent slab problems of uneven size: N_x (= N_r) in [128, 1024], N_y (= N_t)
+# in [8, 64], N_z in [32, 256], 41-141 tilts over +-[60, 72] degrees, sphere
+# phantoms, Gaussian noise sigma 0.02, gain 1, offset 0; each reconstructed until the
+# stop threshold (relative update 1e-3) is met.  big_frac of the draws are replaced
+# by 1024 x 128 x 512 problems (the large jobs of a served request stream).
+@dataclass
+class Slab:
+    name: str
+    n_r: int
+    n_t: int
+    n_views: int
+    max_deg: float
+    thickness: float
+    seed: int
+
+
+def slab_mix(n, seed=7, big_frac=0.0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        if big_frac > 0 and rng.uniform() < big_frac:
+            nr, nt, nz, nv = 1024, 128, 512, int(rng.integers(41, 142))
+        else:
+            nr = int(rng.integers(128, 1025))
+            nt = int(rng.integers(8, 65))
+            nz = int(rng.integers(32, 257))
+            nv = int(rng.integers(41, 142))
+        deg = float(rng.uniform(60.0, 72.0))
+        out.append(Slab(f"slab{i}_{nr}x{nt}x{nz}_{nv}v", nr, nt, nv, deg, nz / 2.0,
+                        int(rng.integers(1, 2 ** 31))))
+    return out
+
+
+SLAB_KW = dict(sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3, stop_threshold=1e-3, num_outer=1,
+               num_inner=60, joint_estimation=0, noise_model=0)
+
+
+def make_slab(ctx, s, noise=0.02):
+    """Geometry, angles, phantom and noisy sinogram of slab s (projected on the GPU)."""
+    ang = P.tilt_series(s.n_views, s.max_deg)
+    geom = capi.geometry_init(s.n_r, s.n_t, ang, s.thickness)
+    shape = (geom.N_z, geom.N_x, geom.N_y)
+    vol = P.random_spheres(shape, n=12, seed=s.seed)
+    zero = np.zeros((s.n_views, s.n_r, s.n_t), np.float32)
+    pj = capi.Job(ctx, geom, ang, zero)
+    proj = pj.forward_project(vol)
+    pj.close()
+    y = P.noisy_sinogram(proj, noise, s.seed + 1).astype(np.float32)
+    kw = dict(SLAB_KW, seed=s.seed % 100000)
+    return dict(slab=s, geom=geom, angles=ang, volume=vol, sinogram=y, job_kw=kw)
