@@ -18,6 +18,10 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "inclu
 DEVICE = [
     "--offload-arch=gfx950",
     "-munsafe-fp-atomics",
+    # no fused multiply-add contraction: the compensated sums and every kernel that
+    # shares the line update then round identically, so a job's result does not
+    # depend on which kernel (single job, batch, server) ran it
+    "-ffp-contract=off",
     # no packed fp32 instructions: some of their forms have given wrong results on
     # this hardware without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

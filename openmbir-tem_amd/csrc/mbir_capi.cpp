@@ -369,7 +369,9 @@ int setup_job(mbir_job* j)
   J.x = j->x.p; J.e = j->e.p; J.y = j->y.p; J.wfull = nullptr; J.wview = j->wview.p;
   J.I0 = j->I0.p; J.mu = j->mu.p;
   J.sched = j->sched.p; J.acc = j->acc.p;
-  J.e_atomic = P.schedule == 1 ? 1 : 0;
+  // one job alone: the atomic e-update is the faster form (the batch and server paths
+  // switch to the plain form for conflict-free schedules)
+  J.e_atomic = 1;
   // QGGMRF constants (HAADF/HAADF_QGGMRFPriorModel.cpp:10-19), MRF_P = 2
   const double sp = std::pow(P.sigma_x, 2.0), spq = std::pow(P.sigma_x, 2.0 - P.mrf_q);
   J.mrf_q = (float)P.mrf_q; J.mrf_c = (float)P.mrf_c; J.sigma_p = (float)sp;
@@ -392,6 +394,7 @@ int setup_job(mbir_job* j)
   if (tot >= 2147483647LL) return MBIR_ERR_UNSUPPORTED;
   cptr[L] = (int)tot;
   j->nnz_total = tot;
+  j->cptr_h = cptr;
   HIP_TRY(j->col_ptr.alloc(L + 1));
   HIP_TRY(j->col_row.alloc((size_t)tot));
   HIP_TRY(j->col_a.alloc((size_t)tot));
@@ -443,7 +446,10 @@ void build_schedule(mbir_job* j, bool drop_empty_steps)
       const int z = st.first + k * Zs;
       if (z >= Nz) continue;
       for (int x = st.second; x < Nx; x += S)
-        if (j->nnz_h[(size_t)z * Nx + x] > 0) j->sched_h.push_back(z * Nx + x);  // (:245-249)
+        if (j->nnz_h[(size_t)z * Nx + x] > 0) {  // (:245-249)
+          const int l = z * Nx + x;
+          j->sched_h.push_back(LineRef{l, j->cptr_h[l], j->nnz_h[l], 0});
+        }
     }
     if (drop_empty_steps && (int)j->sched_h.size() == j->step_ptr.back()) continue;
     j->step_ptr.push_back((int)j->sched_h.size());
@@ -452,11 +458,11 @@ void build_schedule(mbir_job* j, bool drop_empty_steps)
 
 // Upload the sweep's schedule (from sched_src, default the host schedule) and
 // clear its accumulators, on stream s.
-int prepare_sweep_on(mbir_job* j, hipStream_t s, const int* sched_src)
+int prepare_sweep_on(mbir_job* j, hipStream_t s, const LineRef* sched_src)
 {
   if (!sched_src) sched_src = j->sched_h.data();
   if (!j->sched_h.empty())
-    HIP_TRY(hipMemcpyAsync(j->sched.p, sched_src, j->sched_h.size() * sizeof(int),
+    HIP_TRY(hipMemcpyAsync(j->sched.p, sched_src, j->sched_h.size() * sizeof(LineRef),
                            hipMemcpyHostToDevice, s));
   HIP_TRY(launch_fill_f64((double*)j->acc.p, 0.0, sizeof(SweepAccum) / sizeof(double), s));
   return MBIR_OK;
@@ -573,6 +579,7 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
     int rc = prepare_sweep(j);
     if (rc < 0) return rc;
     dj[i] = j->J;
+    dj[i].e_atomic = j->prob.schedule == 1 ? 1 : 0;
     max_steps = std::max(max_steps, (int)j->step_ptr.size() - 1);
   }
   // jobs ordered by kernel shape so that each step's entries form one run per shape

@@ -43,6 +43,7 @@ struct DevBuf {
 using mbir::DevJob;
 using mbir::SweepAccum;
 using mbir::BatchEntry;
+using mbir::LineRef;
 
 struct mbir_ctx {
   int device = 0;
@@ -72,13 +73,16 @@ struct mbir_job {
   // device buffers
   DevBuf<float> x, e, y, wview, vlr_val, colv;
   DevBuf<double> cosv, sinv, Hd, I0, mu, alpha, I0b, mub, sums6, yest, cost, prj;
-  DevBuf<int> vlr_t0, vlr_n, line_nnz, sched, colr, step_ptr_d, col_ptr, col_row;
+  DevBuf<int> vlr_t0, vlr_n, line_nnz, colr, step_ptr_d, col_ptr, col_row;
+  DevBuf<LineRef> sched;
   DevBuf<float> col_a, col_ka, col_kw;
   std::vector<int> nnz_h;  // per-line column entries (host copy)
   int64_t nnz_total = 0;
   DevBuf<uint8_t> mask;
   DevBuf<SweepAccum> acc;
-  std::vector<int> sched_h, step_ptr;
+  std::vector<LineRef> sched_h;
+  std::vector<int> step_ptr;
+  std::vector<int> cptr_h;  // column-table offsets of all lines (host copy)
   std::atomic<int> cancel{0};
   int timing = 0;
   int sweep_counter = 0;
@@ -104,7 +108,7 @@ namespace mbir {
 // and counts the sweep; nuisance_update_on enqueues joint estimation, the noise
 // update and the column-weight refresh on s without waiting.
 void build_schedule(mbir_job* j, bool drop_empty_steps);
-int prepare_sweep_on(mbir_job* j, hipStream_t s, const int* sched_src);
+int prepare_sweep_on(mbir_job* j, hipStream_t s, const LineRef* sched_src);
 int sweep_status(mbir_job* j, const SweepAccum& a, double ms, mbir_sweep_stats* st);
 int nuisance_update_on(mbir_job* j, hipStream_t s);
 

@@ -19,6 +19,15 @@ constexpr int kMaxSlots = 2048;       // N_theta * NB per column
 constexpr int kMaxNB = 8;             // detector bins per view per column
 constexpr int kVlrCap = 4;            // detector rows per voxel slice
 
+// One scheduled line: its id and its column's place in the column table, so a
+// line update starts without a dependent lookup.
+struct LineRef {
+  int line;
+  int p0;  // first column-table entry
+  int nc;  // column entries
+  int pad;
+};
+
 struct SweepAccum {  // device accumulators of one sweep
   double sum_abs_update;
   double sum_abs_value;
@@ -62,15 +71,17 @@ struct DevJob {
   float mrf_q, mrf_c, sigma_p, sigma_pq, alpha, b0;  // b0 = P / (sigma^P c)
   float pq2, inv_sigma_p, inv_sigma_pq;            // 2 - q, 1 / sigma^P, 1 / sigma^(P-q)
   double mrf_qd, mrf_cd, sigma_pd, sigma_pqd;
-  int e_atomic;          // 1: lines of a step may share detector bins (relaxed schedule)
-  const int* sched;      // [N_z * N_x] line ids of the current sweep, step by step
+  int e_atomic;          // 1: error-sinogram update by fp32 atomics (required when lines of
+                         // a step may share detector bins: relaxed schedule), 0: plain
+                         // read-modify-write (conflict-free steps only)
+  const LineRef* sched;  // [N_z * N_x] lines of the current sweep, step by step
   SweepAccum* acc;       // the current sweep's accumulators
 };
 
 
 // One job's share of one batched launch: the lines of its current step.
 struct BatchEntry {
-  const int* lines;  // line ids of the step
+  const LineRef* lines;  // lines of the step
   int nlines;
   int job;           // index into the launch's DevJob table
   int wg_off;        // first workgroup of this entry within the launch
@@ -98,7 +109,7 @@ struct TickArgs {
 hipError_t launch_line_nnz(const DevJob& J, int* line_nnz, float* x, hipStream_t s);
 hipError_t launch_forward_project(const DevJob& J, const float* vol, double* out, hipStream_t s);
 hipError_t launch_init_error(const DevJob& J, const double* yest, hipStream_t s);
-hipError_t launch_icd_step(const DevJob& J, const int* lines, int nlines, int outer, int inner,
+hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, int outer, int inner,
                            int zero_skipping, SweepAccum* acc, hipStream_t s);
 void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv);
 // one step of several jobs of one kernel shape: entries [e0, e0 + ne) of ent,

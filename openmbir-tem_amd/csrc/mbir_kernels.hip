@@ -293,17 +293,20 @@ __global__ void k_column_weights(DevJob J, float* ka, float* kw, int nnz)
 // Slices y are independent in the data term (disjoint VLR rows); the prior
 // couples y and y +- 1, so even slices are solved first and odd slices second,
 // which makes one line update equal to a sequential order of its voxels.
-//   1. the 3x3 block of lines around (z, x) is staged in LDS (coalesced along y);
-//   2. the column is built in FP64, one thread per view, and compacted into LDS
-//      with a wave scan;
-//   3. theta1/theta2: every wave streams a share of the column's (theta, r)
+//   1./2./3. one memory round trip brings the 3x3 block of lines around (z, x)
+//      and the line's column (from the table built at setup) into LDS, and the
+//      sinogram rows of each lane's first theta batch, whose error-sinogram
+//      loads are then in flight while LDS is written;
+//   4. theta1/theta2: every wave streams a share of the column's (theta, r)
 //      detector rows, lanes running along the contiguous row t, U rows in flight
-//      per lane, Kahan-compensated fp32 partial sums combined in FP64;
-//   4. the voxel solve of each parity runs on lane groups: the 26 neighbour
-//      terms of one voxel are spread over G lanes and combined by shuffles;
-//   5. e -= I0 a vlr dx: plain read-modify-write, since no two lines of a
-//      conflict-free step share a detector bin (fp32 atomics for the relaxed
-//      schedule, whose lines may share bins).
+//      per lane, Kahan-compensated fp32 partial sums combined in FP64 (lane
+//      shuffles within a wave, LDS across waves);
+//   5. the voxel solve of each parity runs on lane groups: the 26 neighbour
+//      terms of one voxel are spread over G lanes and combined by DPP moves;
+//   6. e -= I0 a vlr dx: fp32 atomics, or a plain read-modify-write where the
+//      caller asks for it (no two lines of a conflict-free step share a detector
+//      bin, so both give the same e; under a full device the plain form is faster);
+//   7. per-slice statistics reduced by one wave, one set of global atomics.
 template <int YPL>
 struct IcdSmem {
   float ka[kMaxSlots];   // I0[th] * a of the line's column entries
@@ -313,137 +316,129 @@ struct IcdSmem {
   float x[kMaxNy];       // this line; the even phase writes its new values here
   float dx[kMaxNy];
   double T1[kMaxNy], T2[kMaxNy];
-  unsigned char cand[kMaxNy];
-  unsigned char need[kMaxNy];
-  // theta partial sums: (8 waves x 64/TY rows) x (TY x YPL slices) = kIcdBlock x YPL
-  float p1h[kIcdBlock * YPL], p1l[kIcdBlock * YPL], p2h[kIcdBlock * YPL], p2l[kIcdBlock * YPL];
-  double st_su, st_sv;
-  unsigned long long st_nn;
-  unsigned int st_nz, st_nu;
+  // theta partial sums per wave and slice (FP64)
+  double pd1[kIcdWaves][64 * YPL], pd2[kIcdWaves][64 * YPL];
+  double su[kMaxNy], sv[kMaxNy];  // per slice: ROI |x_new - x_old|, |x_old|
+  unsigned char vst[kMaxNy];      // per slice: 1 updated, 2 zero-skipped
+  unsigned char vns[kMaxNy];      // per slice: detector rows (VLR size)
 };
 
-template <int YPL, int NV, bool FW>
-__device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepAccum* acc,
-                         IcdSmem<YPL>& sm)
+// PF: issue the first theta batch's loads before the LDS staging (shorter line
+// latency, more registers: the form for launches that do not fill the device).
+template <int YPL, int NV, bool FW, bool PF>
+__device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_on,
+                         SweepAccum* acc, IcdSmem<YPL>& sm)
 {
   constexpr int U = YPL == 1 ? 24 : (YPL == 2 ? 12 : 6);  // detector rows in flight per lane
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int Ny = J.N_y;
+  const int line = __builtin_amdgcn_readfirstlane(ref.line);
   const int z = line / J.N_x, xi = line % J.N_x;
+  const bool roi = J.mask[line] != 0;  // used by the solve
   const size_t lbase = (size_t)line * Ny;
   const bool zs_active = zs_on && !first;
 
-  // ---- 1. the 3x3 neighbour lines (branch-free loads from clamped addresses) ----
-  const int W = Ny + 2;
-  constexpr int kNbIt = (9 * (kMaxNy + 2) + kIcdBlock - 1) / kIcdBlock;
-  float nbv_[kNbIt];
-#pragma unroll
-  for (int it = 0; it < kNbIt; ++it) {  // all loads first, stores after
-    const int q = tid + it * kIcdBlock;
-    const int l = q / W, yy = q - l * W - 1;
-    const int zz = z + l / 3 - 1, xx = xi + l % 3 - 1;
-    const bool in = q < 9 * W && yy >= 0 && yy < Ny && zz >= 0 && zz < J.N_z && xx >= 0 &&
-                    xx < J.N_x;
-    const float val = J.x[in ? ((size_t)zz * J.N_x + xx) * Ny + yy : 0];
-    nbv_[it] = in ? val : 0.f;
-  }
-#pragma unroll
-  for (int it = 0; it < kNbIt; ++it) {
-    const int q = tid + it * kIcdBlock;
-    const int l = q / W, yy = q - l * W - 1;
-    if (q < 9 * W) sm.nbl[l][yy + 1] = nbv_[it];
-  }
-  if (tid == 0) {
-    sm.st_su = 0.0;
-    sm.st_sv = 0.0;
-    sm.st_nn = 0ull;
-    sm.st_nz = 0u;
-    sm.st_nu = 0u;
-  }
-
-  // ---- 2. the line's column from the table, into LDS ----
-  const int p0 = J.col_ptr[line];
-  const int nc = J.col_ptr[line + 1] - p0;
-#pragma unroll
-  for (int it = 0; it < kMaxSlots / kIcdBlock; ++it) {  // loads first, stores after
-    const int i = tid + it * kIcdBlock;
-    const int ic = p0 + (i < nc ? i : 0);
-    const int rw = J.col_row[ic];
-    const float ka = J.col_ka[ic], kw = J.col_kw[ic];
-    if (i < nc) {
-      sm.row[i] = rw;
-      sm.ka[i] = ka;
-      sm.kw[i] = kw;
-    }
-  }
-  __syncthreads();  // A: neighbour lines and column visible
-  if (tid < Ny) {
-    const float v = sm.nbl[4][tid + 1];
-    sm.x[tid] = v;
-    // zero-skip candidate with the pre-update neighbourhood (:311-338)
-    bool cand = false;
-    if (zs_active && v == 0.f) {
-      cand = true;
-#pragma unroll
-      for (int l = 0; l < 9; ++l)
-#pragma unroll
-        for (int dy = -1; dy <= 1; ++dy)
-          if (!(l == 4 && dy == 0) && sm.nbl[l][tid + 1 + dy] > 0.f) cand = false;
-    }
-    sm.cand[tid] = cand;
-  }
-  __syncthreads();  // B: column, line copy and candidates visible
-  if (tid < Ny) {
-    bool need = !sm.cand[tid];
-    if ((tid & 1) && !need) {  // odd slices see the even slices' new values
-      if (tid - 1 >= 0 && !sm.cand[tid - 1]) need = true;
-      if (tid + 1 < Ny && !sm.cand[tid + 1]) need = true;
-    }
-    sm.need[tid] = need;
-  }
-  __syncthreads();  // C
-
-  // ---- 3. theta1 / theta2 ----
+  // ---- 1. this lane's slices for the theta and update passes: lanes run along the
+  // TY-wide (power of two >= N_y, at most 64) slice index, 64 / TY column entries per wave ----
   int TY = 1;
   while (TY < Ny && TY < 64) TY <<= 1;
   const int RPW = 64 / TY, sub = lane / TY, yl = lane % TY;
   const int P = kIcdWaves * RPW, p = wave * RPW + sub;
-  Kahan t1[YPL], t2[YPL];
   int vt0[YPL], vn[YPL];
   float vv[YPL][NV];
   bool act[YPL];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
     const int y = yl + TY * jy;
-    act[jy] = y < Ny && sm.need[y];
-    vt0[jy] = 0;
-    vn[jy] = 0;
+    act[jy] = y < Ny;
+    const int yc = act[jy] ? y : 0;
+    vt0[jy] = J.vlr_t0[yc];
+    vn[jy] = act[jy] ? J.vlr_n[yc] : 0;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) vv[jy][k] = 0.f;
-    if (y < Ny) {
-      vt0[jy] = J.vlr_t0[y];
-      vn[jy] = J.vlr_n[y];
+    for (int k = 0; k < NV; ++k) vv[jy][k] = J.vlr_val[yc * kVlrCap + k];
+  }
+
+  // ---- 2. one round trip: the 3x3 neighbour lines, the column table (for LDS) and
+  // the sinogram rows of this lane's first theta batch ----
+  const int W = Ny + 2, nnb = 9 * W;
+  const int p0 = __builtin_amdgcn_readfirstlane(ref.p0);
+  const int nc = __builtin_amdgcn_readfirstlane(ref.nc);
+  constexpr int kNbIt = (9 * (kMaxNy + 2) + kIcdBlock - 1) / kIcdBlock;
+  constexpr int kColIt = kMaxSlots / kIcdBlock;
+  auto entry = [&](int i) { return nc > 0 ? p0 + (i < nc ? i : nc - 1) : 0; };
+  float nbv_[kNbIt];
+  int rw_[kColIt];
+  float ka_[kColIt], kw_[kColIt];
 #pragma unroll
-      for (int k = 0; k < NV; ++k) vv[jy][k] = J.vlr_val[y * kVlrCap + k];
+  for (int it = 0; it < kNbIt; ++it) {
+    if (it * kIcdBlock < nnb) {  // uniform
+      const int q = tid + it * kIcdBlock;
+      const int l = q / W, yy = q - l * W - 1;
+      const int zz = z + l / 3 - 1, xx = xi + l % 3 - 1;
+      const bool in = q < nnb && yy >= 0 && yy < Ny && zz >= 0 && zz < J.N_z && xx >= 0 &&
+                      xx < J.N_x;
+      const float val = J.x[in ? ((size_t)zz * J.N_x + xx) * Ny + yy : 0];
+      nbv_[it] = in ? val : 0.f;
     }
   }
-  for (int base = p; base < nc; base += P * U) {
-    float ev[U][YPL][NV], wf[FW ? U : 1][YPL][NV];
+#pragma unroll
+  for (int it = 0; it < kColIt; ++it) {
+    if (it * kIcdBlock < nc) {  // uniform
+      const int ic = entry(tid + it * kIcdBlock);
+      rw_[it] = J.col_row[ic];
+      ka_[it] = J.col_ka[ic];
+      kw_[it] = J.col_kw[ic];
+    }
+  }
+  int r0[PF ? U : 1];
+  if constexpr (PF) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) r0[u] = J.col_row[entry(p + u * P)];
+  }
+  const int vn_t = J.vlr_n[tid < Ny ? tid : 0];
+#pragma unroll
+  for (int it = 0; it < kNbIt; ++it) {
+    const int q = tid + it * kIcdBlock;
+    const int l = q / W, yy = q - l * W - 1;
+    if (it * kIcdBlock < nnb && q < nnb) sm.nbl[l][yy + 1] = nbv_[it];
+  }
+#pragma unroll
+  for (int it = 0; it < kColIt; ++it) {
+    const int i = tid + it * kIcdBlock;
+    if (it * kIcdBlock < nc && i < nc) {
+      sm.row[i] = rw_[it];
+      sm.ka[i] = ka_[it];
+      sm.kw[i] = kw_[it];
+    }
+  }
+  // ---- 3. the first theta batch's error-sinogram loads, in flight across the
+  // barrier below ----
+  float ev0[PF ? U : 1][YPL][NV], wf0[PF && FW ? U : 1][YPL][NV];
+  if constexpr (PF) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int i = base + u * P;
-      const bool ok = i < nc;
-      const int rw = sm.row[ok ? i : 0];
+      const bool ok = p + u * P < nc;
 #pragma unroll
       for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
           const bool use = ok && act[jy] && k < vn[jy];
-          const int idx = use ? rw + vt0[jy] + k : 0;
-          ev[u][jy][k] = J.e[idx];
-          if constexpr (FW) wf[u][jy][k] = J.wfull[idx];
+          const int idx = use ? r0[u] + vt0[jy] + k : 0;
+          ev0[u][jy][k] = J.e[idx];
+          if constexpr (FW) wf0[u][jy][k] = J.wfull[idx];
         }
     }
+  }
+  __syncthreads();  // A: neighbour lines and column visible
+  if (tid < Ny) {  // read after barrier D
+    sm.x[tid] = sm.nbl[4][tid + 1];
+    sm.vns[tid] = vn_t;
+  }
+
+  // ---- 4. theta1 / theta2 (:343-369): every slice of the line (a zero-skipped
+  // slice's sums are simply unused) ----
+  Kahan t1[YPL], t2[YPL];
+  auto accumulate = [&](int base, const auto& ev, const auto& wf) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int i = base + u * P;
@@ -465,30 +460,65 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
             }
       }
     }
+  };
+  int base0 = p;
+  if constexpr (PF) {
+    accumulate(p, ev0, wf0);
+    base0 = p + P * U;
   }
+  for (int base = base0; base < nc; base += P * U) {
+    float ev[U][YPL][NV], wf[FW ? U : 1][YPL][NV];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * P;
+      const bool ok = i < nc;
+      const int rw = sm.row[ok ? i : 0];
+#pragma unroll
+      for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          const bool use = ok && act[jy] && k < vn[jy];
+          const int idx = use ? rw + vt0[jy] + k : 0;
+          ev[u][jy][k] = J.e[idx];
+          if constexpr (FW) wf[u][jy][k] = J.wfull[idx];
+        }
+    }
+    accumulate(base, ev, wf);
+  }
+  // FP64 partials, summed first over the rows of a wave that hold the same slice
+  // (lanes yl, yl + TY, ...), then over the waves
+  double d1[YPL], d2[YPL];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
-    const int o = p * (TY * YPL) + yl + TY * jy;
-    sm.p1h[o] = t1[jy].s;
-    sm.p1l[o] = -t1[jy].c;
-    sm.p2h[o] = t2[jy].s;
-    sm.p2l[o] = -t2[jy].c;
+    d1[jy] = (double)t1[jy].s - (double)t1[jy].c;
+    d2[jy] = (double)t2[jy].s - (double)t2[jy].c;
   }
+  for (int m = TY; m < 64; m <<= 1)
+#pragma unroll
+    for (int jy = 0; jy < YPL; ++jy) {
+      d1[jy] += __shfl_xor(d1[jy], m);
+      d2[jy] += __shfl_xor(d2[jy], m);
+    }
+  if (sub == 0)
+#pragma unroll
+    for (int jy = 0; jy < YPL; ++jy) {
+      sm.pd1[wave][yl + TY * jy] = d1[jy];
+      sm.pd2[wave][yl + TY * jy] = d2[jy];
+    }
   __syncthreads();  // D
   if (tid < Ny) {
     double T1 = 0.0, T2 = 0.0;
-    for (int q = 0; q < P; ++q) {
-      const int o = q * (TY * YPL) + tid;
-      T1 += (double)sm.p1h[o] + (double)sm.p1l[o];
-      T2 += (double)sm.p2h[o] + (double)sm.p2l[o];
+#pragma unroll
+    for (int w = 0; w < kIcdWaves; ++w) {
+      T1 += sm.pd1[w][tid];
+      T2 += sm.pd2[w][tid];
     }
     sm.T1[tid] = -T1;  // THETA1 *= -1 (:369)
     sm.T2[tid] = T2;
   }
   __syncthreads();  // E
 
-  // ---- 4. voxel solve on lane groups, even slices then odd slices ----
-  const bool roi = J.mask[line] != 0;
+  // ---- 5. voxel solve on lane groups, even slices then odd slices ----
   for (int phase = 0; phase < 2; ++phase) {
     const int vpp = (Ny + 1 - phase) / 2;  // slices of this parity
     int G = 16;                            // lanes per voxel (one DPP row at most)
@@ -520,8 +550,10 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
     const bool anypos = hi > 0.f;
     if (active && gl == 0) {
       float xnew = v;
+      double su = 0.0, sv = 0.0;
+      unsigned char vst = 2;
       if (zs_active && v == 0.f && !anypos) {
-        atomicAdd(&sm.st_nz, 1u);
+        // zero-skipped (:311-338)
       } else {
         const double T1 = sm.T1[y], T2 = sm.T2[y];
         double dlo = lo, dhi = hi;
@@ -536,20 +568,22 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
         if (nv < 0.0) nv = 0.0;  // POSITIVITY_CONSTRAINT (:394-400)
         xnew = (float)nv;
         J.x[lbase + y] = xnew;
-        atomicAdd(&sm.st_nu, 1u);
-        atomicAdd(&sm.st_nn, (unsigned long long)nc * (unsigned long long)J.vlr_n[y]);
+        vst = 1;
         if (roi) {
-          atomicAdd(&sm.st_su, fabs((double)xnew - (double)v));
-          atomicAdd(&sm.st_sv, fabs((double)v));
+          su = fabs((double)xnew - (double)v);
+          sv = fabs((double)v);
         }
       }
+      sm.su[y] = su;
+      sm.sv[y] = sv;
+      sm.vst[y] = vst;
       sm.x[y] = xnew;
       sm.dx[y] = xnew - v;
     }
     __syncthreads();  // F, G
   }
 
-  // ---- 5. e -= I0 a vlr dx (:423-449) ----
+  // ---- 6. e -= I0 a vlr dx (:423-449) ----
   float d[YPL];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
@@ -569,12 +603,13 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
 #pragma unroll
             for (int k = 0; k < NV; ++k)
               if (d[jy] != 0.f && k < vn[jy])
-                unsafeAtomicAdd(J.e + rwi + vt0[jy] + k, -(ka * vv[jy][k] * d[jy]));
+                unsafeAtomicAdd(J.e + rwi + vt0[jy] + k,
+                                -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy]));
         }
       }
     }
   } else {
-    // every load of a batch in flight before its stores; e - v equals e + (-v)
+    // every load of a batch in flight before its stores
     for (int base = p; base < nc; base += P * U) {
       float ev[U][YPL][NV];
 #pragma unroll
@@ -601,38 +636,60 @@ __device__ void icd_line(const DevJob& J, int line, int first, int zs_on, SweepA
 #pragma unroll
             for (int k = 0; k < NV; ++k)
               if (d[jy] != 0.f && k < vn[jy])
-                J.e[rwi + vt0[jy] + k] = ev[u][jy][k] - ka * vv[jy][k] * d[jy];
+                // the atomic form's roundings: product first, then one add (no fma)
+                J.e[rwi + vt0[jy] + k] =
+                    __fadd_rn(ev[u][jy][k], -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy]));
         }
       }
     }
   }
 
-  // ---- 6. sweep statistics: one set of global atomics per line ----
-  if (tid == 0) {
-    if (sm.st_su != 0.0) atomicAdd(&acc->sum_abs_update, sm.st_su);
-    if (sm.st_sv != 0.0) atomicAdd(&acc->sum_abs_value, sm.st_sv);
-    if (sm.st_nz) atomicAdd(&acc->zero_skipped, (unsigned long long)sm.st_nz);
-    if (sm.st_nu) atomicAdd(&acc->updated, (unsigned long long)sm.st_nu);
-    if (sm.st_nn) atomicAdd(&acc->nnz_updated, sm.st_nn);
+  // ---- 7. sweep statistics (:412-421): the first wave reduces the per-slice
+  // values (visible since the last solve barrier), one set of global atomics per line ----
+  if (wave == 0) {
+    double su = 0.0, sv = 0.0;
+    int nz = 0, nu = 0, nn = 0;
+    for (int y = lane; y < Ny; y += 64) {
+      const int f = sm.vst[y];
+      su += sm.su[y];
+      sv += sm.sv[y];
+      nz += f == 2;
+      nu += f == 1;
+      nn += f == 1 ? nc * (int)sm.vns[y] : 0;
+    }
+    for (int m = 32; m > 0; m >>= 1) {
+      su += __shfl_xor(su, m);
+      sv += __shfl_xor(sv, m);
+      nz += __shfl_xor(nz, m);
+      nu += __shfl_xor(nu, m);
+      nn += __shfl_xor(nn, m);
+    }
+    if (lane == 0) {
+      if (su != 0.0) atomicAdd(&acc->sum_abs_update, su);
+      if (sv != 0.0) atomicAdd(&acc->sum_abs_value, sv);
+      if (nz) atomicAdd(&acc->zero_skipped, (unsigned long long)nz);
+      if (nu) atomicAdd(&acc->updated, (unsigned long long)nu);
+      if (nn) atomicAdd(&acc->nnz_updated, (unsigned long long)nn);
+    }
   }
   __syncthreads();  // H: LDS may be reused by the next line
 }
 
 template <int YPL, int NV>
-__global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const int* lines, int nlines,
+__global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef* lines, int nlines,
                                                         int first, int zs_on, SweepAccum* acc)
 {
   __shared__ IcdSmem<YPL> sm;
   if ((int)blockIdx.x >= nlines) return;
-  const int line = lines[blockIdx.x];
-  icd_line<YPL, NV, false>(J, line, first, zs_on, acc, sm);
+  const LineRef ref = lines[blockIdx.x];
+  icd_line<YPL, NV, false, true>(J, ref, first, zs_on, acc, sm);
 }
 
 // One conflict-free step of every job of a batch that has kernel shape
 // (YPL, NV): entries [e0, e0 + ne) of the sweep's entry table; workgroup g serves
 // entry e with wg_off[e] <= g < wg_off[e] + nlines[e].  Jobs never share memory,
 // so one launch needs no ordering between them.
-template <int YPL, int NV>
+template <int YPL, int NV, bool PF>
 __global__ __launch_bounds__(kIcdBlock) void k_icd_batch_step(const DevJob* __restrict__ jobs,
                                                               const BatchEntry* __restrict__ ent,
                                                               int e0, int ne)
@@ -650,14 +707,14 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_batch_step(const DevJob* __re
   const int i = g - E.wg_off;
   if (i < 0 || i >= E.nlines) return;
   const DevJob J = jobs[__builtin_amdgcn_readfirstlane(E.job)];
-  const int line = __builtin_amdgcn_readfirstlane(E.lines[i]);
-  icd_line<YPL, NV, false>(J, line, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm);
+  const LineRef ref = E.lines[i];
+  icd_line<YPL, NV, false, PF>(J, ref, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm);
 }
 
 // One launch of the request server: entries travel by value in the kernel
 // arguments (no host-to-device copy per launch), each naming a job slot and a
 // contiguous run of its current sweep's schedule.
-template <int YPL, int NV>
+template <int YPL, int NV, bool PF>
 __global__ __launch_bounds__(kIcdBlock) void k_icd_tick(const DevJob* __restrict__ jobs,
                                                         const TickArgs args)
 {
@@ -679,8 +736,8 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_tick(const DevJob* __restrict
   const int flags = __builtin_amdgcn_readfirstlane(args.e[lo].flags);
   if (i < 0 || i >= nl) return;
   const DevJob J = jobs[job];
-  const int line = __builtin_amdgcn_readfirstlane(J.sched[l0 + i]);
-  icd_line<YPL, NV, false>(J, line, flags & 1, (flags >> 1) & 1, J.acc, sm);
+  const LineRef ref = J.sched[l0 + i];
+  icd_line<YPL, NV, false, PF>(J, ref, flags & 1, (flags >> 1) & 1, J.acc, sm);
 }
 
 // jobs[slot] = J: a job descriptor written in stream order.
@@ -931,7 +988,7 @@ void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv)
 }
 
 template <int YPL>
-static void launch_step_nv(const DevJob& J, const int* lines, int nlines, int first, int zs,
+static void launch_step_nv(const DevJob& J, const LineRef* lines, int nlines, int first, int zs,
                            SweepAccum* acc, hipStream_t s)
 {
   if (J.vlr_max <= 1)
@@ -945,7 +1002,7 @@ static void launch_step_nv(const DevJob& J, const int* lines, int nlines, int fi
                        first, zs, acc);
 }
 
-hipError_t launch_icd_step(const DevJob& J, const int* lines, int nlines, int outer, int inner,
+hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, int outer, int inner,
                            int zero_skipping, SweepAccum* acc, hipStream_t s)
 {
   if (nlines <= 0) return hipSuccess;
@@ -961,18 +1018,32 @@ hipError_t launch_icd_step(const DevJob& J, const int* lines, int nlines, int ou
   return hipGetLastError();
 }
 
+// Launches that fill the device take the higher-occupancy form of the line update,
+// smaller ones the shorter-latency form (both compute the same values).
+constexpr int kFullDeviceWg = 512;
+
 hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const BatchEntry* ent,
                                  int e0, int ne, int nwg, hipStream_t s)
 {
   if (ne <= 0 || nwg <= 0) return hipSuccess;
-  if (ypl == 1 && nv == 1)
-    hipLaunchKernelGGL((k_icd_batch_step<1, 1>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, ent, e0, ne);
-  else if (ypl == 1)
-    hipLaunchKernelGGL((k_icd_batch_step<1, 2>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, ent, e0, ne);
-  else if (nv == 1)
-    hipLaunchKernelGGL((k_icd_batch_step<2, 1>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, ent, e0, ne);
-  else
-    hipLaunchKernelGGL((k_icd_batch_step<2, 2>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, ent, e0, ne);
+  const bool pf = nwg <= kFullDeviceWg;
+#define MBIR_BATCH(Y, N)                                                                      \
+  if (pf)                                                                                     \
+    hipLaunchKernelGGL((k_icd_batch_step<Y, N, true>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, \
+                       ent, e0, ne);                                                          \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_icd_batch_step<Y, N, false>), dim3(nwg), dim3(kIcdBlock), 0, s,     \
+                       jobs, ent, e0, ne);
+  if (ypl == 1 && nv == 1) {
+    MBIR_BATCH(1, 1)
+  } else if (ypl == 1) {
+    MBIR_BATCH(1, 2)
+  } else if (nv == 1) {
+    MBIR_BATCH(2, 1)
+  } else {
+    MBIR_BATCH(2, 2)
+  }
+#undef MBIR_BATCH
   return hipGetLastError();
 }
 
@@ -980,14 +1051,22 @@ hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& 
                            hipStream_t s)
 {
   if (args.n <= 0 || args.n > kTickMax || nwg <= 0) return hipSuccess;
-  if (ypl == 1 && nv == 1)
-    hipLaunchKernelGGL((k_icd_tick<1, 1>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args);
-  else if (ypl == 1)
-    hipLaunchKernelGGL((k_icd_tick<1, 2>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args);
-  else if (nv == 1)
-    hipLaunchKernelGGL((k_icd_tick<2, 1>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args);
-  else
-    hipLaunchKernelGGL((k_icd_tick<2, 2>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args);
+  const bool pf = nwg <= kFullDeviceWg;
+#define MBIR_TICK(Y, N)                                                                        \
+  if (pf)                                                                                      \
+    hipLaunchKernelGGL((k_icd_tick<Y, N, true>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args); \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_icd_tick<Y, N, false>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args);
+  if (ypl == 1 && nv == 1) {
+    MBIR_TICK(1, 1)
+  } else if (ypl == 1) {
+    MBIR_TICK(1, 2)
+  } else if (nv == 1) {
+    MBIR_TICK(2, 1)
+  } else {
+    MBIR_TICK(2, 2)
+  }
+#undef MBIR_TICK
   return hipGetLastError();
 }
 

@@ -49,7 +49,7 @@ struct Req {
   bool finishing = false;  // final nuisance update in flight
   int ypl = 1, nv = 1;
   SweepAccum* hacc = nullptr;  // pinned
-  int* hsched = nullptr;       // pinned copy of the current sweep's schedule
+  LineRef* hsched = nullptr;   // pinned copy of the current sweep's schedule
   hipEvent_t ev = nullptr;
   int64_t visits = 0, updated = 0;
   double t_submit = 0, t_admit = 0, t_start = 0, t_done = 0;
@@ -65,6 +65,15 @@ int64_t estimate_bytes(const mbir_problem& p)
   const int64_t NB = (int64_t)std::ceil(2.0 * g.delta_xz / g.delta_r - 1e-12) + 1;
   // x; e, y; FP64 projection during setup; per-line arrays; column table bound
   return V * 4 + M * 8 + M * 8 + L * 21 + L * g.N_theta * NB * 16 + (1 << 20);
+}
+
+// The job's descriptor for the server's launches: under a full device the plain
+// error-sinogram update is the faster form (conflict-free schedules only).
+DevJob tick_desc(const mbir_job* j)
+{
+  DevJob d = j->J;
+  d.e_atomic = j->prob.schedule == 1 ? 1 : 0;
+  return d;
 }
 
 }  // namespace
@@ -136,7 +145,7 @@ void mbir_server::setup_loop()
     int rc = mbir_job_create(ctx, &r->prob, &job);
     const size_t L = rc == 0 ? (size_t)job->g.N_z * job->g.N_x : 0;
     if (rc == 0 && (hipHostMalloc((void**)&r->hacc, sizeof(SweepAccum)) != hipSuccess ||
-                    hipHostMalloc((void**)&r->hsched, std::max<size_t>(L, 1) * sizeof(int)) !=
+                    hipHostMalloc((void**)&r->hsched, std::max<size_t>(L, 1) * sizeof(LineRef)) !=
                         hipSuccess ||
                     hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess))
       rc = MBIR_ERR_HIP;
@@ -189,7 +198,8 @@ int mbir_server::start_sweep(Req* r)
 {
   mbir_job* j = r->job;
   build_schedule(j, true);
-  if (!j->sched_h.empty()) std::memcpy(r->hsched, j->sched_h.data(), j->sched_h.size() * sizeof(int));
+  if (!j->sched_h.empty())
+    std::memcpy(r->hsched, j->sched_h.data(), j->sched_h.size() * sizeof(LineRef));
   int rc = prepare_sweep_on(j, stream, r->hsched);
   if (rc < 0) return rc;
   r->step = 0;
@@ -227,7 +237,7 @@ int mbir_server::end_sweep(Req* r)
   const int outer = r->outer;
   if (j->prob.joint_estimation || j->prob.noise_model) {
     if (nuisance_update_on(j, stream) < 0) return -1;
-    if (launch_set_devjob(djobs.p, r->slot, j->J, stream) != hipSuccess) return -1;
+    if (launch_set_devjob(djobs.p, r->slot, tick_desc(j), stream) != hipSuccess) return -1;
   }
   r->outer++;
   r->outer_done = r->outer;
@@ -279,7 +289,7 @@ void mbir_server::run_loop()
       lk.unlock();
       bool bad = false;
       for (Req* r : fresh) {
-        if (launch_set_devjob(djobs.p, r->slot, r->job->J, stream) != hipSuccess ||
+        if (launch_set_devjob(djobs.p, r->slot, tick_desc(r->job), stream) != hipSuccess ||
             start_sweep(r) < 0)
           bad = true;
       }

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <iomanip>
 #include <vector>
 
 #include "HAADF_ReconstructionEngine.h"
@@ -60,6 +61,7 @@ int main(int argc, char** argv)
   eng->execute();
   const std::string pre = argv[14];
   std::ofstream txt(pre + ".txt");
+  txt << std::setprecision(17);
   txt << "error " << eng->getErrorCondition() << "\n";
   txt << "dims " << geom->N_z << " " << geom->N_x << " " << geom->N_y << "\n";
   txt << "sweeps " << eng->getSweeps() << "\n";

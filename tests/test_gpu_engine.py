@@ -36,34 +36,46 @@ def run_cli(y, ang, n_r, n_t, thickness, sigma, q, outer, inner, joint, noise, s
         return vol, nu, costs
 
 
-def test_adapter_execute_matches_oracle():
-    """30 outer x 10 inner with joint gain/offset and noise estimation.  The ICD
-    update order is random and the joint problem is not convex, so runs that differ
-    only in seed end at different points: over seeds 1-8 the oracle's volumes differ
-    by 2.5-4 % RMS and its final costs (about -20387) by a standard deviation of 9
-    and a range of 29 (1.4e-3 relative).  The adapter must sit within twice the
-    volume spread and 2e-3 of the oracle's cost."""
+def test_adapter_execute_matches_oracle(gpu_ctx):
+    """30 outer x 10 inner with joint gain/offset and noise estimation.  The adapter
+    must give exactly the C-ABI's result for the same problem and seed.  Against the
+    oracle: the ICD update order is random and the joint problem is not convex, so
+    runs that differ only in seed end at different points (oracle volumes 2.5-4 % RMS
+    apart, final costs about -20387 with a standard deviation of 9).  The largest
+    distance of four oracle runs from their mean is the scatter; oracle runs with
+    other seeds lie up to 2.1 scatters from that mean, and the GPU volume must lie
+    within 3.  Its cost must be within 2e-3 of the oracle runs' mean cost."""
     rng = np.random.default_rng(11)
     nv = 31
     g, ang, vol, y = small_problem(n_views=nv, gains=rng.uniform(0.85, 1.15, nv),
                                    offsets=rng.uniform(-0.04, 0.04, nv))
     xg, nug, costs = run_cli(y, ang, g.N_r, g.N_t, g.LengthZ / 2, 0.1, 1.2, 30, 10, 1, 1, 5)
     kw = dict(num_outer=30, num_inner=10, joint_estimation=1, noise_model=1, stop_threshold=1e-4)
-    eng = O.Engine(g, ang, y, O.params(**kw))
-    cr, _ = eng.run()
-    eng2 = O.Engine(g, ang, y, O.params(seed=2, **kw))
-    eng2.run()
-    I0, mu, al = eng.nuisance()
-    np.testing.assert_allclose(nug[0], I0, rtol=1e-2)
-    np.testing.assert_allclose(nug[1], mu, atol=5e-3)
-    xr = eng.x()
+    job = capi.Job(gpu_ctx, capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2), ang, y,
+                   seed=5, **kw)
+    rs, cj = job.run()
+    np.testing.assert_array_equal(xg, job.volume())
+    for a, b in zip(nug, job.nuisance()):
+        np.testing.assert_array_equal(a, b)
+    assert costs[-1] == pytest.approx(cj[-1], rel=1e-6)  # text round trip of the CLI
+
+    runs = []
+    for seed in (12345, 1, 2, 3):
+        eng = O.Engine(g, ang, y, O.params(seed=seed, **kw))
+        cr, _ = eng.run()
+        runs.append((eng.x(), cr[-1], eng.nuisance()))
+    xbar = np.mean([r[0] for r in runs], axis=0)
 
     def rms(a):
         return np.sqrt(np.mean(a ** 2))
 
-    spread = rms(xr - eng2.x())
-    assert rms(xr - xg) <= max(2.0 * spread, 1e-2 * rms(xr)), (rms(xr - xg), spread)
-    assert costs[-1] == pytest.approx(cr[-1], rel=2e-3)
+    spread = max(rms(r[0] - xbar) for r in runs)
+    assert rms(xg - xbar) <= 3.0 * spread, (rms(xg - xbar), spread)
+    assert costs[-1] == pytest.approx(np.mean([r[1] for r in runs]), rel=2e-3)
+    I0 = np.mean([r[2][0] for r in runs], axis=0)
+    mu = np.mean([r[2][1] for r in runs], axis=0)
+    np.testing.assert_allclose(nug[0], I0, rtol=1e-2)
+    np.testing.assert_allclose(nug[1], mu, atol=5e-3)
 
 
 def test_reference_shepp_logan_fixture(gpu_ctx):
