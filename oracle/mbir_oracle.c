@@ -435,6 +435,8 @@ struct orc_engine {
   double* init_variance;
   double stop_threshold;
   int32_t sweep_counter;
+  uint8_t* sel;    /* bright field: Bragg selector [N_theta][N_r][N_t] */
+  double bragg;    /* bright field: current Bragg threshold */
 };
 
 #define VLR_CAP 8
@@ -449,7 +451,7 @@ void orc_engine_destroy(orc_engine* e)
   free(e->H); free(e->Ht); free(e->vlr_count); free(e->vlr_t); free(e->vlr_val);
   free(e->col_ptr); free(e->col_idx); free(e->col_val); free(e->mask); free(e->x);
   free(e->counts); free(e->e); free(e->w); free(e->yest); free(e->I0); free(e->mu);
-  free(e->alpha); free(e->init_variance);
+  free(e->alpha); free(e->init_variance); free(e->sel);
   free(e);
 }
 
@@ -587,6 +589,13 @@ orc_engine* orc_engine_create(const orc_geom* g, const double* angles_deg, const
   /* forward projection (:646-684) and calculateMeasurementWeight (..._Extra.cpp:693-771) */
   e->counts = (double*)malloc(sizeof(double) * M);
   memcpy(e->counts, counts, sizeof(double) * M);
+  if (p->bf_mode) {
+    /* processRawCounts, BrightField/BFForwardModel.cpp:939-972 (run before the
+     * setup above in the reference; nothing above reads the counts) */
+    for (size_t i = 0; i < M; ++i) e->counts[i] = -log((e->counts[i] + p->bf_offset) / ORC_BF_MAX);
+    e->sel = (uint8_t*)malloc(M);
+    e->bragg = p->bragg_T;
+  }
   e->yest = (double*)malloc(sizeof(double) * M);
   e->e = (double*)malloc(sizeof(double) * M);
   e->w = (double*)malloc(sizeof(double) * M);
@@ -598,6 +607,11 @@ orc_engine* orc_engine_create(const orc_geom* g, const double* angles_deg, const
         size_t idx = ((size_t)it * g->N_r + ir) * g->N_t + t;
         e->e[idx] = e->counts[idx] - e->yest[idx] - e->mu[it];
         double wv = 1.0; /* IDENTITY_NOISE_MODEL, HAADF/HAADFConstants.h */
+        if (p->bf_mode) {
+          /* BF calculateMeasurementWeight, BFForwardModel.cpp:328-385 */
+          wv = ORC_BF_MAX / exp(e->counts[idx]);
+          e->sel[idx] = 1;
+        }
         if (p->noise_model) wv /= e->alpha[it];
         e->w[idx] = wv;
       }
@@ -654,8 +668,15 @@ static int update_voxel(orc_engine* e, int32_t j_new, int32_t k_new, int32_t i, 
     for (int k = 0; k < e->vlr_count[i]; ++k) {
       size_t eidx = ((size_t)it * g->N_r + ir) * g->N_t + e->vlr_t[i * VLR_CAP + k];
       double PE = kConst0 * e->vlr_val[i * VLR_CAP + k];
-      THETA2 += (PE * PE * e->w[eidx]);
-      THETA1 += (e->e[eidx] * PE * e->w[eidx]);
+      if (e->p.bf_mode && e->sel[eidx] == 0) {
+        /* rejected entry: BFForwardModel::computeTheta, :874-882 */
+        double QC = (e->p.bragg_delta * e->bragg) / (fabs(e->e[eidx]) * sqrt(e->w[eidx]));
+        THETA2 += QC * (PE * PE * e->w[eidx]);
+        THETA1 += QC * (e->e[eidx] * PE * e->w[eidx]);
+      } else {
+        THETA2 += (PE * PE * e->w[eidx]);
+        THETA1 += (e->e[eidx] * PE * e->w[eidx]);
+      }
     }
   }
   THETA1 *= -1;
@@ -685,6 +706,8 @@ static int update_voxel(orc_engine* e, int32_t j_new, int32_t k_new, int32_t i, 
       size_t eidx = ((size_t)it * g->N_r + ir) * g->N_t + e->vlr_t[i * VLR_CAP + k];
       double kConst2 = (e->I0[it] * (e->col_val[q] * e->vlr_val[i * VLR_CAP + k] * dx));
       e->e[eidx] -= kConst2;
+      if (e->p.bf_mode) /* selector refresh, BFForwardModel::updateErrorSinogram :924-929 */
+        e->sel[eidx] = fabs(e->e[eidx] * sqrt(e->w[eidx])) < e->bragg ? 1 : 0;
     }
   }
   return 1;
@@ -795,9 +818,72 @@ int32_t orc_engine_sweep_threads(orc_engine* e, int32_t outer, int32_t inner, in
   return finish_sweep(e, au, am, stats, zz, vv);
 }
 
+/* BF jointEstimation: per-view offset (log dose) only, BrightField/BFForwardModel.cpp:390-441 */
+static void bf_joint_estimation(orc_engine* e)
+{
+  const orc_geom* g = &e->g;
+  size_t per = (size_t)g->N_r * g->N_t;
+  const double Td = e->bragg * e->p.bragg_delta;
+  for (int it = 0; it < g->N_theta; ++it) {
+    double num_sum = 0, den_sum = 0;
+    for (size_t k = 0; k < per; ++k) {
+      size_t idx = it * per + k;
+      double ev = e->e[idx], w = e->w[idx];
+      if (e->sel[idx] == 1) {
+        num_sum += ev * w;
+        den_sum += w;
+      } else {
+        num_sum += (Td * ev / fabs(ev)) * sqrt(w);
+        den_sum += (Td * sqrt(w)) / fabs(ev);
+      }
+    }
+    double alpha = num_sum / den_sum;
+    for (size_t k = 0; k < per; ++k) e->e[it * per + k] -= alpha;
+    e->mu[it] += alpha;
+  }
+}
+
+/* updateSelector, BrightField/BFForwardModel.cpp:517-531 */
+void orc_engine_update_selector(orc_engine* e)
+{
+  size_t M = sino_size(&e->g);
+  for (size_t i = 0; i < M; ++i)
+    e->sel[i] = e->e[i] * e->e[i] * e->w[i] < e->bragg * e->bragg ? 1 : 0;
+}
+
+/* BF updateWeights: one variance for all views, BrightField/BFForwardModel.cpp:447-511 */
+static void bf_update_weights(orc_engine* e)
+{
+  const orc_geom* g = &e->g;
+  size_t per = (size_t)g->N_r * g->N_t;
+  for (int it = 0; it < g->N_theta; ++it)
+    for (size_t k = 0; k < per; ++k) e->w[it * per + k] *= e->alpha[it];
+  double sum1 = 0, sum2 = 0;
+  for (int it = 0; it < g->N_theta; ++it)
+    for (size_t k = 0; k < per; ++k) {
+      size_t idx = it * per + k;
+      if (e->sel[idx] == 1) sum1 += e->e[idx] * e->e[idx] * e->w[idx];
+      else sum2 += fabs(e->e[idx]) * sqrt(e->alpha[it] * e->w[idx]);
+    }
+  double update = (sum1 + (e->p.bragg_delta * e->bragg) * sum2) / ((double)g->N_theta * g->N_r * g->N_t);
+  for (int it = 0; it < g->N_theta; ++it) e->alpha[it] = update;
+  for (int it = 0; it < g->N_theta; ++it)
+    for (size_t k = 0; k < per; ++k) e->w[it * per + k] /= e->alpha[it];
+}
+
+double orc_bf_beta(double ev, double w, double T, double delta)
+{
+  if (ev * ev * w < T * T) return ev * ev * w;  /* the selector's test */
+  return 2 * T * delta * fabs(ev) * sqrt(w) + T * T * (1 - 2 * delta);
+}
+
 /* jointEstimation, gain + offset branch, HAADF/HAADF_ReconstructionEngine_Extra.cpp:429-637 */
 int32_t orc_engine_joint_estimation(orc_engine* e)
 {
+  if (e->p.bf_mode) {
+    bf_joint_estimation(e);
+    return 0;
+  }
   const orc_geom* g = &e->g;
   int NT = g->N_theta;
   size_t per = (size_t)g->N_r * g->N_t;
@@ -854,6 +940,10 @@ int32_t orc_engine_joint_estimation(orc_engine* e)
 /* updateWeights, IDENTITY_NOISE_MODEL branch, ..._Extra.cpp:795-881 */
 void orc_engine_update_weights(orc_engine* e)
 {
+  if (e->p.bf_mode) {
+    bf_update_weights(e);
+    return;
+  }
   const orc_geom* g = &e->g;
   size_t per = (size_t)g->N_r * g->N_t;
   for (int it = 0; it < g->N_theta; ++it) {
@@ -875,7 +965,17 @@ double orc_engine_cost_terms(orc_engine* e, double* terms3)
   const orc_geom* g = &e->g;
   size_t M = sino_size(g);
   double cost = 0;
-  for (size_t i = 0; i < M; ++i) cost += e->e[i] * e->e[i] * e->w[i];
+  if (e->p.bf_mode) {
+    /* BFForwardModel::forwardCost, :812-846 */
+    const double T = e->bragg, d = e->p.bragg_delta;
+    for (size_t i = 0; i < M; ++i) {
+      double ev = e->e[i];
+      if (e->sel[i] == 1) cost += ev * ev * e->w[i];
+      else cost += 2 * T * d * fabs(ev) * sqrt(e->w[i]) + T * T * (1 - 2 * d);
+    }
+  } else {
+    for (size_t i = 0; i < M; ++i) cost += e->e[i] * e->e[i] * e->w[i];
+  }
   cost /= 2;
   double data = cost;
   double temp = 0;
@@ -911,7 +1011,11 @@ double orc_engine_cost_terms(orc_engine* e, double* terms3)
 #undef XV
   cost += temp;
   double noise = 0;
-  if (e->p.noise_model) {
+  if (e->p.bf_mode && e->p.noise_model) {
+    /* (N_theta N_r N_t / 2) log(alpha[0]), integer halving as in :839 */
+    noise = (double)(M / 2) * log(e->alpha[0]);
+    cost += noise;
+  } else if (e->p.noise_model) {
     for (size_t i = 0; i < M; ++i)
       if (e->w[i] != 0) noise += log(2 * M_PI * (1.0 / e->w[i]));
     noise /= 2;
@@ -927,11 +1031,72 @@ double orc_engine_cost_terms(orc_engine* e, double* terms3)
 
 double orc_engine_cost(orc_engine* e) { return orc_engine_cost_terms(e, NULL); }
 
+/* BFReconstructionEngine::execute() loop, BrightField/BFReconstructionEngine.cpp:575-872,
+ * with regular (full, random-order) sweeps in place of the NHICD voxel lists:
+ * the Bragg threshold is DefBraggThreshold for the first outer iteration's inner
+ * sweeps when NumIter > 1, and the desired T after its last inner sweep; the stop
+ * test ends the run before the nuisance step; the nuisance step runs only when
+ * NumOuterIter > 1 (offset, then the shared variance, each followed by the selector). */
+static int32_t bf_run(orc_engine* e, double* costs, int32_t cap, int32_t* n_costs, int32_t* n_sweeps)
+{
+  int32_t nc = 0, ns = 0;
+#define PUSH_COST()                                       \
+  do {                                                    \
+    if (e->p.compute_cost) {                              \
+      double cv = orc_engine_cost(e);                     \
+      if (costs && nc < cap) costs[nc] = cv;              \
+      nc++;                                               \
+    }                                                     \
+  } while (0)
+  PUSH_COST();
+  int32_t num_iter = e->p.num_inner;
+  const double des = e->p.bragg_T;
+  double tmp = num_iter > 1 ? ORC_DEF_BRAGG : des;
+  e->bragg = tmp;
+  for (int outer = 0; outer < e->p.num_outer; ++outer) {
+    if (outer > 0) {
+      num_iter = 1;
+      e->bragg = tmp;
+    }
+    int32_t status = 1;
+    for (int inner = 0; inner < num_iter; ++inner) {
+      if (num_iter > 1 && inner == 0) e->bragg = ORC_DEF_BRAGG;
+      double st[4];
+      status = orc_engine_sweep(e, outer, inner, st);
+      ns++;
+      if (status == 0) break;
+      PUSH_COST();
+      if (inner == num_iter - 1 && inner != 0) {
+        tmp = des;
+        e->bragg = tmp;
+      }
+    }
+    if (status == 0 && outer >= 1) break;
+    if (e->p.num_outer > 1) {
+      if (e->p.joint_estimation) {
+        bf_joint_estimation(e);
+        orc_engine_update_selector(e);
+        PUSH_COST();
+      }
+      if (e->p.noise_model) {
+        bf_update_weights(e);
+        orc_engine_update_selector(e);
+        PUSH_COST();
+      }
+    }
+  }
+#undef PUSH_COST
+  if (n_costs) *n_costs = nc;
+  if (n_sweeps) *n_sweeps = ns;
+  return 0;
+}
+
 /* execute() outer/inner loop, HAADF/HAADF_ReconstructionEngine.cpp:700-815, with
  * COST_CALCULATE evaluations (:700-702, UpdateVoxels.cpp:750-763, :780-787). */
 int32_t orc_engine_run(orc_engine* e, double* costs, int32_t cap, int32_t* n_costs,
                        int32_t* n_sweeps)
 {
+  if (e->p.bf_mode) return bf_run(e, costs, cap, n_costs, n_sweeps);
   int32_t nc = 0, ns = 0;
 #define PUSH_COST()                                       \
   do {                                                    \
@@ -1014,10 +1179,20 @@ void orc_engine_theta(const orc_engine* e, int32_t z, int32_t x, int32_t y, doub
     for (int k = 0; k < e->vlr_count[y]; ++k) {
       size_t eidx = ((size_t)it * g->N_r + ir) * g->N_t + e->vlr_t[y * VLR_CAP + k];
       double PE = k0 * e->vlr_val[y * VLR_CAP + k];
-      T2 += PE * PE * e->w[eidx];
-      T1 += e->e[eidx] * PE * e->w[eidx];
+      double QC = 1.0;
+      if (e->p.bf_mode && e->sel[eidx] == 0)
+        QC = (e->p.bragg_delta * e->bragg) / (fabs(e->e[eidx]) * sqrt(e->w[eidx]));
+      T2 += QC * (PE * PE * e->w[eidx]);
+      T1 += QC * (e->e[eidx] * PE * e->w[eidx]);
     }
   }
   t1t2[0] = -T1;
   t1t2[1] = T2;
 }
+
+void orc_engine_get_sel(const orc_engine* e, uint8_t* sel)
+{
+  if (e->sel) memcpy(sel, e->sel, sino_size(&e->g));
+}
+double orc_engine_get_bragg(const orc_engine* e) { return e->bragg; }
+void orc_engine_set_bragg(orc_engine* e, double T) { e->bragg = T; }

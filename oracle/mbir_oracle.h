@@ -57,7 +57,16 @@ typedef struct orc_params {
   int32_t zero_skipping;    /* AdvancedParameters::ZERO_SKIPPING */
   int32_t compute_cost;     /* COST_CALCULATE */
   uint32_t seed;            /* order RNG seed (the reference seeds from the clock) */
+  /* bright-field mode (BrightField/BFForwardModel.cpp, BFReconstructionEngine.cpp):
+   * counts are raw transmitted counts; noise_model is NOISE_ESTIMATION there */
+  int32_t bf_mode;
+  double bragg_T;      /* Bragg threshold T (BFReconstructionArgsParser.cpp:176, default 3) */
+  double bragg_delta;  /* delta of the Bragg rejector (:179, default 0.5) */
+  double bf_offset;    /* added to the counts before the log (:182, default 0) */
 } orc_params;
+
+#define ORC_BF_MAX 1.0            /* BrightField/BFConstants.h */
+#define ORC_DEF_BRAGG 5e10        /* DefBraggThreshold, BrightField/BFConstants.h */
 
 /* ---- setup pieces (pure functions) ---- */
 int orc_geometry_init(int32_t N_r, int32_t N_t, int32_t N_theta, double delta_r, double delta_t,
@@ -125,6 +134,15 @@ int32_t orc_engine_get_col(const orc_engine* e, int32_t z, int32_t x, uint32_t* 
 void orc_engine_project(const orc_engine* e, const double* x, double* yest);
 /* theta1/theta2 of voxel (z,x,y) against the current e, w (reference :343-369) */
 void orc_engine_theta(const orc_engine* e, int32_t z, int32_t x, int32_t y, double* t1t2);
+
+/* bright-field pieces (BrightField/BFForwardModel.cpp) */
+void orc_engine_get_sel(const orc_engine* e, uint8_t* sel);
+double orc_engine_get_bragg(const orc_engine* e);
+void orc_engine_set_bragg(orc_engine* e, double T);
+void orc_engine_update_selector(orc_engine* e);     /* updateSelector, :517-531 */
+/* Bragg data term of one entry: e^2 w if |e sqrt(w)| < T, else
+ * 2 T delta |e| sqrt(w) + T^2 (1 - 2 delta)  (forwardCost, :812-846, before the 1/2) */
+double orc_bf_beta(double e, double w, double T, double delta);
 
 /* MT19937 (Matsumoto & Nishimura 1998), as boost::mt19937 / std::mt19937 */
 typedef struct orc_mt { uint32_t mt[624]; int32_t idx; } orc_mt;

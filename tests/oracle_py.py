@@ -41,6 +41,8 @@ class Params(C.Structure):
         ("num_outer", C.c_int32), ("num_inner", C.c_int32),
         ("joint_estimation", C.c_int32), ("noise_model", C.c_int32),
         ("zero_skipping", C.c_int32), ("compute_cost", C.c_int32), ("seed", C.c_uint32),
+        ("bf_mode", C.c_int32), ("bragg_T", C.c_double), ("bragg_delta", C.c_double),
+        ("bf_offset", C.c_double),
     ]
 
 
@@ -96,6 +98,11 @@ def load():
         "orc_engine_get_col": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _U, _D, C.c_int32]),
         "orc_engine_project": (None, [C.c_void_p, _D, _D]),
         "orc_engine_theta": (None, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _D]),
+        "orc_engine_get_sel": (None, [C.c_void_p, C.c_void_p]),
+        "orc_engine_get_bragg": (C.c_double, [C.c_void_p]),
+        "orc_engine_set_bragg": (None, [C.c_void_p, C.c_double]),
+        "orc_engine_update_selector": (None, [C.c_void_p]),
+        "orc_bf_beta": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -129,9 +136,10 @@ def geometry(N_r, N_t, angles_deg, thickness, delta=1.0, extend_object=0):
 
 def params(sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3, target_gain=1.0, stop_threshold=1e-3,
            num_outer=1, num_inner=20, joint_estimation=0, noise_model=0, zero_skipping=1,
-           compute_cost=1, seed=12345):
+           compute_cost=1, seed=12345, bf_mode=0, bragg_T=3.0, bragg_delta=0.5, bf_offset=0.0):
     return Params(sigma_x, mrf_q, mrf_c, target_gain, stop_threshold, num_outer, num_inner,
-                  joint_estimation, noise_model, zero_skipping, compute_cost, seed)
+                  joint_estimation, noise_model, zero_skipping, compute_cost, seed, bf_mode,
+                  bragg_T, bragg_delta, bf_offset)
 
 
 class Engine:
@@ -222,6 +230,19 @@ class Engine:
 
     def profile(self):
         return self._get(lib().orc_engine_get_profile, (self.g.N_theta, 1536))
+
+    def sel(self):
+        out = np.zeros(self.sshape, np.uint8)
+        lib().orc_engine_get_sel(self.h, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def bragg(self, T=None):
+        if T is not None:
+            lib().orc_engine_set_bragg(self.h, float(T))
+        return lib().orc_engine_get_bragg(self.h)
+
+    def update_selector(self):
+        lib().orc_engine_update_selector(self.h)
 
     def nuisance(self):
         nt = self.g.N_theta
