@@ -99,6 +99,19 @@ typedef struct mbir_problem {
   /* sweep kernels: 0 = one launch per conflict-free step with the job's own kernel
    * (plain per-job GPU ICD); 1 = the batched step kernel (a batch of one job) */
   int32_t kernel_mode;
+  /* Bright-field mode (BrightField/BFForwardModel.cpp, BFReconstructionEngine.cpp:575-872):
+   * counts are transmitted counts; the measurement is -log(counts + bf_offset)
+   * (processRawCounts :939-972), the weights (counts + bf_offset) / alpha (:328-385),
+   * entries whose |e sqrt(w)| reaches the Bragg threshold T are rejected by the
+   * selector; joint_estimation estimates the per-view offset (log dose) only and
+   * noise_model is NOISE_ESTIMATION (one variance for all views); the first outer
+   * iteration's inner sweeps run with the threshold 5e10 when num_inner > 1.  The
+   * reference's BF prior uses mrf_c = 0.01 (BrightField/BF_QGGMRFPriorModel.cpp:54).
+   * Needs schedule 0, N_y <= 256, one detector row per slice; single-job sweeps only. */
+  int32_t bf_mode;
+  double bragg_T;     /* Bragg threshold T (BFReconstructionArgsParser.cpp:176, default 3) */
+  double bragg_delta; /* delta of the Bragg rejector (:179, default 0.5) */
+  double bf_offset;   /* added to the counts before the log (:182, default 0) */
 } mbir_problem;
 
 /* Per-sweep statistics (updateVoxels, HAADF_ReconstructionEngine_UpdateVoxels.cpp:717-786) */
@@ -190,6 +203,12 @@ int32_t mbir_job_get_detector_response(mbir_job* job, double* out);
 /* Thread-safe cooperative cancel (AbstractFilter::setCancel); the next sweep
  * or run step returns MBIR_ERR_CANCELLED. */
 int32_t mbir_job_cancel(mbir_job* job);
+/* Bright field: the Bragg selector (1 kept, 0 rejected) and the weights w, both
+ * [N_theta][N_r][N_t] (BFForwardModel m_Selector, m_Weight); -3 for other jobs. */
+int32_t mbir_job_get_selector(mbir_job* job, uint8_t* out);
+int32_t mbir_job_get_weights(mbir_job* job, float* out);
+/* Bright field: the Bragg threshold the next sweep uses (the run loop sets it itself) */
+int32_t mbir_job_set_bragg_threshold(mbir_job* job, double T);
 /* device bytes held by the job (volume, sinograms, tables) */
 int64_t mbir_job_device_bytes(const mbir_job* job);
 /* 1 = time every ICD kernel step with HIP events on the job's stream */

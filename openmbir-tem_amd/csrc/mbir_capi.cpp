@@ -334,7 +334,33 @@ int setup_job(mbir_job* j)
   HIP_TRY(j->colr.alloc(kMaxSlots));
   j->bytes = (int64_t)(V * 4 + 2 * M * 4 + L * 9 + (size_t)NT * 64 * 8);
 
-  HIP_TRY(hipMemcpyAsync(j->y.p, P.counts, M * sizeof(float), hipMemcpyHostToDevice, s));
+  // bright field: -log(counts + offset) and the count weights in FP64 on the host
+  // (processRawCounts, BrightField/BFForwardModel.cpp:939-972; calculateMeasurementWeight
+  // :328-385 with BF_MAX = 1)
+  std::vector<float> bf_y, bf_w;
+  if (P.bf_mode) {
+    for (int c : vn)
+      if (c > 1) return MBIR_ERR_UNSUPPORTED;  // one detector row per slice
+    if (P.schedule != 0) return MBIR_ERR_UNSUPPORTED;
+    bf_y.resize(M);
+    bf_w.resize(M);
+    const size_t per = (size_t)g.N_r * g.N_t;
+    for (size_t i = 0; i < M; ++i) {
+      const double c = (double)P.counts[i] + P.bf_offset;
+      if (!(c > 0.0)) return MBIR_ERR_INPUT;  // negative counts (:956-960)
+      const double yv = -std::log(c / 1.0);
+      double w = 1.0 / std::exp(yv);
+      if (P.noise_model) w /= j->alphah[i / per];
+      bf_y[i] = (float)yv;
+      bf_w[i] = (float)w;  // positive: every entry selected (:347)
+    }
+    HIP_TRY(j->wfull.alloc(M));
+    HIP_TRY(j->alpha_b.alloc(NT));
+    HIP_TRY(hipMemcpyAsync(j->wfull.p, bf_w.data(), M * sizeof(float), hipMemcpyHostToDevice, s));
+    j->bytes += (int64_t)M * 4;
+  }
+  HIP_TRY(hipMemcpyAsync(j->y.p, P.bf_mode ? bf_y.data() : P.counts, M * sizeof(float),
+                         hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(j->cosv.p, cosv.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(j->sinv.p, sinv.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(j->Hd.p, j->H.data(), j->H.size() * sizeof(double), hipMemcpyHostToDevice, s));
@@ -347,7 +373,7 @@ int setup_job(mbir_job* j)
   HIP_TRY(hipMemcpyAsync(j->alpha.p, j->alphah.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
   // weights: identity noise model, w = 1 / alpha when the noise model is on (…_Extra.cpp:740-757)
   std::vector<float> wv(NT);
-  for (int k = 0; k < NT; ++k) wv[k] = P.noise_model ? (float)(1.0 / j->alphah[k]) : 1.0f;
+  for (int k = 0; k < NT; ++k) wv[k] = P.noise_model && !P.bf_mode ? (float)(1.0 / j->alphah[k]) : 1.0f;
   HIP_TRY(hipMemcpyAsync(j->wview.p, wv.data(), NT * sizeof(float), hipMemcpyHostToDevice, s));
   if (P.init_volume)
     HIP_TRY(hipMemcpyAsync(j->x.p, P.init_volume, V * sizeof(float), hipMemcpyHostToDevice, s));
@@ -372,6 +398,11 @@ int setup_job(mbir_job* j)
   // one job alone: the atomic e-update is the faster form (the batch and server paths
   // switch to the plain form for conflict-free schedules)
   J.e_atomic = 1;
+  if (P.bf_mode) {
+    J.wfull = j->wfull.p;
+    J.e_atomic = 0;  // the selector refresh is a read-modify-write of its own
+    set_bragg(j, P.bragg_T);
+  }
   // QGGMRF constants (HAADF/HAADF_QGGMRFPriorModel.cpp:10-19), MRF_P = 2
   const double sp = std::pow(P.sigma_x, 2.0), spq = std::pow(P.sigma_x, 2.0 - P.mrf_q);
   J.mrf_q = (float)P.mrf_q; J.mrf_c = (float)P.mrf_c; J.sigma_p = (float)sp;
@@ -487,11 +518,30 @@ int sweep_status(mbir_job* j, const SweepAccum& a, double ms, mbir_sweep_stats* 
   return status;
 }
 
+void set_bragg(mbir_job* j, double T)
+{
+  j->bragg = T;
+  j->J.bragg = (float)T;
+  j->J.bragg_dT = (float)(j->prob.bragg_delta * T);
+  j->J.bragg_d = T;
+  j->J.delta_d = j->prob.bragg_delta;
+}
+
 // jointEstimation then updateWeights (HAADF/HAADF_ReconstructionEngine.cpp:768-786)
 // and the refresh of the gain- and weight-folded column values, enqueued on s.
 int nuisance_update_on(mbir_job* j, hipStream_t s)
 {
   const int NT = j->g.N_theta;
+  if (j->J.wfull) {
+    // bright field (BrightField/BFReconstructionEngine.cpp:841-870): the offset, then
+    // the shared variance, each followed by the selector refresh
+    if (j->prob.joint_estimation) HIP_TRY(launch_bf_offset(j->J, j->sums6.p, j->mu.p, s));
+    if (j->prob.noise_model) {
+      HIP_TRY(launch_bf_noise(j->J, j->alpha.p, j->alpha_b.p, j->cost.p + 4, s));
+      std::swap(j->alpha.p, j->alpha_b.p);
+    }
+    return MBIR_OK;
+  }
   if (j->prob.joint_estimation) {
     HIP_TRY(launch_joint_sums(j->J, j->sums6.p, s));
     HIP_TRY(launch_joint_solve(NT, j->sums6.p, j->prob.target_gain, j->I0.p, j->mu.p, j->I0b.p,
@@ -575,7 +625,7 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
     mbir_job* j = jobs[i];
     if (j->ctx != ctx) return MBIR_ERR_INPUT;
     if (j->cancel.load()) return MBIR_ERR_CANCELLED;
-    if (j->J.N_y > 128 || j->J.vlr_max > 2) return MBIR_ERR_UNSUPPORTED;
+    if (j->J.N_y > 128 || j->J.vlr_max > 2 || j->J.wfull) return MBIR_ERR_UNSUPPORTED;
     int rc = prepare_sweep(j);
     if (rc < 0) return rc;
     dj[i] = j->J;
@@ -798,6 +848,63 @@ extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, 
   int rc = push_cost();
   if (rc < 0) return rc;
   int num_iter = j->prob.num_inner;
+  if (j->J.wfull) {
+    // bright field: BFReconstructionEngine::execute's loop (BrightField/
+    // BFReconstructionEngine.cpp:575-872) with full random-order sweeps: the
+    // threshold 5e10 for the first outer iteration's inner sweeps when NumIter > 1,
+    // T from its last inner sweep on; the stop test ends the run before the nuisance
+    // step, which runs only when NumOuterIter > 1
+    const double kDefBragg = 5e10;  // DefBraggThreshold, BrightField/BFConstants.h
+    const double des = j->prob.bragg_T;
+    double tmp = num_iter > 1 ? kDefBragg : des;
+    set_bragg(j, tmp);
+    for (int outer = 0; outer < j->prob.num_outer; ++outer) {
+      if (outer > 0) {
+        num_iter = 1;
+        set_bragg(j, tmp);
+      }
+      for (int inner = 0; inner < num_iter; ++inner) {
+        if (num_iter > 1 && inner == 0) set_bragg(j, kDefBragg);
+        status = run_sweep(j, outer, inner, nullptr);
+        if (status < 0) return status;
+        ns++;
+        if (status == 0) break;
+        if ((rc = push_cost()) < 0) return rc;
+        if (inner == num_iter - 1 && inner != 0) {
+          tmp = des;
+          set_bragg(j, tmp);
+        }
+      }
+      if (j->cancel.load()) return MBIR_ERR_CANCELLED;
+      outer_done = outer + 1;
+      if (status == 0 && outer >= 1) break;
+      if (j->prob.num_outer > 1) {
+        const int je = j->prob.joint_estimation, nm = j->prob.noise_model;
+        if (je) {
+          j->prob.noise_model = 0;
+          rc = nuisance_update(j);
+          j->prob.noise_model = nm;
+          if (rc < 0) return rc;
+          if ((rc = push_cost()) < 0) return rc;
+        }
+        if (nm) {
+          j->prob.joint_estimation = 0;
+          rc = nuisance_update(j);
+          j->prob.joint_estimation = je;
+          if (rc < 0) return rc;
+          if ((rc = push_cost()) < 0) return rc;
+        }
+      }
+    }
+    if (rs) {
+      rs->sweeps = ns;
+      rs->outer_iterations = outer_done;
+      rs->status = status;
+      rs->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (n_costs) *n_costs = nc;
+    return MBIR_OK;
+  }
   for (int outer = 0; outer < j->prob.num_outer; ++outer) {
     if (outer != 0) num_iter = 1;  // :712-715
     for (int inner = 0; inner < num_iter; ++inner) {
@@ -970,5 +1077,36 @@ extern "C" int32_t mbir_job_set_timing(mbir_job* j, int32_t on)
 {
   if (!j) return MBIR_ERR_INPUT;
   j->timing = on ? 1 : 0;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_get_selector(mbir_job* j, uint8_t* out)
+{
+  if (!j || !out) return MBIR_ERR_INPUT;
+  if (!j->J.wfull) return MBIR_ERR_UNSUPPORTED;
+  const size_t M = (size_t)j->g.N_theta * j->g.N_r * j->g.N_t;
+  std::vector<float> w(M);
+  HIP_TRY(hipMemcpyAsync(w.data(), j->wfull.p, M * sizeof(float), hipMemcpyDeviceToHost, j->stream));
+  HIP_TRY(hipStreamSynchronize(j->stream));
+  for (size_t i = 0; i < M; ++i) out[i] = w[i] >= 0.f ? 1 : 0;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_get_weights(mbir_job* j, float* out)
+{
+  if (!j || !out) return MBIR_ERR_INPUT;
+  if (!j->J.wfull) return MBIR_ERR_UNSUPPORTED;
+  const size_t M = (size_t)j->g.N_theta * j->g.N_r * j->g.N_t;
+  HIP_TRY(hipMemcpyAsync(out, j->wfull.p, M * sizeof(float), hipMemcpyDeviceToHost, j->stream));
+  HIP_TRY(hipStreamSynchronize(j->stream));
+  for (size_t i = 0; i < M; ++i) out[i] = std::fabs(out[i]);
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_set_bragg_threshold(mbir_job* j, double T)
+{
+  if (!j || !(T > 0)) return MBIR_ERR_INPUT;
+  if (!j->J.wfull) return MBIR_ERR_UNSUPPORTED;
+  set_bragg(j, T);
   return MBIR_OK;
 }

@@ -76,6 +76,9 @@ struct mbir_job {
   DevBuf<int> vlr_t0, vlr_n, line_nnz, colr, step_ptr_d, col_ptr, col_row;
   DevBuf<LineRef> sched;
   DevBuf<float> col_a, col_ka, col_kw;
+  DevBuf<float> wfull;       // bright field: per-entry weights, sign = selector
+  DevBuf<double> alpha_b;    // bright field: the next shared variance
+  double bragg = 0;          // bright field: current Bragg threshold
   std::vector<int> nnz_h;  // per-line column entries (host copy)
   int64_t nnz_total = 0;
   DevBuf<uint8_t> mask;
@@ -95,7 +98,7 @@ struct mbir_job {
     I0b.release(); mub.release(); sums6.release(); yest.release(); cost.release(); prj.release();
     vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
     step_ptr_d.release(); mask.release(); acc.release(); col_ptr.release(); col_row.release();
-    col_a.release(); col_ka.release(); col_kw.release();
+    col_a.release(); col_ka.release(); col_kw.release(); wfull.release(); alpha_b.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -111,5 +114,7 @@ void build_schedule(mbir_job* j, bool drop_empty_steps);
 int prepare_sweep_on(mbir_job* j, hipStream_t s, const LineRef* sched_src);
 int sweep_status(mbir_job* j, const SweepAccum& a, double ms, mbir_sweep_stats* st);
 int nuisance_update_on(mbir_job* j, hipStream_t s);
+// Bright field: the Bragg threshold used by the next launches (DevJob fields).
+void set_bragg(mbir_job* j, double T);
 
 }  // namespace mbir

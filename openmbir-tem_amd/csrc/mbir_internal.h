@@ -63,7 +63,8 @@ struct DevJob {
   float* x;              // [N_z][N_x][N_y]
   float* e;              // [N_theta][N_r][N_t] error sinogram
   const float* y;        // [N_theta][N_r][N_t] measurements
-  const float* wfull;    // [N_theta][N_r][N_t] or nullptr (per-view weights)
+  float* wfull;          // [N_theta][N_r][N_t] per-entry weights (bright field), its sign the
+                         // Bragg selector (< 0: rejected); nullptr = per-view weights
   const float* wview;    // [N_theta] per-view weight (identity noise model: 1/alpha)
   const double* I0;      // [N_theta]
   const double* mu;      // [N_theta]
@@ -71,6 +72,8 @@ struct DevJob {
   float mrf_q, mrf_c, sigma_p, sigma_pq, alpha, b0;  // b0 = P / (sigma^P c)
   float pq2, inv_sigma_p, inv_sigma_pq;            // 2 - q, 1 / sigma^P, 1 / sigma^(P-q)
   double mrf_qd, mrf_cd, sigma_pd, sigma_pqd;
+  float bragg, bragg_dT;  // bright field: Bragg threshold T and delta T
+  double bragg_d, delta_d;  // the same T and delta in FP64 (cost, nuisance updates)
   int e_atomic;          // 1: error-sinogram update by fp32 atomics (required when lines of
                          // a step may share detector bins: relaxed schedule), 0: plain
                          // read-modify-write (conflict-free steps only)
@@ -135,6 +138,11 @@ hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& 
                            hipStream_t s);
 // jobs[slot] = J, ordered on s (the descriptor travels as a kernel argument)
 hipError_t launch_set_devjob(DevJob* jobs, int slot, const DevJob& J, hipStream_t s);
+// bright field (BrightField/BFForwardModel.cpp): offset-only joint estimation
+// (:390-441) with the selector refresh (:517-531), and the shared variance (:447-511)
+hipError_t launch_bf_offset(const DevJob& J, double* nd, double* mu, hipStream_t s);
+hipError_t launch_bf_noise(const DevJob& J, const double* alpha_old, double* alpha_new,
+                           double* part, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 hipError_t launch_fill_f64(double* p, double v, size_t n, hipStream_t s);
 

@@ -13,6 +13,8 @@
 // into LDS; the per-voxel VLR weights are applied from registers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mbir_internal.h"
 
 namespace mbir {
@@ -451,10 +453,22 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
           for (int k = 0; k < NV; ++k)
             if (act[jy] && k < vn[jy]) {
               const float pe = kav * vv[jy][k];
-              // per-view weight folded in: pe w = kw vlr (the BF path carries w per entry)
+              // per-view weight folded in: pe w = kw vlr.  The bright-field path
+              // carries w per entry, its sign being the Bragg selector: a rejected
+              // entry (w < 0) weighs delta T / (|e| sqrt(|w|)) more
+              // (BrightField/BFForwardModel.cpp:851-892)
               float pw;
-              if constexpr (FW) pw = pe * wf[u][jy][k];
-              else pw = kwv * vv[jy][k];
+              if constexpr (FW) {
+                const float w = wf[u][jy][k];
+                if (w >= 0.f) {
+                  pw = pe * w;
+                } else {
+                  const float aw = -w;
+                  pw = pe * aw * (J.bragg_dT / (fabsf(ev[u][jy][k]) * sqrtf(aw)));
+                }
+              } else {
+                pw = kwv * vv[jy][k];
+              }
               t2[jy].add(pe * pw);
               t1[jy].add(ev[u][jy][k] * pw);
             }
@@ -590,7 +604,54 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
     const int y = yl + TY * jy;
     d[jy] = y < Ny ? sm.dx[y] : 0.f;
   }
-  if (J.e_atomic) {
+  if constexpr (FW) {
+    // bright field: every updated slice (even with dx = 0) refreshes the selector of
+    // the entries it touches, |e sqrt(w)| < T (BFForwardModel::updateErrorSinogram,
+    // :898-932); conflict-free steps only, plain read-modify-write
+    bool upd[YPL];
+#pragma unroll
+    for (int jy = 0; jy < YPL; ++jy) {
+      const int y = yl + TY * jy;
+      upd[jy] = y < Ny && sm.vst[y] == 1;
+    }
+    for (int base = p; base < nc; base += P * U) {
+      float ev[U][YPL][NV], wv[U][YPL][NV];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * P;
+        const bool ok = i < nc;
+        const int rw = sm.row[ok ? i : 0];
+#pragma unroll
+        for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+          for (int k = 0; k < NV; ++k) {
+            const bool use = ok && upd[jy] && k < vn[jy];
+            const int idx = use ? rw + vt0[jy] + k : 0;
+            ev[u][jy][k] = J.e[idx];
+            wv[u][jy][k] = J.wfull[idx];
+          }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * P;
+        if (i < nc) {
+          const float ka = sm.ka[i];
+          const int rwi = sm.row[i];
+#pragma unroll
+          for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+              if (upd[jy] && k < vn[jy]) {
+                const float en =
+                    __fadd_rn(ev[u][jy][k], -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy]));
+                const float aw = fabsf(wv[u][jy][k]);
+                J.e[rwi + vt0[jy] + k] = en;
+                J.wfull[rwi + vt0[jy] + k] = fabsf(en * sqrtf(aw)) < J.bragg ? aw : -aw;
+              }
+        }
+      }
+    }
+  } else if (J.e_atomic) {
     for (int base = p; base < nc; base += P * U) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -675,14 +736,14 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   __syncthreads();  // H: LDS may be reused by the next line
 }
 
-template <int YPL, int NV>
+template <int YPL, int NV, bool FW = false>
 __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef* lines, int nlines,
                                                         int first, int zs_on, SweepAccum* acc)
 {
   __shared__ IcdSmem<YPL> sm;
   if ((int)blockIdx.x >= nlines) return;
   const LineRef ref = lines[blockIdx.x];
-  icd_line<YPL, NV, false, true>(J, ref, first, zs_on, acc, sm);
+  icd_line<YPL, NV, FW, true>(J, ref, first, zs_on, acc, sm);
 }
 
 // One conflict-free step of every job of a batch that has kernel shape
@@ -853,8 +914,18 @@ __global__ __launch_bounds__(kBlock) void k_cost_partials(DevJob J, double* part
   double data = 0, prior = 0;
   for (size_t i = g0; i < M; i += gs) {
     const double ev = J.e[i];
-    const double w = J.wfull ? (double)J.wfull[i] : (double)J.wview[i / per];
-    data += ev * ev * w;
+    if (J.wfull) {
+      // BFForwardModel::forwardCost (:812-846): Bragg rejector beyond T
+      const double w = (double)J.wfull[i];
+      if (w >= 0.0) {
+        data += ev * ev * w;
+      } else {
+        const double T = J.bragg_d, dl = J.delta_d;
+        data += 2.0 * T * dl * fabs(ev) * sqrt(-w) + T * T * (1.0 - 2.0 * dl);
+      }
+    } else {
+      data += ev * ev * (double)J.wview[i / per];
+    }
   }
   const int Ny = J.N_y, Nx = J.N_x, Nz = J.N_z;
   for (size_t v = g0; v < V; v += gs) {
@@ -920,6 +991,111 @@ __global__ __launch_bounds__(kBlock) void k_cost_final(DevJob J, const double* p
     out3[0] = d / 2;
     out3[1] = p;
     out3[2] = n / 2;
+    if (noise_model && J.wfull) {
+      // bright field: (N_theta N_r N_t / 2) log(alpha[0]), integer halving (:837-841)
+      const size_t M = (size_t)J.N_theta * J.N_r * J.N_t;
+      out3[2] = (double)(M / 2) * log(alpha[0]);
+    }
+  }
+}
+
+// Bright field, offset (log dose) per view: the minimiser of the selector's
+// half-quadratic surrogate, rejected entries weighing delta T / (|e| sqrt(w))
+// (BFForwardModel::jointEstimation, :396-418).  FP64 sums, one block per view.
+__global__ __launch_bounds__(kBlock) void k_bf_offset_sums(DevJob J, double* nd)
+{
+  __shared__ double red[kBlock];
+  const int th = blockIdx.x;
+  const size_t per = (size_t)J.N_r * J.N_t, base = (size_t)th * per;
+  const double Td = J.bragg_d * J.delta_d;
+  double num = 0, den = 0;
+  for (size_t k = threadIdx.x; k < per; k += blockDim.x) {
+    const double ev = J.e[base + k], w = J.wfull[base + k];
+    if (w >= 0.0) {
+      num += ev * w;
+      den += w;
+    } else {
+      num += (Td * ev / fabs(ev)) * sqrt(-w);
+      den += (Td * sqrt(-w)) / fabs(ev);
+    }
+  }
+  num = block_sum_d(num, red);
+  den = block_sum_d(den, red);
+  if (threadIdx.x == 0) {
+    nd[2 * th] = num;
+    nd[2 * th + 1] = den;
+  }
+}
+
+// e -= a_th, mu_th += a_th (:420-431), then updateSelector: e^2 w < T^2 (:517-531).
+__global__ void k_bf_offset_apply(DevJob J, const double* nd, double* mu)
+{
+  const size_t per = (size_t)J.N_r * J.N_t, M = per * J.N_theta;
+  const double T2 = J.bragg_d * J.bragg_d;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < M;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int th = (int)(i / per);
+    const double a = nd[2 * th] / nd[2 * th + 1];
+    const float en = (float)((double)J.e[i] - a);
+    const float aw = fabsf(J.wfull[i]);
+    J.e[i] = en;
+    J.wfull[i] = (double)en * (double)en * (double)aw < T2 ? aw : -aw;
+    if (i == (size_t)th * per) mu[th] += a;
+  }
+}
+
+// Shared variance (:447-511): with the unscaled weights w_u = w alpha_th,
+// alpha = (sum_sel e^2 w_u + delta T sum_rej |e| sqrt(alpha_th w_u)) / M.
+__global__ __launch_bounds__(kBlock) void k_bf_noise_partials(DevJob J, const double* alpha,
+                                                              double* part)
+{
+  __shared__ double red[kBlock];
+  const size_t per = (size_t)J.N_r * J.N_t, M = per * J.N_theta;
+  double s1 = 0, s2 = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < M;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double at = alpha[i / per];
+    const double ev = J.e[i], w = J.wfull[i];
+    const double wu = fabs(w) * at;
+    if (w >= 0.0) s1 += ev * ev * wu;
+    else s2 += fabs(ev) * sqrt(at * wu);
+  }
+  s1 = block_sum_d(s1, red);
+  s2 = block_sum_d(s2, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1;
+    part[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bf_noise_final(DevJob J, const double* part,
+                                                           int nparts, double* alpha_new)
+{
+  __shared__ double red[kBlock];
+  double s1 = 0, s2 = 0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+    s1 += part[2 * b];
+    s2 += part[2 * b + 1];
+  }
+  s1 = block_sum_d(s1, red);
+  s2 = block_sum_d(s2, red);
+  const double M = (double)J.N_theta * (double)J.N_r * (double)J.N_t;
+  const double upd = (s1 + (J.delta_d * J.bragg_d) * s2) / M;
+  for (int th = threadIdx.x; th < J.N_theta; th += blockDim.x) alpha_new[th] = upd;
+}
+
+// w = w_u / alpha_new, then updateSelector (:503-510, :517-531).
+__global__ void k_bf_noise_apply(DevJob J, const double* alpha_old, const double* alpha_new)
+{
+  const size_t per = (size_t)J.N_r * J.N_t, M = per * J.N_theta;
+  const double T2 = J.bragg_d * J.bragg_d;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < M;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int th = (int)(i / per);
+    const double wn = (fabs((double)J.wfull[i]) * alpha_old[th]) / alpha_new[th];
+    const double ev = J.e[i];
+    const float aw = (float)wn;
+    J.wfull[i] = ev * ev * wn < T2 ? aw : -aw;
   }
 }
 
@@ -991,7 +1167,10 @@ template <int YPL>
 static void launch_step_nv(const DevJob& J, const LineRef* lines, int nlines, int first, int zs,
                            SweepAccum* acc, hipStream_t s)
 {
-  if (J.vlr_max <= 1)
+  if (J.wfull)  // bright field: per-entry weights and selector (one row per slice)
+    hipLaunchKernelGGL((k_icd_step<YPL, 1, true>), dim3(nlines), dim3(kIcdBlock), 0, s, J, lines,
+                       nlines, first, zs, acc);
+  else if (J.vlr_max <= 1)
     hipLaunchKernelGGL((k_icd_step<YPL, 1>), dim3(nlines), dim3(kIcdBlock), 0, s, J, lines, nlines,
                        first, zs, acc);
   else if (J.vlr_max == 2)
@@ -1117,6 +1296,28 @@ hipError_t launch_cost(const DevJob& J, const double* alpha, int noise_model, do
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_cost_final, dim3(1), dim3(kBlock), 0, s, J, part, nparts, alpha,
                      noise_model, out3);
+  return hipGetLastError();
+}
+
+hipError_t launch_bf_offset(const DevJob& J, double* nd, double* mu, hipStream_t s)
+{
+  const size_t M = (size_t)J.N_theta * J.N_r * J.N_t;
+  hipLaunchKernelGGL(k_bf_offset_sums, dim3(J.N_theta), dim3(kBlock), 0, s, J, nd);
+  hipLaunchKernelGGL(k_bf_offset_apply, dim3(grid_for(M)), dim3(kBlock), 0, s, J,
+                     (const double*)nd, mu);
+  return hipGetLastError();
+}
+
+hipError_t launch_bf_noise(const DevJob& J, const double* alpha_old, double* alpha_new,
+                           double* part, hipStream_t s)
+{
+  const size_t M = (size_t)J.N_theta * J.N_r * J.N_t;
+  const int nb = std::min(grid_for(M), 1024);
+  hipLaunchKernelGGL(k_bf_noise_partials, dim3(nb), dim3(kBlock), 0, s, J, alpha_old, part);
+  hipLaunchKernelGGL(k_bf_noise_final, dim3(1), dim3(kBlock), 0, s, J, (const double*)part, nb,
+                     alpha_new);
+  hipLaunchKernelGGL(k_bf_noise_apply, dim3(grid_for(M)), dim3(kBlock), 0, s, J, alpha_old,
+                     (const double*)alpha_new);
   return hipGetLastError();
 }
 

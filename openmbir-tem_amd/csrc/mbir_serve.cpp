@@ -149,7 +149,8 @@ void mbir_server::setup_loop()
                         hipSuccess ||
                     hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess))
       rc = MBIR_ERR_HIP;
-    if (rc == 0 && (job->J.N_y > 128 || job->J.vlr_max > 2)) rc = MBIR_ERR_UNSUPPORTED;
+    if (rc == 0 && (job->J.N_y > 128 || job->J.vlr_max > 2 || job->J.wfull))
+      rc = MBIR_ERR_UNSUPPORTED;
 
     lk.lock();
     setups_running--;

@@ -54,6 +54,8 @@ class Problem(C.Structure):
         ("joint_estimation", C.c_int32), ("noise_model", C.c_int32),
         ("zero_skipping", C.c_int32), ("seed", C.c_uint32),
         ("schedule", C.c_int32), ("rows_per_step", C.c_int32), ("kernel_mode", C.c_int32),
+        ("bf_mode", C.c_int32), ("bragg_T", C.c_double), ("bragg_delta", C.c_double),
+        ("bf_offset", C.c_double),
     ]
 
 
@@ -123,6 +125,9 @@ _SIG = {
     "mbir_job_device_bytes": (C.c_int64, [C.c_void_p]),
     "mbir_job_set_timing": (C.c_int32, [C.c_void_p, C.c_int32]),
     "mbir_problem_device_bytes": (C.c_int64, [C.POINTER(Problem)]),
+    "mbir_job_get_selector": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "mbir_job_get_weights": (C.c_int32, [C.c_void_p, _F]),
+    "mbir_job_set_bragg_threshold": (C.c_int32, [C.c_void_p, C.c_double]),
     "mbir_server_create": (C.c_int32, [C.c_void_p, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     "mbir_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(Problem), C.POINTER(C.c_uint64)]),
     "mbir_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(JobReport)]),
@@ -211,7 +216,8 @@ def make_problem(geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3
                  target_gain=1.0, default_offset=0.0, default_variance=1.0, stop_threshold=1e-3,
                  num_outer=1, num_inner=20, joint_estimation=0, noise_model=0, zero_skipping=1,
                  seed=12345, schedule=0, rows_per_step=1, kernel_mode=0, init_volume=None,
-                 default_recon=0.0, init_gain=None, init_offset=None, init_variance=None):
+                 default_recon=0.0, init_gain=None, init_offset=None, init_variance=None,
+                 bf_mode=0, bragg_T=3.0, bragg_delta=0.5, bf_offset=0.0):
     """mbir_problem for the arrays given; returns (Problem, arrays to keep alive)."""
     ang = np.ascontiguousarray(angles_deg, np.float64)
     cnt = np.ascontiguousarray(counts, np.float32)
@@ -241,6 +247,7 @@ def make_problem(geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3
     p.joint_estimation, p.noise_model, p.zero_skipping = joint_estimation, noise_model, zero_skipping
     p.seed = seed
     p.schedule, p.rows_per_step, p.kernel_mode = schedule, rows_per_step, kernel_mode
+    p.bf_mode, p.bragg_T, p.bragg_delta, p.bf_offset = bf_mode, bragg_T, bragg_delta, bf_offset
     return p, keep
 
 
@@ -346,6 +353,19 @@ class Job:
         out = np.zeros((self.geom.N_theta, 64))
         _chk("mbir_job_get_detector_response", lib().mbir_job_get_detector_response(self.h, _dptr(out)))
         return out
+
+    def selector(self):
+        out = np.zeros(self.sshape, np.uint8)
+        _chk("mbir_job_get_selector", lib().mbir_job_get_selector(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def weights(self):
+        out = np.zeros(self.sshape, np.float32)
+        _chk("mbir_job_get_weights", lib().mbir_job_get_weights(self.h, _fptr(out)))
+        return out
+
+    def set_bragg(self, T):
+        _chk("mbir_job_set_bragg_threshold", lib().mbir_job_set_bragg_threshold(self.h, T))
 
     def cancel(self):
         _chk("mbir_job_cancel", lib().mbir_job_cancel(self.h))

@@ -1,0 +1,111 @@
+"""GPU: the bright-field path against the FP64 oracle.
+
+From identical states (the same volume, counts and nuisance parameters) the GPU's
+log counts, weights, selector, offset and variance steps and cost must match the
+oracle to float32 precision; over sweeps the update order differs, so costs must
+fall and the converged volumes agree within the oracle's own seed-to-seed spread.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from mbirtem import capi
+from problems import bf_problem
+
+pytestmark = pytest.mark.gpu
+
+BF = dict(bf_mode=1, mrf_c=0.01, sigma_x=0.01, mrf_q=1.2, bragg_T=3.0, bragg_delta=0.5)
+
+
+def _pair(ctx, g, ang, counts, x0=None, offsets=None, **kw):
+    prm = dict(BF, **kw)
+    eng = O.Engine(g, ang, counts, O.params(**prm), x0=x0, offsets=offsets)
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
+    job = capi.Job(ctx, geom, ang, counts.astype(np.float32), init_volume=x0,
+                   init_offset=offsets, **prm)
+    return eng, job  # both start from the variance 1 (the defaults)
+
+
+def test_bf_setup_matches_oracle(gpu_ctx):
+    g, ang, vol, counts = bf_problem()
+    eng, job = _pair(gpu_ctx, g, ang, counts, x0=vol, noise_model=1)
+    ref_e = eng.e()
+    assert np.max(np.abs(job.error_sinogram() - ref_e)) <= 1e-5 * np.max(np.abs(ref_e))
+    np.testing.assert_allclose(job.weights(), eng.w(), rtol=1e-6)
+    assert job.selector().min() == 1
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+
+
+def test_bf_offset_and_variance_steps_match_oracle(gpu_ctx):
+    g, ang, vol, counts = bf_problem(bragg_frac=0.05)
+    offs = np.full(len(ang), -np.log(2000.0) + 0.03)
+    eng, job = _pair(gpu_ctx, g, ang, counts, x0=vol, offsets=offs, joint_estimation=1,
+                     noise_model=1)
+    for step in range(3):  # the first step uses the all-selected setup selector
+        eng.joint_estimation()
+        eng.update_selector()
+        eng.update_weights()
+        eng.update_selector()
+        job.nuisance_update()
+        I0, mu, al = eng.nuisance()
+        I0g, mug, alg = job.nuisance()
+        np.testing.assert_allclose(mug, mu, rtol=0, atol=2e-6)
+        np.testing.assert_allclose(alg, al, rtol=2e-5)
+        np.testing.assert_array_equal(I0g, I0)  # the gain stays at the target
+        sel_g, sel_r = job.selector(), eng.sel()
+        # entries at the threshold may fall either side in float32
+        assert np.mean(sel_g != sel_r) < 1e-3, np.mean(sel_g != sel_r)
+    assert eng.sel().mean() < 0.99  # rejections took part
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-5)
+
+
+def test_bf_sweeps_lower_the_cost_and_keep_e_consistent(gpu_ctx):
+    g, ang, vol, counts = bf_problem(bragg_frac=0.05)
+    offs = np.full(len(ang), -np.log(2000.0))
+    eng, job = _pair(gpu_ctx, g, ang, counts, offsets=offs)
+    costs = [job.cost()]
+    for it in range(8):
+        job.sweep(0, it)
+        costs.append(job.cost())
+    assert all(b <= a * (1 + 1e-6) + 1e-9 for a, b in zip(costs, costs[1:]))
+    # e = y - A x - mu for the GPU's own volume (oracle projection, FP64)
+    y = -np.log(counts)
+    e_ref = y - eng.project(job.volume().astype(np.float64)) - offs[:, None, None]
+    assert np.max(np.abs(job.error_sinogram() - e_ref)) <= 2e-5 * np.max(np.abs(e_ref))
+    # the selector is the rule |e sqrt(w)| < T for every refreshed entry
+    e, w, sel = job.error_sinogram(), job.weights(), job.selector()
+    rule = (np.abs(e * np.sqrt(w)) < 3.0).astype(np.uint8)
+    assert np.mean(sel != rule) < 1e-3
+
+
+def test_bf_run_matches_oracle(gpu_ctx):
+    """BFReconstructionEngine's loop (NumIter 6 then 14 outer iterations with the
+    offset and variance steps).  Oracle runs with other seeds lie within a few
+    percent of each other; the GPU must lie within 3 of their scatters of their mean."""
+    g, ang, vol, counts = bf_problem(bragg_frac=0.03)
+    offs = np.full(len(ang), -np.log(2000.0))
+    kw = dict(num_outer=15, num_inner=6, joint_estimation=1, noise_model=1, stop_threshold=1e-5)
+    runs = []
+    for seed in (12345, 1, 2, 3):
+        e = O.Engine(g, ang, counts, O.params(**dict(BF, seed=seed, **kw)), offsets=offs)
+        c, _ = e.run()
+        runs.append((e.x(), c[-1], e.nuisance()))
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
+    job = capi.Job(gpu_ctx, geom, ang, counts.astype(np.float32), init_offset=offs, seed=7,
+                   **dict(BF, **kw))
+    rs, cg = job.run()
+    assert rs.sweeps == 6 + 14 or rs.status == 0
+
+    def rms(a):
+        return np.sqrt(np.mean(a ** 2))
+
+    xbar = np.mean([r[0] for r in runs], axis=0)
+    spread = max(rms(r[0] - xbar) for r in runs)
+    xg = job.volume().astype(np.float64)
+    assert rms(xg - xbar) <= max(3.0 * spread, 1e-2 * rms(xbar)), (rms(xg - xbar), spread)
+    assert cg[-1] == pytest.approx(np.mean([r[1] for r in runs]), rel=2e-3)
+    mu = np.mean([r[2][1] for r in runs], axis=0)
+    al = np.mean([r[2][2] for r in runs], axis=0)
+    _, mug, alg = job.nuisance()
+    np.testing.assert_allclose(mug, mu, atol=2e-3)
+    np.testing.assert_allclose(alg, al, rtol=5e-2)

@@ -11,22 +11,7 @@ import pytest
 from scipy import optimize
 
 import oracle_py as O
-from mbirtem import phantom as P
-
-
-def bf_problem(n_r=32, n_t=6, n_views=25, max_deg=60.0, thickness=8, dose=2000.0, seed=3,
-               bragg_frac=0.0):
-    """Transmitted counts: Poisson(dose * exp(-A x)), optional bright (Bragg) outliers."""
-    ang = P.tilt_series(n_views, max_deg)
-    g = O.geometry(n_r, n_t, ang, thickness=thickness)
-    vol = 0.02 * P.random_ellipsoids((g.N_z, g.N_x, g.N_y), n=3, seed=seed).astype(np.float64)
-    eng = O.Engine(g, ang, np.zeros((n_views, n_r, n_t)), O.params(), x0=vol)
-    rng = np.random.default_rng(seed + 1)
-    counts = rng.poisson(dose * np.exp(-eng.yest())).astype(np.float64)
-    if bragg_frac > 0:
-        m = rng.uniform(size=counts.shape) < bragg_frac
-        counts[m] *= 0.3  # diffraction: far fewer transmitted counts than the model says
-    return g, ang, vol, np.maximum(counts, 1.0)
+from problems import bf_problem
 
 
 def bf_params(**kw):
