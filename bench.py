@@ -51,6 +51,10 @@ def parse():
                     help="independent slab problems for the batched and served figures (0 = skip)")
     ap.add_argument("--arrival-rate", type=float, default=4.0)
     ap.add_argument("--big-frac", type=float, default=0.05)
+    ap.add_argument("--slab-only-batched", action="store_true",
+                    help="only the batched run of the slab problems (no plain runs, no served stream)")
+    ap.add_argument("--no-bf", action="store_true", help="skip the bright-field figure")
+    ap.add_argument("--bf-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "icd_step_traffic.json"))
@@ -184,12 +188,21 @@ def server_runs(ctx, inputs, arrivals):
     return res, st
 
 
-def slab_figures(ctx, n, rate, big_frac):
+def slab_figures(ctx, n, rate, big_frac, only_batched=False):
     """batched_slabs (all n problems at once) and served (Poisson arrivals) figures."""
     import numpy as np
 
     from mbirtem import workloads
     specs = workloads.slab_mix(n, seed=7)
+    if only_batched:  # the server's launches alone (for profiling)
+        inputs = [workloads.make_slab(ctx, s) for s in specs]
+        t0 = time.perf_counter()
+        res, st = server_runs(ctx, inputs, [0.0] * n)
+        dt = time.perf_counter() - t0
+        visits = sum(r["visits"] for r in res)
+        log(f"batched slabs only: {n} jobs in {dt:.2f}s, {visits / dt:.4g} visits/s")
+        return {"jobs": n, "value": visits / dt, "unit": "voxel-updates/s",
+                "launches": int(st.launches)}, None
     rng = np.random.default_rng(11)
     served_specs = [workloads.slab_mix(1, seed=1000 + i, big_frac=1.0)[0]
                     if rng.uniform() < big_frac else s for i, s in enumerate(specs)]
@@ -268,6 +281,89 @@ def slab_figures(ctx, n, rate, big_frac):
     log(f"served: p50 {served['time_to_converged_p50_s']:.2f}s p99 "
         f"{served['time_to_converged_p99_s']:.2f}s vs plain FIFO p99 {served['plain_fifo_p99_s']:.2f}s")
     return batched, served
+
+
+def bf_figures(ctx, steps):
+    """Bright field 512x64x256, 121 views: throughput of sweep + offset + variance steps,
+    and the float32 engine's error against the FP64 oracle from identical states on
+    the 128x16x64, 61-view bright-field sample."""
+    import numpy as np
+
+    from mbirtem import capi, workloads
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_py as O
+
+    wl = workloads.BF_512x64x256
+    t0 = time.time()
+    w = workloads.make_bf(ctx, wl)
+    job, g = w["job"], w["geom"]
+    V = g.N_x * g.N_y * g.N_z
+    log(f"bf: setup {time.time() - t0:.1f}s")
+    job.sweep(0, 0)
+    job.sweep(0, 1)
+    job.set_bragg(workloads.BF_KW["bragg_T"])
+    t0 = time.perf_counter()
+    for k in range(steps):
+        job.sweep(1 + k, 0)
+        job.nuisance_update()
+    dt = time.perf_counter() - t0
+    value = V * steps / dt
+    log(f"bf: {steps} steps in {dt:.3f}s -> {value:.4g} visits/s")
+    job.close()
+
+    # float32 engine vs FP64 oracle from identical states
+    s = workloads.BF_128x16x64
+    ws = workloads.make_bf(ctx, s)
+    gs = O.geometry(s.n_r, s.n_t, ws["angles"], s.thickness)
+    kw = ws["job_kw"]
+    prm = O.params(bf_mode=1, mrf_c=kw["mrf_c"], sigma_x=kw["sigma_x"], mrf_q=kw["mrf_q"],
+                   bragg_T=kw["bragg_T"], bragg_delta=kw["bragg_delta"], joint_estimation=1,
+                   noise_model=1, num_outer=s.num_outer, num_inner=s.num_inner,
+                   stop_threshold=s.stop_threshold, seed=s.seed)
+    x = ws["volume"].astype(np.float64)
+    offs = np.full(s.n_views, -np.log(s.dose))
+    eng = O.Engine(gs, ws["angles"], ws["counts"].astype(np.float64), prm, x0=x, offsets=offs)
+    jb = capi.Job(ctx, ws["geom"], ws["angles"], ws["counts"], init_volume=ws["volume"],
+                  init_offset=offs, **{k: v for k, v in kw.items() if k != "default_offset"})
+    pr = eng.project(x)
+    pg = jb.forward_project(ws["volume"])
+    err = {"projection_rel": float(np.max(np.abs(pg - pr)) / np.max(np.abs(pr))),
+           "cost_rel_at_setup": abs(jb.cost() - eng.cost()) / abs(eng.cost())}
+    eng.bragg(kw["bragg_T"])
+    jb.set_bragg(kw["bragg_T"])
+    eng.joint_estimation()
+    eng.update_selector()
+    eng.update_weights()
+    eng.update_selector()
+    jb.nuisance_update()
+    _, mu, al = eng.nuisance()
+    _, mug, alg = jb.nuisance()
+    err["offset_step_abs"] = float(np.max(np.abs(mug - mu)))
+    err["variance_step_rel"] = float(np.max(np.abs(alg - al) / al))
+    err["cost_rel_after_steps"] = abs(jb.cost() - eng.cost()) / abs(eng.cost())
+    jb.close()
+    # whole runs (the update orders differ): volume RMS difference relative to the
+    # oracle's own seed-to-seed difference
+    e1 = O.Engine(gs, ws["angles"], ws["counts"].astype(np.float64), prm, offsets=offs)
+    e1.run()
+    prm2 = O.params(**{f: getattr(prm, f) for f, _ in O.Params._fields_})
+    prm2.seed = s.seed + 1
+    e2 = O.Engine(gs, ws["angles"], ws["counts"].astype(np.float64), prm2, offsets=offs)
+    e2.run()
+    jr = capi.Job(ctx, ws["geom"], ws["angles"], ws["counts"], init_offset=offs,
+                  **{k: v for k, v in kw.items() if k != "default_offset"})
+    jr.run(cap=0)
+    xr = e1.x()
+    rms = lambda a: float(np.sqrt(np.mean(a * a)))
+    err["run_volume_rel_rms"] = rms(jr.volume() - xr) / rms(xr)
+    err["oracle_seed_spread_rel_rms"] = rms(e2.x() - xr) / rms(xr)
+    jr.close()
+    ws["job"].close()
+    log(f"bf: fp32 vs fp64 {err}")
+    return {"workload": wl.name, "value": value, "unit": "voxel-updates/s",
+            "ms_per_step": 1e3 * dt / steps,
+            "step": "ICD sweep + offset (log dose) + shared variance update",
+            "fp32_vs_fp64": err, "fp32_vs_fp64_sample": s.name}
 
 
 def main():
@@ -361,7 +457,9 @@ def main():
         out["batched"] = batched(ctx, wl, args.batch, args.batch_steps, value)
     if args.slabs > 0:
         out["batched_slabs"], out["served"] = slab_figures(ctx, args.slabs, args.arrival_rate,
-                                                           args.big_frac)
+                                                           args.big_frac, args.slab_only_batched)
+    if not args.no_bf:
+        out["bright_field"] = bf_figures(ctx, args.bf_steps)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(wl, w["angles"], w["sinogram"], threads)

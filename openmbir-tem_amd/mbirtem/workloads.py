@@ -120,3 +120,55 @@ def make_slab(ctx, s, noise=0.02):
     y = P.noisy_sinogram(proj, noise, s.seed + 1).astype(np.float32)
     kw = dict(SLAB_KW, seed=s.seed % 100000)
     return dict(slab=s, geom=geom, angles=ang, volume=vol, sinogram=y, job_kw=kw)
+
+
+# Bright field, 512 x 64 x 256, 121 tilts over +-60 degrees: transmitted counts
+# Poisson(dose_k exp(-A x)) with a mean dose of 2000 per pixel (per-view dose
+# U[0.8, 1.2] x 2000, estimated as the per-view offset -log(dose)); random
+# ellipsoids with attenuation U[0.2, 1] x 0.01 per voxel.
+@dataclass
+class BFWorkload:
+    name: str
+    n_r: int
+    n_t: int
+    n_views: int
+    max_deg: float
+    thickness: float
+    n_objects: int
+    dose: float
+    dose_range: tuple
+    mu_scale: float
+    seed: int
+    num_outer: int
+    num_inner: int
+    stop_threshold: float
+
+
+BF_512x64x256 = BFWorkload("bf_512x64x256_121v", 512, 64, 121, 60.0, 128, 20, 2000.0, (0.8, 1.2),
+                           0.01, 3, 30, 10, 1e-3)
+BF_128x16x64 = BFWorkload("bf_128x16x64_61v", 128, 16, 61, 60.0, 32, 6, 2000.0, (0.8, 1.2),
+                          0.01, 5, 12, 6, 1e-4)
+BF_KW = dict(bf_mode=1, mrf_c=0.01, sigma_x=0.002, mrf_q=1.2, bragg_T=3.0, bragg_delta=0.5,
+             joint_estimation=1, noise_model=1)
+
+
+def make_bf(ctx, wl, **job_kw):
+    """Geometry, angles, phantom, counts and a ready bright-field job for workload wl."""
+    ang = P.tilt_series(wl.n_views, wl.max_deg)
+    geom = capi.geometry_init(wl.n_r, wl.n_t, ang, wl.thickness)
+    shape = (geom.N_z, geom.N_x, geom.N_y)
+    vol = wl.mu_scale * P.random_ellipsoids(shape, n=wl.n_objects, seed=wl.seed)
+    zero = np.zeros((wl.n_views, wl.n_r, wl.n_t), np.float32)
+    pj = capi.Job(ctx, geom, ang, zero)
+    proj = pj.forward_project(vol)
+    pj.close()
+    rng = np.random.default_rng(wl.seed + 7)
+    dose = wl.dose * rng.uniform(*wl.dose_range, wl.n_views)
+    counts = rng.poisson(dose[:, None, None] * np.exp(-proj)).astype(np.float32)
+    counts = np.maximum(counts, 1.0)
+    kw = dict(BF_KW, num_outer=wl.num_outer, num_inner=wl.num_inner,
+              stop_threshold=wl.stop_threshold, seed=wl.seed,
+              default_offset=-np.log(wl.dose))
+    kw.update(job_kw)
+    job = capi.Job(ctx, geom, ang, counts, **kw)
+    return dict(geom=geom, angles=ang, volume=vol, counts=counts, dose=dose, job=job, job_kw=kw)
