@@ -97,7 +97,9 @@ typedef struct mbir_problem {
   int32_t schedule;
   int32_t rows_per_step;
   /* sweep kernels: 0 = one launch per conflict-free step with the job's own kernel
-   * (plain per-job GPU ICD); 1 = the batched step kernel (a batch of one job) */
+   * (plain per-job GPU ICD); 1 = the batched step kernel (a batch of one job);
+   * 2 = reserved for a whole sweep in one persistent launch (refused with -3: not
+   * enabled in this build) */
   int32_t kernel_mode;
   /* Bright-field mode (BrightField/BFForwardModel.cpp, BFReconstructionEngine.cpp:575-872):
    * counts are transmitted counts; the measurement is -log(counts + bf_offset)

@@ -563,9 +563,9 @@ namespace {
 
 // Schedule of the job's next sweep, uploaded on the job stream with its
 // accumulators cleared.
-int prepare_sweep(mbir_job* j)
+int prepare_sweep(mbir_job* j, bool drop_empty_steps = false)
 {
-  build_schedule(j, false);
+  build_schedule(j, drop_empty_steps);
   int rc = prepare_sweep_on(j, j->stream, nullptr);
   if (rc < 0) return rc;
   HIP_TRY(hipMemcpyAsync(j->step_ptr_d.p, j->step_ptr.data(), j->step_ptr.size() * sizeof(int),
@@ -723,9 +723,72 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
   return MBIR_OK;
 }
 
+// One sweep in one persistent launch (kernel_mode 2): the same conflict-free steps
+// in the same order, each step's lines starting when the previous step has finished.
+// Not reachable: run_sweep refuses kernel_mode 2 (see there).
+[[maybe_unused]] int run_sweep_persistent(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
+{
+  hipStream_t s = j->stream;
+  // no empty steps: each step waits for the one before it
+  int rc = prepare_sweep(j, true);
+  if (rc < 0) return rc;
+  const int nsteps = (int)j->step_ptr.size() - 1;
+  const int nlines = j->step_ptr.back();
+  const size_t need = 2 + (size_t)std::max(nsteps, 1);
+  if (j->psync.n < need) {
+    j->psync.release();
+    HIP_TRY(j->psync.alloc(need));
+  }
+  HIP_TRY(launch_fill_u32(j->psync.p, 0u, need, s));
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  int ypl, nv;
+  icd_shape(j->J, j->J.vlr_max, &ypl, &nv);
+  // at most the resident workgroups, and no more than two steps can use: the others
+  // would only poll the completion counts
+  int widest = 1;
+  for (int k = 0; k < nsteps; ++k) widest = std::max(widest, j->step_ptr[k + 1] - j->step_ptr[k]);
+  const int grid = std::min(cus * icd_sweep_blocks_per_cu(ypl, nv, j->J.wfull != nullptr),
+                            2 * widest);
+  DevJob J = j->J;
+  J.e_atomic = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (j->timing) {
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
+  }
+  HIP_TRY(launch_icd_sweep(J, j->sched.p, j->step_ptr_d.p, nsteps, nlines, j->psync.p + 2,
+                           j->psync.p, (outer == 0 && inner == 0) ? 1 : 0,
+                           j->prob.zero_skipping, grid, s));
+  if (j->timing) HIP_TRY(hipEventRecord(e1, s));
+  unsigned int fail = 0;
+  HIP_TRY(hipMemcpyAsync(&fail, j->psync.p + 1, sizeof(fail), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0;
+  if (j->timing) {
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  if (fail) {
+    std::fprintf(stderr, "mbir: persistent sweep: a step wait passed its deadline\n");
+    return MBIR_ERR_HIP;
+  }
+  return finish_sweep(j, ms, st);
+}
+
 int run_sweep(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
 {
   if (j->cancel.load()) return MBIR_ERR_CANCELLED;
+  // the persistent one-launch sweep is switched off: its first GPU runs did not finish
+  // (the step waits never observed the completion counts); until the cause is shown,
+  // kernel_mode 2 is refused
+  if (j->prob.kernel_mode == 2) return MBIR_ERR_UNSUPPORTED;
   if (j->prob.kernel_mode == 1) {
     mbir_sweep_stats tmp{};
     std::vector<mbir_job*> one{j};

@@ -79,6 +79,7 @@ struct mbir_job {
   DevBuf<float> wfull;       // bright field: per-entry weights, sign = selector
   DevBuf<double> alpha_b;    // bright field: the next shared variance
   double bragg = 0;          // bright field: current Bragg threshold
+  DevBuf<unsigned int> psync;  // persistent sweep: {claim, fail} then per-step done counts
   std::vector<int> nnz_h;  // per-line column entries (host copy)
   int64_t nnz_total = 0;
   DevBuf<uint8_t> mask;
@@ -99,6 +100,7 @@ struct mbir_job {
     vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
     step_ptr_d.release(); mask.release(); acc.release(); col_ptr.release(); col_row.release();
     col_a.release(); col_ka.release(); col_kw.release(); wfull.release(); alpha_b.release();
+    psync.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };

@@ -136,6 +136,14 @@ hipError_t launch_cost(const DevJob& J, const double* alpha, int noise_model, do
 // one launch of the request server: kernel shape (ypl, nv), nwg workgroups
 hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& args, int nwg,
                            hipStream_t s);
+// One whole sweep of one job in a single persistent launch (the lines of
+// lines[step_ptr[k] .. step_ptr[k+1]) form step k); done[nsteps] and the 8-byte sync
+// block {claim, fail} must be zero; grid = workgroups (at most the resident ones).
+hipError_t launch_icd_sweep(const DevJob& J, const LineRef* lines, const int* step_ptr,
+                            int nsteps, int nlines, unsigned int* done, void* sync, int first,
+                            int zs, int grid, hipStream_t s);
+int icd_sweep_blocks_per_cu(int ypl, int nv, bool fw);
+hipError_t launch_fill_u32(unsigned int* p, unsigned int v, size_t n, hipStream_t s);
 // jobs[slot] = J, ordered on s (the descriptor travels as a kernel argument)
 hipError_t launch_set_devjob(DevJob* jobs, int slot, const DevJob& J, hipStream_t s);
 // bright field (BrightField/BFForwardModel.cpp): offset-only joint estimation

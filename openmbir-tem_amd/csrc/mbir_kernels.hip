@@ -113,6 +113,26 @@ struct Kahan {
   }
 };
 
+// Loads and stores of data that other workgroups of a persistent sweep wrote or will
+// read (error sinogram, volume, bright-field weights): device-scope relaxed atomics,
+// coherent across the XCDs' L2 caches without cache flushes (COH); plain otherwise.
+template <bool COH>
+__device__ inline float ld(const float* p)
+{
+  if constexpr (COH)
+    return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    return *p;
+}
+template <bool COH>
+__device__ inline void st(float* p, float v)
+{
+  if constexpr (COH)
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    *p = v;
+}
+
 __device__ inline float filter_weight(int dy, int dz, int dx)
 {
   // FILTER (HAADF/HAADF_ReconstructionEngine.cpp:266-294): faces 0.0523,
@@ -327,7 +347,7 @@ struct IcdSmem {
 
 // PF: issue the first theta batch's loads before the LDS staging (shorter line
 // latency, more registers: the form for launches that do not fill the device).
-template <int YPL, int NV, bool FW, bool PF>
+template <int YPL, int NV, bool FW, bool PF, bool COH = false>
 __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_on,
                          SweepAccum* acc, IcdSmem<YPL>& sm)
 {
@@ -379,7 +399,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       const int zz = z + l / 3 - 1, xx = xi + l % 3 - 1;
       const bool in = q < nnb && yy >= 0 && yy < Ny && zz >= 0 && zz < J.N_z && xx >= 0 &&
                       xx < J.N_x;
-      const float val = J.x[in ? ((size_t)zz * J.N_x + xx) * Ny + yy : 0];
+      const float val = ld<COH>(J.x + (in ? ((size_t)zz * J.N_x + xx) * Ny + yy : 0));
       nbv_[it] = in ? val : 0.f;
     }
   }
@@ -426,8 +446,8 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
         for (int k = 0; k < NV; ++k) {
           const bool use = ok && act[jy] && k < vn[jy];
           const int idx = use ? r0[u] + vt0[jy] + k : 0;
-          ev0[u][jy][k] = J.e[idx];
-          if constexpr (FW) wf0[u][jy][k] = J.wfull[idx];
+          ev0[u][jy][k] = ld<COH>(J.e + idx);
+          if constexpr (FW) wf0[u][jy][k] = ld<COH>(J.wfull + idx);
         }
     }
   }
@@ -493,8 +513,8 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
         for (int k = 0; k < NV; ++k) {
           const bool use = ok && act[jy] && k < vn[jy];
           const int idx = use ? rw + vt0[jy] + k : 0;
-          ev[u][jy][k] = J.e[idx];
-          if constexpr (FW) wf[u][jy][k] = J.wfull[idx];
+          ev[u][jy][k] = ld<COH>(J.e + idx);
+          if constexpr (FW) wf[u][jy][k] = ld<COH>(J.wfull + idx);
         }
     }
     accumulate(base, ev, wf);
@@ -581,7 +601,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
         nv = nv < dlo ? dlo : (nv > dhi ? dhi : nv);
         if (nv < 0.0) nv = 0.0;  // POSITIVITY_CONSTRAINT (:394-400)
         xnew = (float)nv;
-        J.x[lbase + y] = xnew;
+        st<COH>(J.x + lbase + y, xnew);
         vst = 1;
         if (roi) {
           su = fabs((double)xnew - (double)v);
@@ -627,8 +647,8 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
           for (int k = 0; k < NV; ++k) {
             const bool use = ok && upd[jy] && k < vn[jy];
             const int idx = use ? rw + vt0[jy] + k : 0;
-            ev[u][jy][k] = J.e[idx];
-            wv[u][jy][k] = J.wfull[idx];
+            ev[u][jy][k] = ld<COH>(J.e + idx);
+            wv[u][jy][k] = ld<COH>(J.wfull + idx);
           }
       }
 #pragma unroll
@@ -645,13 +665,13 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
                 const float en =
                     __fadd_rn(ev[u][jy][k], -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy]));
                 const float aw = fabsf(wv[u][jy][k]);
-                J.e[rwi + vt0[jy] + k] = en;
-                J.wfull[rwi + vt0[jy] + k] = fabsf(en * sqrtf(aw)) < J.bragg ? aw : -aw;
+                st<COH>(J.e + rwi + vt0[jy] + k, en);
+                st<COH>(J.wfull + rwi + vt0[jy] + k, fabsf(en * sqrtf(aw)) < J.bragg ? aw : -aw);
               }
         }
       }
     }
-  } else if (J.e_atomic) {
+  } else if (!COH && J.e_atomic) {
     for (int base = p; base < nc; base += P * U) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -683,7 +703,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
 #pragma unroll
           for (int k = 0; k < NV; ++k) {
             const bool use = ok && d[jy] != 0.f && k < vn[jy];
-            ev[u][jy][k] = J.e[use ? rw + vt0[jy] + k : 0];
+            ev[u][jy][k] = ld<COH>(J.e + (use ? rw + vt0[jy] + k : 0));
           }
       }
 #pragma unroll
@@ -698,8 +718,8 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
             for (int k = 0; k < NV; ++k)
               if (d[jy] != 0.f && k < vn[jy])
                 // the atomic form's roundings: product first, then one add (no fma)
-                J.e[rwi + vt0[jy] + k] =
-                    __fadd_rn(ev[u][jy][k], -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy]));
+                st<COH>(J.e + rwi + vt0[jy] + k,
+                        __fadd_rn(ev[u][jy][k], -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy])));
         }
       }
     }
@@ -799,6 +819,77 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_tick(const DevJob* __restrict
   const DevJob J = jobs[job];
   const LineRef ref = J.sched[l0 + i];
   icd_line<YPL, NV, false, PF>(J, ref, flags & 1, (flags >> 1) & 1, J.acc, sm);
+}
+
+// One ICD sweep of one job in a single launch.  Workgroups claim the sweep's lines in
+// schedule order from a device counter; a line of step k starts once every line of
+// step k - 1 has finished (a per-step completion count), which makes the sweep the
+// same sequence of conflict-free steps as one launch per step.  Claiming in order
+// means a line only ever waits for lines claimed before it by running workgroups, so
+// the waits always end; a wait longer than the deadline sets *fail and every
+// workgroup leaves.  Error sinogram and volume go through device-scope accesses.
+struct SweepSync {
+  unsigned int claim;  // next line
+  unsigned int fail;   // 1: a wait timed out
+};
+
+template <int YPL, int NV, bool FW>
+__global__ __launch_bounds__(kIcdBlock) void k_icd_sweep(DevJob J, const LineRef* __restrict__ lines,
+                                                         const int* __restrict__ step_ptr,
+                                                         int nsteps, unsigned int* done,
+                                                         SweepSync* sync, int first, int zs_on,
+                                                         unsigned long long deadline_ticks)
+{
+  __shared__ IcdSmem<YPL> sm;
+  __shared__ int s_q, s_k;
+  const int nl = step_ptr[nsteps];
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int q = (int)__hip_atomic_fetch_add(&sync->claim, 1u, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+      int k = 0;
+      if (q < nl) {
+        int lo = 0, hi = nsteps - 1;  // the step that holds line q
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (step_ptr[mid] <= q) lo = mid;
+          else hi = mid - 1;
+        }
+        k = lo;
+      }
+      if (q < nl && k > 0) {
+        const unsigned int need = (unsigned int)(step_ptr[k] - step_ptr[k - 1]);
+        // poll sparingly: every poll is a device-scope load of one shared line
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        unsigned int polls = 0;
+        while (__hip_atomic_load(done + (k - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+               need) {
+          // two bounds: the real-time clock and the poll count (each poll sleeps ~512
+          // clocks, so 2^23 polls are seconds) in case the clock does not advance
+          if ((++polls & 15u) == 0u &&
+              (__hip_atomic_load(&sync->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+               __builtin_amdgcn_s_memrealtime() - t0 > deadline_ticks || polls > (1u << 23))) {
+            __hip_atomic_store(&sync->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            q = nl;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(8);
+        }
+      }
+      s_q = q;
+      s_k = k;
+    }
+    __syncthreads();
+    const int q = s_q, k = s_k;
+    if (q >= nl) break;
+    const LineRef ref = lines[q];
+    icd_line<YPL, NV, FW, true, true>(J, ref, first, zs_on, J.acc, sm);
+    // every store of this line has completed (device scope) before the count moves
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_add(done + k, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // jobs[slot] = J: a job descriptor written in stream order.
@@ -1103,6 +1194,10 @@ __global__ void k_fill_f32(float* p, float v, size_t n)
 {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ void k_fill_u32(unsigned int* p, unsigned int v, size_t n)
+{
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
 __global__ void k_fill_f64(double* p, double v, size_t n)
 {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -1249,6 +1344,56 @@ hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& 
   return hipGetLastError();
 }
 
+hipError_t launch_icd_sweep(const DevJob& J, const LineRef* lines, const int* step_ptr,
+                            int nsteps, int nlines, unsigned int* done, void* sync, int first,
+                            int zs, int grid, hipStream_t s)
+{
+  if (nlines <= 0 || nsteps <= 0) return hipSuccess;
+  int ypl, nv;
+  icd_shape(J, J.vlr_max, &ypl, &nv);
+  const unsigned long long deadline = 400000000ull;  // 4 s of the 100 MHz real-time clock
+  SweepSync* sy = (SweepSync*)sync;
+  const int g = grid < nlines ? grid : nlines;
+#define MBIR_SWEEP(Y, N, F)                                                                    \
+  hipLaunchKernelGGL((k_icd_sweep<Y, N, F>), dim3(g), dim3(kIcdBlock), 0, s, J, lines, step_ptr, \
+                     nsteps, done, sy, first, zs, deadline)
+  if (J.wfull) {
+    if (nv != 1 || ypl > 2) return hipErrorInvalidValue;
+    if (ypl == 1) MBIR_SWEEP(1, 1, true);
+    else MBIR_SWEEP(2, 1, true);
+  } else if (ypl == 1 && nv == 1) {
+    MBIR_SWEEP(1, 1, false);
+  } else if (ypl == 1 && nv == 2) {
+    MBIR_SWEEP(1, 2, false);
+  } else if (ypl == 2 && nv == 1) {
+    MBIR_SWEEP(2, 1, false);
+  } else if (ypl == 2 && nv == 2) {
+    MBIR_SWEEP(2, 2, false);
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef MBIR_SWEEP
+  return hipGetLastError();
+}
+
+int icd_sweep_blocks_per_cu(int ypl, int nv, bool fw)
+{
+  int n = 0;
+  hipError_t e = hipErrorInvalidValue;
+  if (fw)
+    e = ypl == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<1, 1, true>, kIcdBlock, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<2, 1, true>, kIcdBlock, 0);
+  else if (ypl == 1)
+    e = nv == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<1, 1, false>, kIcdBlock, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<1, 2, false>, kIcdBlock, 0);
+  else
+    e = nv == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<2, 1, false>, kIcdBlock, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<2, 2, false>, kIcdBlock, 0);
+  return e == hipSuccess && n > 0 ? n : 1;
+}
+
+hipError_t launch_fill_u32(unsigned int* p, unsigned int v, size_t n, hipStream_t s);
+
 hipError_t launch_set_devjob(DevJob* jobs, int slot, const DevJob& J, hipStream_t s)
 {
   hipLaunchKernelGGL(k_set_devjob, dim3(1), dim3(64), 0, s, jobs, slot, J);
@@ -1318,6 +1463,12 @@ hipError_t launch_bf_noise(const DevJob& J, const double* alpha_old, double* alp
                      alpha_new);
   hipLaunchKernelGGL(k_bf_noise_apply, dim3(grid_for(M)), dim3(kBlock), 0, s, J, alpha_old,
                      (const double*)alpha_new);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(unsigned int* p, unsigned int v, size_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(kBlock), 0, s, p, v, n);
   return hipGetLastError();
 }
 

@@ -57,3 +57,31 @@ def test_batch_cost_monotone(gpu_ctx):
         for a, b in zip(last, now):
             assert b <= a * (1 + 1e-6)
         last = now
+
+
+@pytest.mark.skip(reason="the persistent one-launch sweep (kernel_mode 2) did not finish on "
+                         "the GPU twice; the path is switched off (mbir_job_create accepts the "
+                         "mode but sweeps return -3) until the cause of the unfinished step "
+                         "waits is shown")
+def test_persistent_sweep_equals_per_step(gpu_ctx):
+    """kernel_mode 2 runs each sweep in one persistent launch: the same steps in the
+    same order, so the same bits as one launch per step."""
+    for prob in (dict(), dict(n_r=64, n_t=16, n_views=45, max_deg=70.0, thickness=16, seed=8)):
+        g, ang, vol, y = small_problem(**prob)
+        a = _job(gpu_ctx, g, ang, y, kernel_mode=0, seed=4)
+        b = _job(gpu_ctx, g, ang, y, kernel_mode=2, seed=4)
+        for it in range(6):
+            _, sa = a.sweep(0, it)
+            _, sb = b.sweep(0, it)
+            assert sa.updated > 0
+            assert (sa.updated, sa.zero_skipped, sa.status) == (sb.updated, sb.zero_skipped, sb.status)
+        np.testing.assert_array_equal(a.volume(), b.volume())
+        np.testing.assert_array_equal(a.error_sinogram(), b.error_sinogram())
+
+
+def test_persistent_sweep_is_refused(gpu_ctx):
+    g, ang, vol, y = small_problem()
+    b = _job(gpu_ctx, g, ang, y, kernel_mode=2, seed=4)
+    with pytest.raises(capi.MbirError) as ei:
+        b.sweep(0, 0)
+    assert ei.value.code == -3
