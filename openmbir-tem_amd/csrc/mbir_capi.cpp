@@ -266,7 +266,8 @@ int setup_job(mbir_job* j)
   const int NB = (int)std::ceil(2.0 * g.delta_xz / g.delta_r - 1e-12) + 1;
   if (NB > kMaxNB || NT * NB > kMaxSlots || g.N_y > kMaxNy || NT > kIcdBlock)
     return MBIR_ERR_UNSUPPORTED;
-  if ((double)M >= 2147483647.0) return MBIR_ERR_UNSUPPORTED;
+  // the line update addresses the error sinogram by 32-bit byte offsets
+  if (((double)M + g.N_t) * 4.0 >= 4294967295.0) return MBIR_ERR_UNSUPPORTED;
 
   // conflict-free spacing: lines of one row dx apart never share a detector bin
   // if dx * |cos th| >= 2 * 64 * OffsetR for every view (H support radius).
@@ -309,7 +310,9 @@ int setup_job(mbir_job* j)
   // ---- device state ----
   hipStream_t s = j->stream;
   HIP_TRY(j->x.alloc(V));
-  HIP_TRY(j->e.alloc(M));
+  // a spare detector row behind e (and wfull): padding entries of the line update
+  HIP_TRY(j->e.alloc(M + g.N_t));
+  HIP_TRY(launch_fill_f32(j->e.p + M, 0.f, g.N_t, s));
   HIP_TRY(j->y.alloc(M));
   HIP_TRY(j->wview.alloc(NT));
   HIP_TRY(j->vlr_val.alloc((size_t)g.N_y * kVlrCap));
@@ -328,7 +331,6 @@ int setup_job(mbir_job* j)
   HIP_TRY(j->mask.alloc(L));
   HIP_TRY(j->line_nnz.alloc(L));
   HIP_TRY(j->sched.alloc(L));
-  HIP_TRY(j->step_ptr_d.alloc(L + 1));
   HIP_TRY(j->acc.alloc(1));
   HIP_TRY(j->colv.alloc(kMaxSlots));
   HIP_TRY(j->colr.alloc(kMaxSlots));
@@ -354,7 +356,8 @@ int setup_job(mbir_job* j)
       bf_y[i] = (float)yv;
       bf_w[i] = (float)w;  // positive: every entry selected (:347)
     }
-    HIP_TRY(j->wfull.alloc(M));
+    HIP_TRY(j->wfull.alloc(M + g.N_t));
+    HIP_TRY(launch_fill_f32(j->wfull.p + M, 0.f, g.N_t, s));
     HIP_TRY(j->alpha_b.alloc(NT));
     HIP_TRY(hipMemcpyAsync(j->wfull.p, bf_w.data(), M * sizeof(float), hipMemcpyHostToDevice, s));
     j->bytes += (int64_t)M * 4;
@@ -392,6 +395,7 @@ int setup_job(mbir_job* j)
   J.cosv = j->cosv.p; J.sinv = j->sinv.p; J.H = j->Hd.p;
   J.vlr_t0 = j->vlr_t0.p; J.vlr_n = j->vlr_n.p; J.vlr_val = j->vlr_val.p;
   J.mask = j->mask.p; J.line_nnz = j->line_nnz.p;
+  J.spare_b = (uint32_t)(M * 4);
   J.x = j->x.p; J.e = j->e.p; J.y = j->y.p; J.wfull = nullptr; J.wview = j->wview.p;
   J.I0 = j->I0.p; J.mu = j->mu.p;
   J.sched = j->sched.p; J.acc = j->acc.p;
@@ -425,6 +429,17 @@ int setup_job(mbir_job* j)
   if (tot >= 2147483647LL) return MBIR_ERR_UNSUPPORTED;
   cptr[L] = (int)tot;
   j->nnz_total = tot;
+  {
+    // LDS of the line update: the longest column padded to a multiple of 4 P entries,
+    // P = 8 * 64 / TY streams for TY = min(64, the power of two >= N_y)
+    int ty = 1;
+    while (ty < g.N_y && ty < 64) ty <<= 1;
+    const int q = 4 * kIcdWaves * (64 / ty);
+    int nmax = 1;
+    for (size_t l = 0; l < L; ++l) nmax = std::max(nmax, j->nnz_h[l]);
+    J.col_slots = (nmax + q - 1) / q * q;
+    if (J.col_slots > kMaxSlots) return MBIR_ERR_UNSUPPORTED;
+  }
   j->cptr_h = cptr;
   HIP_TRY(j->col_ptr.alloc(L + 1));
   HIP_TRY(j->col_row.alloc((size_t)tot));
@@ -567,10 +582,7 @@ int prepare_sweep(mbir_job* j, bool drop_empty_steps = false)
 {
   build_schedule(j, drop_empty_steps);
   int rc = prepare_sweep_on(j, j->stream, nullptr);
-  if (rc < 0) return rc;
-  HIP_TRY(hipMemcpyAsync(j->step_ptr_d.p, j->step_ptr.data(), j->step_ptr.size() * sizeof(int),
-                         hipMemcpyHostToDevice, j->stream));
-  return MBIR_OK;
+  return rc < 0 ? rc : MBIR_OK;
 }
 
 // Read the accumulators and apply the stop test.
@@ -698,9 +710,13 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
     const int eend = l_e0[k] + l_ne[k];
     while (e < eend) {
       int f = e, wg = 0;
-      while (f < eend && ent[f].ypl == ent[e].ypl && ent[f].nv == ent[e].nv) wg += ent[f++].nlines;
+      int slots = 4;
+      while (f < eend && ent[f].ypl == ent[e].ypl && ent[f].nv == ent[e].nv) {
+        slots = std::max(slots, jobs[ent[f].job]->J.col_slots);
+        wg += ent[f++].nlines;
+      }
       HIP_TRY(launch_icd_batch_step(ent[e].ypl, ent[e].nv, ctx->djobs.p, ctx->ents.p, e, f - e,
-                                    wg, s));
+                                    wg, slots, s));
       e = f;
     }
   }
@@ -723,71 +739,12 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
   return MBIR_OK;
 }
 
-// One sweep in one persistent launch (kernel_mode 2): the same conflict-free steps
-// in the same order, each step's lines starting when the previous step has finished.
-// Not reachable: run_sweep refuses kernel_mode 2 (see there).
-[[maybe_unused]] int run_sweep_persistent(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
-{
-  hipStream_t s = j->stream;
-  // no empty steps: each step waits for the one before it
-  int rc = prepare_sweep(j, true);
-  if (rc < 0) return rc;
-  const int nsteps = (int)j->step_ptr.size() - 1;
-  const int nlines = j->step_ptr.back();
-  const size_t need = 2 + (size_t)std::max(nsteps, 1);
-  if (j->psync.n < need) {
-    j->psync.release();
-    HIP_TRY(j->psync.alloc(need));
-  }
-  HIP_TRY(launch_fill_u32(j->psync.p, 0u, need, s));
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  }
-  int ypl, nv;
-  icd_shape(j->J, j->J.vlr_max, &ypl, &nv);
-  // at most the resident workgroups, and no more than two steps can use: the others
-  // would only poll the completion counts
-  int widest = 1;
-  for (int k = 0; k < nsteps; ++k) widest = std::max(widest, j->step_ptr[k + 1] - j->step_ptr[k]);
-  const int grid = std::min(cus * icd_sweep_blocks_per_cu(ypl, nv, j->J.wfull != nullptr),
-                            2 * widest);
-  DevJob J = j->J;
-  J.e_atomic = 0;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (j->timing) {
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, s));
-  }
-  HIP_TRY(launch_icd_sweep(J, j->sched.p, j->step_ptr_d.p, nsteps, nlines, j->psync.p + 2,
-                           j->psync.p, (outer == 0 && inner == 0) ? 1 : 0,
-                           j->prob.zero_skipping, grid, s));
-  if (j->timing) HIP_TRY(hipEventRecord(e1, s));
-  unsigned int fail = 0;
-  HIP_TRY(hipMemcpyAsync(&fail, j->psync.p + 1, sizeof(fail), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  float ms = 0;
-  if (j->timing) {
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-  }
-  if (fail) {
-    std::fprintf(stderr, "mbir: persistent sweep: a step wait passed its deadline\n");
-    return MBIR_ERR_HIP;
-  }
-  return finish_sweep(j, ms, st);
-}
-
 int run_sweep(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
 {
   if (j->cancel.load()) return MBIR_ERR_CANCELLED;
-  // the persistent one-launch sweep is switched off: its first GPU runs did not finish
-  // (the step waits never observed the completion counts); until the cause is shown,
-  // kernel_mode 2 is refused
+  // a whole sweep in one persistent launch is not offered: its first form did not
+  // finish on the device (its step waits never saw the completion counts) and was
+  // taken out; kernel_mode 2 is refused
   if (j->prob.kernel_mode == 2) return MBIR_ERR_UNSUPPORTED;
   if (j->prob.kernel_mode == 1) {
     mbir_sweep_stats tmp{};

@@ -73,13 +73,12 @@ struct mbir_job {
   // device buffers
   DevBuf<float> x, e, y, wview, vlr_val, colv;
   DevBuf<double> cosv, sinv, Hd, I0, mu, alpha, I0b, mub, sums6, yest, cost, prj;
-  DevBuf<int> vlr_t0, vlr_n, line_nnz, colr, step_ptr_d, col_ptr, col_row;
+  DevBuf<int> vlr_t0, vlr_n, line_nnz, colr, col_ptr, col_row;
   DevBuf<LineRef> sched;
   DevBuf<float> col_a, col_ka, col_kw;
   DevBuf<float> wfull;       // bright field: per-entry weights, sign = selector
   DevBuf<double> alpha_b;    // bright field: the next shared variance
   double bragg = 0;          // bright field: current Bragg threshold
-  DevBuf<unsigned int> psync;  // persistent sweep: {claim, fail} then per-step done counts
   std::vector<int> nnz_h;  // per-line column entries (host copy)
   int64_t nnz_total = 0;
   DevBuf<uint8_t> mask;
@@ -98,9 +97,8 @@ struct mbir_job {
     cosv.release(); sinv.release(); Hd.release(); I0.release(); mu.release(); alpha.release();
     I0b.release(); mub.release(); sums6.release(); yest.release(); cost.release(); prj.release();
     vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
-    step_ptr_d.release(); mask.release(); acc.release(); col_ptr.release(); col_row.release();
+    mask.release(); acc.release(); col_ptr.release(); col_row.release();
     col_a.release(); col_ka.release(); col_kw.release(); wfull.release(); alpha_b.release();
-    psync.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };

@@ -77,6 +77,10 @@ struct DevJob {
   int e_atomic;          // 1: error-sinogram update by fp32 atomics (required when lines of
                          // a step may share detector bins: relaxed schedule), 0: plain
                          // read-modify-write (conflict-free steps only)
+  int col_slots;         // padded column entries of this job's longest line (a multiple of
+                         // 4 P, P = 8 * 64 / TY streams): the line update's dynamic LDS
+  uint32_t spare_b;      // M * 4: byte offset of the spare detector row (N_t zeros) that
+                         // follows e and wfull, the target of padding column entries
   const LineRef* sched;  // [N_z * N_x] lines of the current sweep, step by step
   SweepAccum* acc;       // the current sweep's accumulators
 };
@@ -105,6 +109,7 @@ struct TickEntry {
 constexpr int kTickMax = 96;  // entries per launch, passed by value as a kernel argument
 struct TickArgs {
   int n;
+  int slots;  // column entries (after padding) of the launch's longest line: dynamic LDS
   TickEntry e[kTickMax];
 };
 
@@ -118,7 +123,7 @@ void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv);
 // one step of several jobs of one kernel shape: entries [e0, e0 + ne) of ent,
 // nwg workgroups in total
 hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const BatchEntry* ent,
-                                 int e0, int ne, int nwg, hipStream_t s);
+                                 int e0, int ne, int nwg, int slots, hipStream_t s);
 hipError_t launch_build_columns(const DevJob& J, int* row, float* a, hipStream_t s);
 hipError_t launch_column_weights(const DevJob& J, float* ka, float* kw, int nnz, hipStream_t s);
 hipError_t launch_export_column(const DevJob& J, int z, int x, float* vals, int* rows,
@@ -136,13 +141,6 @@ hipError_t launch_cost(const DevJob& J, const double* alpha, int noise_model, do
 // one launch of the request server: kernel shape (ypl, nv), nwg workgroups
 hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& args, int nwg,
                            hipStream_t s);
-// One whole sweep of one job in a single persistent launch (the lines of
-// lines[step_ptr[k] .. step_ptr[k+1]) form step k); done[nsteps] and the 8-byte sync
-// block {claim, fail} must be zero; grid = workgroups (at most the resident ones).
-hipError_t launch_icd_sweep(const DevJob& J, const LineRef* lines, const int* step_ptr,
-                            int nsteps, int nlines, unsigned int* done, void* sync, int first,
-                            int zs, int grid, hipStream_t s);
-int icd_sweep_blocks_per_cu(int ypl, int nv, bool fw);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, size_t n, hipStream_t s);
 // jobs[slot] = J, ordered on s (the descriptor travels as a kernel argument)
 hipError_t launch_set_devjob(DevJob* jobs, int slot, const DevJob& J, hipStream_t s);

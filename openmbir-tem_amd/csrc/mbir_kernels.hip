@@ -113,26 +113,6 @@ struct Kahan {
   }
 };
 
-// Loads and stores of data that other workgroups of a persistent sweep wrote or will
-// read (error sinogram, volume, bright-field weights): device-scope relaxed atomics,
-// coherent across the XCDs' L2 caches without cache flushes (COH); plain otherwise.
-template <bool COH>
-__device__ inline float ld(const float* p)
-{
-  if constexpr (COH)
-    return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  else
-    return *p;
-}
-template <bool COH>
-__device__ inline void st(float* p, float v)
-{
-  if constexpr (COH)
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  else
-    *p = v;
-}
-
 __device__ inline float filter_weight(int dy, int dz, int dx)
 {
   // FILTER (HAADF/HAADF_ReconstructionEngine.cpp:266-294): faces 0.0523,
@@ -316,140 +296,220 @@ __global__ void k_column_weights(DevJob J, float* ka, float* kw, int nnz)
 // couples y and y +- 1, so even slices are solved first and odd slices second,
 // which makes one line update equal to a sequential order of its voxels.
 //   1./2./3. one memory round trip brings the 3x3 block of lines around (z, x)
-//      and the line's column (from the table built at setup) into LDS, and the
-//      sinogram rows of each lane's first theta batch, whose error-sinogram
-//      loads are then in flight while LDS is written;
-//   4. theta1/theta2: every wave streams a share of the column's (theta, r)
-//      detector rows, lanes running along the contiguous row t, U rows in flight
-//      per lane, Kahan-compensated fp32 partial sums combined in FP64 (lane
-//      shuffles within a wave, LDS across waves);
+//      (one wave per line, lanes along y) and the line's column (from the table
+//      built at setup) into LDS, and the sinogram rows of each lane's first theta
+//      batch, whose error-sinogram loads are then in flight while LDS is written;
+//   4. theta1/theta2: the column is cut into contiguous streams of entries, one
+//      or more per wave; lanes run along the contiguous detector row t, U entries
+//      in flight per lane, addresses as 32-bit byte offsets from a scalar base,
+//      four entries' offsets and gains per 16-byte LDS read; Kahan-compensated
+//      fp32 partial sums combined in FP64 (lane shuffles within a wave, LDS
+//      across waves);
 //   5. the voxel solve of each parity runs on lane groups: the 26 neighbour
 //      terms of one voxel are spread over G lanes and combined by DPP moves;
-//   6. e -= I0 a vlr dx: fp32 atomics, or a plain read-modify-write where the
-//      caller asks for it (no two lines of a conflict-free step share a detector
-//      bin, so both give the same e; under a full device the plain form is faster);
+//   6. e -= I0 a vlr dx: no two lines of a conflict-free step share a detector
+//      bin, so the first theta batch's entries are stored from the values read
+//      for theta; further entries by fp32 atomics (one job alone) or a plain
+//      read-modify-write (full batches), all rounding alike;
 //   7. per-slice statistics reduced by one wave, one set of global atomics.
+// Byte-offset access through a uniform base pointer: base in scalar registers plus
+// a 32-bit offset in one vector register (global load/store with a scalar base).
+__device__ inline float ldo(const float* base, uint32_t boff)
+{
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + boff);
+}
+__device__ inline float* pto(float* base, uint32_t boff)
+{
+  return reinterpret_cast<float*>(reinterpret_cast<char*>(base) + boff);
+}
+
+// Per-line LDS of the line update for lines of at most 64 YPL voxels.  The column
+// (row offsets and gains) lives in dynamic LDS sized per launch (IcdCol).
 template <int YPL>
 struct IcdSmem {
-  float ka[kMaxSlots];   // I0[th] * a of the line's column entries
-  float kw[kMaxSlots];   // I0[th] * a * w[th]
-  int row[kMaxSlots];    // (th * N_r + r) * N_t
-  float nbl[9][kMaxNy + 2];  // lines (z+dz, x+dx), y = -1 .. N_y (0 outside the volume)
-  float x[kMaxNy];       // this line; the even phase writes its new values here
-  float dx[kMaxNy];
-  double T1[kMaxNy], T2[kMaxNy];
+  static constexpr int NY = 64 * YPL;
+  float nbl[9][NY + 2];  // lines (z+dz, x+dx), y = -1 .. N_y (0 outside the volume)
+  float x[NY];           // this line; the even phase writes its new values here
+  float dx[NY];
+  double T1[NY], T2[NY];
   // theta partial sums per wave and slice (FP64)
-  double pd1[kIcdWaves][64 * YPL], pd2[kIcdWaves][64 * YPL];
-  double su[kMaxNy], sv[kMaxNy];  // per slice: ROI |x_new - x_old|, |x_old|
-  unsigned char vst[kMaxNy];      // per slice: 1 updated, 2 zero-skipped
-  unsigned char vns[kMaxNy];      // per slice: detector rows (VLR size)
+  double pd1[kIcdWaves][NY], pd2[kIcdWaves][NY];
+  double su[NY], sv[NY];      // per slice: ROI |x_new - x_old|, |x_old|
+  unsigned char vst[NY];      // per slice: 1 updated, 2 zero-skipped
+  unsigned char vns[NY];      // per slice: detector rows (VLR size)
 };
+
+// The line's column in dynamic LDS, entries padded to a multiple of four per stream:
+// slots entries each of rb (byte offset of the entry's detector row, (th N_r + r) N_t 4),
+// ka = I0[th] a and kw = I0[th] a w[th].  Every line of a launch has at most `slots`
+// entries after padding (the host sizes the launch by its jobs' col_slots).
+struct IcdCol {
+  uint32_t* rb;
+  float* ka;
+  float* kw;
+};
+extern __shared__ __align__(16) uint32_t icd_dyn[];
+__device__ inline IcdCol icd_col(int slots)
+{
+  return IcdCol{icd_dyn, reinterpret_cast<float*>(icd_dyn + slots),
+                reinterpret_cast<float*>(icd_dyn + 2 * slots)};
+}
+// dynamic LDS bytes of a launch whose lines need at most `slots` padded entries
+inline size_t icd_col_bytes(int slots) { return (size_t)slots * 12; }
+
+// Waves per SIMD the batched line kernels are compiled for: the form that fills the
+// device at least 6 (80 VGPRs), the prefetching form 5, wide shapes 4, and no more
+// than the LDS of a CU (160 KiB) allows with 8-wave workgroups.
+template <int YPL, int NV>
+constexpr int icd_waves_per_eu(bool pf)
+{
+  // the static part and a column of 1024 entries (273 views at 3 bins per view)
+  const int by_lds = 2 * (160 * 1024 / ((int)sizeof(IcdSmem<YPL>) + 1024 * 12));
+  const int want = YPL * NV >= 4 ? 4 : (pf ? 5 : 6);
+  return by_lds < want ? by_lds : want;
+}
 
 // PF: issue the first theta batch's loads before the LDS staging (shorter line
 // latency, more registers: the form for launches that do not fill the device).
-template <int YPL, int NV, bool FW, bool PF, bool COH = false>
+template <int YPL, int NV, bool FW, bool PF>
 __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_on,
-                         SweepAccum* acc, IcdSmem<YPL>& sm)
+                         SweepAccum* acc, IcdSmem<YPL>& sm, const IcdCol col)
 {
-  constexpr int U = YPL == 1 ? 24 : (YPL == 2 ? 12 : 6);  // detector rows in flight per lane
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // entries per lane and batch, a multiple of four (about 24 loads in flight per lane)
+  constexpr int UR = 24 / (YPL * NV);
+  constexpr int U = UR < 4 ? 4 : (UR + 3) / 4 * 4;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int Ny = J.N_y;
   const int line = __builtin_amdgcn_readfirstlane(ref.line);
   const int z = line / J.N_x, xi = line % J.N_x;
   const bool roi = J.mask[line] != 0;  // used by the solve
   const size_t lbase = (size_t)line * Ny;
   const bool zs_active = zs_on && !first;
+  const int p0 = __builtin_amdgcn_readfirstlane(ref.p0);
+  const int nc = __builtin_amdgcn_readfirstlane(ref.nc);
 
-  // ---- 1. this lane's slices for the theta and update passes: lanes run along the
-  // TY-wide (power of two >= N_y, at most 64) slice index, 64 / TY column entries per wave ----
-  int TY = 1;
-  while (TY < Ny && TY < 64) TY <<= 1;
-  const int RPW = 64 / TY, sub = lane / TY, yl = lane % TY;
+  // ---- 1. lanes run along the TY-wide (power of two >= N_y, at most 64) slice index;
+  // a wave holds 64 / TY streams of column entries, stream p owns the contiguous
+  // entries [p C, p C + cnt) of the column; the entries [nc, P C) are padding that
+  // point at the spare detector row behind the error sinogram with zero gains, so
+  // whole groups of four entries are loaded and updated without per-entry tests ----
+  int lg = 0;
+  while ((1 << lg) < Ny && lg < 6) ++lg;
+  const int TY = 1 << lg, RPW = 64 >> lg, sub = lane >> lg, yl = lane & (TY - 1);
   const int P = kIcdWaves * RPW, p = wave * RPW + sub;
-  int vt0[YPL], vn[YPL];
-  float vv[YPL][NV];
-  bool act[YPL];
+  const int C = (((nc + P - 1) >> (3 + 6 - lg)) + 3) & ~3;  // entries per stream, 4 | C
+  const int ntot = P * C;                                     // <= kMaxSlots
+  const int s0 = p * C;
+  const int cnt = min(max(nc - s0, 0), C);
+  uint32_t vtb[YPL][NV];  // byte offset of the lane's detector row position t
+  float vv[YPL][NV];      // 0 where the lane has no entry: it then adds nothing
+  bool vok[YPL][NV];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
     const int y = yl + TY * jy;
-    act[jy] = y < Ny;
-    const int yc = act[jy] ? y : 0;
-    vt0[jy] = J.vlr_t0[yc];
-    vn[jy] = act[jy] ? J.vlr_n[yc] : 0;
+    const bool act = y < Ny;
+    const int yc = act ? y : 0;
+    const int t0 = J.vlr_t0[yc];
+    const int n = act ? J.vlr_n[yc] : 0;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) vv[jy][k] = J.vlr_val[yc * kVlrCap + k];
-  }
-
-  // ---- 2. one round trip: the 3x3 neighbour lines, the column table (for LDS) and
-  // the sinogram rows of this lane's first theta batch ----
-  const int W = Ny + 2, nnb = 9 * W;
-  const int p0 = __builtin_amdgcn_readfirstlane(ref.p0);
-  const int nc = __builtin_amdgcn_readfirstlane(ref.nc);
-  constexpr int kNbIt = (9 * (kMaxNy + 2) + kIcdBlock - 1) / kIcdBlock;
-  constexpr int kColIt = kMaxSlots / kIcdBlock;
-  auto entry = [&](int i) { return nc > 0 ? p0 + (i < nc ? i : nc - 1) : 0; };
-  float nbv_[kNbIt];
-  int rw_[kColIt];
-  float ka_[kColIt], kw_[kColIt];
-#pragma unroll
-  for (int it = 0; it < kNbIt; ++it) {
-    if (it * kIcdBlock < nnb) {  // uniform
-      const int q = tid + it * kIcdBlock;
-      const int l = q / W, yy = q - l * W - 1;
-      const int zz = z + l / 3 - 1, xx = xi + l % 3 - 1;
-      const bool in = q < nnb && yy >= 0 && yy < Ny && zz >= 0 && zz < J.N_z && xx >= 0 &&
-                      xx < J.N_x;
-      const float val = ld<COH>(J.x + (in ? ((size_t)zz * J.N_x + xx) * Ny + yy : 0));
-      nbv_[it] = in ? val : 0.f;
+    for (int k = 0; k < NV; ++k) {
+      vok[jy][k] = k < n;
+      vv[jy][k] = k < n ? J.vlr_val[yc * kVlrCap + k] : 0.f;
+      vtb[jy][k] = k < n ? (uint32_t)(t0 + k) * 4u : 0u;
     }
   }
+
+  // ---- 2. one round trip: the 3x3 neighbour lines (wave w: line w, wave 0 also
+  // line 8, lanes along y), the column table (for LDS) and, PF, the detector rows of
+  // this lane's first batch ----
+  float nbv[2][YPL];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int l = wave + 8 * h;
+    const int zz = z + l / 3 - 1, xx = xi + l % 3 - 1;
+    const bool in = l < 9 && zz >= 0 && zz < J.N_z && xx >= 0 && xx < J.N_x;
+    const float* src = J.x + ((size_t)(in ? zz : 0) * J.N_x + (in ? xx : 0)) * Ny;
+#pragma unroll
+    for (int jy = 0; jy < YPL; ++jy) {
+      const int y = lane + 64 * jy;
+      nbv[h][jy] = (in && y < Ny) ? src[y] : 0.f;
+    }
+  }
+  constexpr int kColIt = kMaxSlots / kIcdBlock;
+  uint32_t rw_[kColIt];
+  float ka_[kColIt], kw_[kColIt];
 #pragma unroll
   for (int it = 0; it < kColIt; ++it) {
     if (it * kIcdBlock < nc) {  // uniform
-      const int ic = entry(tid + it * kIcdBlock);
-      rw_[it] = J.col_row[ic];
+      const int i = tid + it * kIcdBlock;
+      const int ic = p0 + (i < nc ? i : nc - 1);
+      rw_[it] = (uint32_t)J.col_row[ic] * 4u;
       ka_[it] = J.col_ka[ic];
       kw_[it] = J.col_kw[ic];
     }
   }
-  int r0[PF ? U : 1];
+  uint32_t r0[PF ? U : 1];
   if constexpr (PF) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) r0[u] = J.col_row[entry(p + u * P)];
+    for (int u = 0; u < U; ++u) r0[u] = u < cnt ? (uint32_t)J.col_row[p0 + s0 + u] * 4u : J.spare_b;
   }
   const int vn_t = J.vlr_n[tid < Ny ? tid : 0];
 #pragma unroll
-  for (int it = 0; it < kNbIt; ++it) {
-    const int q = tid + it * kIcdBlock;
-    const int l = q / W, yy = q - l * W - 1;
-    if (it * kIcdBlock < nnb && q < nnb) sm.nbl[l][yy + 1] = nbv_[it];
+  for (int h = 0; h < 2; ++h) {
+    const int l = wave + 8 * h;
+    if (l < 9) {
+#pragma unroll
+      for (int jy = 0; jy < YPL; ++jy) {
+        const int y = lane + 64 * jy;
+        if (y < Ny) sm.nbl[l][y + 1] = nbv[h][jy];
+      }
+      if (lane == 0) sm.nbl[l][0] = 0.f;
+      if (lane == 1) sm.nbl[l][Ny + 1] = 0.f;
+    }
   }
 #pragma unroll
   for (int it = 0; it < kColIt; ++it) {
     const int i = tid + it * kIcdBlock;
-    if (it * kIcdBlock < nc && i < nc) {
-      sm.row[i] = rw_[it];
-      sm.ka[i] = ka_[it];
-      sm.kw[i] = kw_[it];
+    if (it * kIcdBlock < ntot && i < ntot) {
+      const bool in = i < nc;
+      col.rb[i] = in ? rw_[it] : J.spare_b;
+      col.ka[i] = in ? ka_[it] : 0.f;
+      col.kw[i] = in ? kw_[it] : 0.f;
     }
   }
-  // ---- 3. the first theta batch's error-sinogram loads, in flight across the
-  // barrier below ----
-  float ev0[PF ? U : 1][YPL][NV], wf0[PF && FW ? U : 1][YPL][NV];
+  // ---- 3. (PF) the first batch's error-sinogram loads, in flight across the barrier ----
+  // four entries of this lane's stream: row offsets and gains by 16-byte LDS reads
+  auto rows4 = [&](int i) { return *reinterpret_cast<const uint4*>(col.rb + s0 + i); };
+  auto ka4 = [&](int i) { return *reinterpret_cast<const float4*>(col.ka + s0 + i); };
+  auto kw4 = [&](int i) { return *reinterpret_cast<const float4*>(col.kw + s0 + i); };
+  auto load_entry = [&](uint32_t rb, float (&ev)[YPL][NV], float (&wf)[YPL][NV]) {
+#pragma unroll
+    for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        ev[jy][k] = ldo(J.e, rb + vtb[jy][k]);
+        if constexpr (FW) wf[jy][k] = ldo(J.wfull, rb + vtb[jy][k]);
+      }
+  };
+  // the batch [base, base + U) of this lane's stream, rows from LDS
+  auto load_batch = [&](int base, float (&ev)[U][YPL][NV], float (&wf)[U][YPL][NV]) {
+#pragma unroll
+    for (int q = 0; q < U / 4; ++q) {
+      if (base + 4 * q < cnt) {
+        const uint4 r = rows4(base + 4 * q);
+        load_entry(r.x, ev[4 * q], wf[4 * q]);
+        load_entry(r.y, ev[4 * q + 1], wf[4 * q + 1]);
+        load_entry(r.z, ev[4 * q + 2], wf[4 * q + 2]);
+        load_entry(r.w, ev[4 * q + 3], wf[4 * q + 3]);
+      }
+    }
+  };
+  float ev0[PF ? U : 1][YPL][NV], wf0[PF ? U : 1][YPL][NV];
   if constexpr (PF) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const bool ok = p + u * P < nc;
-#pragma unroll
-      for (int jy = 0; jy < YPL; ++jy)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-          const bool use = ok && act[jy] && k < vn[jy];
-          const int idx = use ? r0[u] + vt0[jy] + k : 0;
-          ev0[u][jy][k] = ld<COH>(J.e + idx);
-          if constexpr (FW) wf0[u][jy][k] = ld<COH>(J.wfull + idx);
-        }
-    }
+    for (int u = 0; u < U; ++u)
+      if (((u & ~3)) < cnt) load_entry(r0[u], ev0[u], wf0[u]);
   }
   __syncthreads();  // A: neighbour lines and column visible
   if (tid < Ny) {  // read after barrier D
@@ -462,64 +522,54 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   Kahan t1[YPL], t2[YPL];
   auto accumulate = [&](int base, const auto& ev, const auto& wf) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = base + u * P;
-      if (i < nc) {
-        const float kav = sm.ka[i];
-        const float kwv = sm.kw[i];
+    for (int q = 0; q < U / 4; ++q) {
+      if (base + 4 * q < cnt) {
+        const float4 a4 = ka4(base + 4 * q);
+        float4 w4 = a4;
+        if constexpr (!FW) w4 = kw4(base + 4 * q);
+        const float kav[4] = {a4.x, a4.y, a4.z, a4.w};
+        const float kwv[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-        for (int jy = 0; jy < YPL; ++jy)
+        for (int t = 0; t < 4; ++t) {
+          const int u = 4 * q + t;
 #pragma unroll
-          for (int k = 0; k < NV; ++k)
-            if (act[jy] && k < vn[jy]) {
-              const float pe = kav * vv[jy][k];
-              // per-view weight folded in: pe w = kw vlr.  The bright-field path
-              // carries w per entry, its sign being the Bragg selector: a rejected
-              // entry (w < 0) weighs delta T / (|e| sqrt(|w|)) more
-              // (BrightField/BFForwardModel.cpp:851-892)
-              float pw;
-              if constexpr (FW) {
-                const float w = wf[u][jy][k];
-                if (w >= 0.f) {
-                  pw = pe * w;
-                } else {
-                  const float aw = -w;
-                  pw = pe * aw * (J.bragg_dT / (fabsf(ev[u][jy][k]) * sqrtf(aw)));
-                }
-              } else {
-                pw = kwv * vv[jy][k];
-              }
-              t2[jy].add(pe * pw);
-              t1[jy].add(ev[u][jy][k] * pw);
+          for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+            const float pe = kav[t] * vv[jy][k];
+            // per-view weight folded in: pe w = kw vlr.  The bright-field path
+            // carries w per entry, its sign being the Bragg selector: a rejected
+            // entry (w < 0) weighs delta T / (|e| sqrt(|w|)) more
+            // (BrightField/BFForwardModel.cpp:851-892); padding and lanes without
+            // an entry have pe = 0 and add nothing
+            float pw;
+            if constexpr (FW) {
+              const float w = wf[u][jy][k];
+              const float aw = fabsf(w);
+              pw = (w >= 0.f || pe == 0.f)
+                       ? pe * aw
+                       : pe * aw * (J.bragg_dT / (fabsf(ev[u][jy][k]) * sqrtf(aw)));
+            } else {
+              pw = kwv[t] * vv[jy][k];
             }
+            t2[jy].add(pe * pw);
+            t1[jy].add(ev[u][jy][k] * pw);
+          }
+        }
       }
     }
   };
-  int base0 = p;
+  int base0 = 0;
   if constexpr (PF) {
-    accumulate(p, ev0, wf0);
-    base0 = p + P * U;
+    accumulate(0, ev0, wf0);
+    base0 = U;
   }
-  for (int base = base0; base < nc; base += P * U) {
-    float ev[U][YPL][NV], wf[FW ? U : 1][YPL][NV];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = base + u * P;
-      const bool ok = i < nc;
-      const int rw = sm.row[ok ? i : 0];
-#pragma unroll
-      for (int jy = 0; jy < YPL; ++jy)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-          const bool use = ok && act[jy] && k < vn[jy];
-          const int idx = use ? rw + vt0[jy] + k : 0;
-          ev[u][jy][k] = ld<COH>(J.e + idx);
-          if constexpr (FW) wf[u][jy][k] = ld<COH>(J.wfull + idx);
-        }
-    }
+  for (int base = base0; base < cnt; base += U) {
+    float ev[U][YPL][NV], wf[U][YPL][NV];
+    load_batch(base, ev, wf);
     accumulate(base, ev, wf);
   }
-  // FP64 partials, summed first over the rows of a wave that hold the same slice
+  // FP64 partials, summed first over the streams of a wave that hold the same slice
   // (lanes yl, yl + TY, ...), then over the waves
   double d1[YPL], d2[YPL];
 #pragma unroll
@@ -601,7 +651,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
         nv = nv < dlo ? dlo : (nv > dhi ? dhi : nv);
         if (nv < 0.0) nv = 0.0;  // POSITIVITY_CONSTRAINT (:394-400)
         xnew = (float)nv;
-        st<COH>(J.x + lbase + y, xnew);
+        J.x[lbase + y] = xnew;
         vst = 1;
         if (roi) {
           su = fabs((double)xnew - (double)v);
@@ -617,109 +667,105 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
     __syncthreads();  // F, G
   }
 
-  // ---- 6. e -= I0 a vlr dx (:423-449) ----
+  // ---- 6. e -= I0 a vlr dx (:423-449): product first, then one add (no fma), in
+  // every form ----
   float d[YPL];
+  bool upd[YPL];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
     const int y = yl + TY * jy;
     d[jy] = y < Ny ? sm.dx[y] : 0.f;
+    upd[jy] = y < Ny && sm.vst[y] == 1;
   }
-  if constexpr (FW) {
-    // bright field: every updated slice (even with dx = 0) refreshes the selector of
-    // the entries it touches, |e sqrt(w)| < T (BFForwardModel::updateErrorSinogram,
-    // :898-932); conflict-free steps only, plain read-modify-write
-    bool upd[YPL];
-#pragma unroll
-    for (int jy = 0; jy < YPL; ++jy) {
-      const int y = yl + TY * jy;
-      upd[jy] = y < Ny && sm.vst[y] == 1;
-    }
-    for (int base = p; base < nc; base += P * U) {
+  auto enew = [&](float e, float ka, int jy, int k) {
+    return __fadd_rn(e, -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy]));
+  };
+  // padding entries (zero gains, spare row) are updated like the others: they leave
+  // the spare row at zero
+  for (int base = 0; base < cnt; base += U) {
+    if constexpr (FW) {
+      // bright field: every updated slice (even with dx = 0) refreshes the selector
+      // of the entries it touches, |e sqrt(w)| < T
+      // (BFForwardModel::updateErrorSinogram, :898-932); plain read-modify-write
       float ev[U][YPL][NV], wv[U][YPL][NV];
+      load_batch(base, ev, wv);
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = base + u * P;
-        const bool ok = i < nc;
-        const int rw = sm.row[ok ? i : 0];
+      for (int q = 0; q < U / 4; ++q) {
+        if (base + 4 * q < cnt) {
+          const uint4 r4 = rows4(base + 4 * q);
+          const float4 a4 = ka4(base + 4 * q);
+          const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
+          const float aa[4] = {a4.x, a4.y, a4.z, a4.w};
 #pragma unroll
-        for (int jy = 0; jy < YPL; ++jy)
+          for (int t = 0; t < 4; ++t)
 #pragma unroll
-          for (int k = 0; k < NV; ++k) {
-            const bool use = ok && upd[jy] && k < vn[jy];
-            const int idx = use ? rw + vt0[jy] + k : 0;
-            ev[u][jy][k] = ld<COH>(J.e + idx);
-            wv[u][jy][k] = ld<COH>(J.wfull + idx);
-          }
-      }
+            for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = base + u * P;
-        if (i < nc) {
-          const float ka = sm.ka[i];
-          const int rwi = sm.row[i];
-#pragma unroll
-          for (int jy = 0; jy < YPL; ++jy)
-#pragma unroll
-            for (int k = 0; k < NV; ++k)
-              if (upd[jy] && k < vn[jy]) {
-                const float en =
-                    __fadd_rn(ev[u][jy][k], -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy]));
-                const float aw = fabsf(wv[u][jy][k]);
-                st<COH>(J.e + rwi + vt0[jy] + k, en);
-                st<COH>(J.wfull + rwi + vt0[jy] + k, fabsf(en * sqrtf(aw)) < J.bragg ? aw : -aw);
-              }
+              for (int k = 0; k < NV; ++k)
+                if (upd[jy] && vok[jy][k]) {
+                  const int u = 4 * q + t;
+                  const float en = enew(ev[u][jy][k], aa[t], jy, k);
+                  const float aw = fabsf(wv[u][jy][k]);
+                  *pto(J.e, rr[t] + vtb[jy][k]) = en;
+                  *pto(J.wfull, rr[t] + vtb[jy][k]) = fabsf(en * sqrtf(aw)) < J.bragg ? aw : -aw;
+                }
         }
       }
-    }
-  } else if (!COH && J.e_atomic) {
-    for (int base = p; base < nc; base += P * U) {
+    } else if (J.e_atomic) {
+      // one job alone: fp32 atomics, no load round trip (no two lines of a
+      // conflict-free step share a detector bin, so both forms give the same e)
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = base + u * P;
-        if (i < nc) {
-          const float ka = sm.ka[i];
-          const int rwi = sm.row[i];
+      for (int q = 0; q < U / 4; ++q) {
+        if (base + 4 * q < cnt) {
+          const uint4 r4 = rows4(base + 4 * q);
+          const float4 a4 = ka4(base + 4 * q);
+          const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
+          const float aa[4] = {a4.x, a4.y, a4.z, a4.w};
 #pragma unroll
-          for (int jy = 0; jy < YPL; ++jy)
+          for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int k = 0; k < NV; ++k)
-              if (d[jy] != 0.f && k < vn[jy])
-                unsafeAtomicAdd(J.e + rwi + vt0[jy] + k,
-                                -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy]));
+            for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+              for (int k = 0; k < NV; ++k)
+                if (d[jy] != 0.f && vok[jy][k])
+                  unsafeAtomicAdd(pto(J.e, rr[t] + vtb[jy][k]),
+                                  -__fmul_rn(__fmul_rn(aa[t], vv[jy][k]), d[jy]));
         }
       }
-    }
-  } else {
-    // every load of a batch in flight before its stores
-    for (int base = p; base < nc; base += P * U) {
+    } else {
+      // full batches: read-modify-write, every load of a batch in flight before its
+      // stores (lanes whose slices did not change load and store nothing)
       float ev[U][YPL][NV];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = base + u * P;
-        const bool ok = i < nc;
-        const int rw = sm.row[ok ? i : 0];
+      for (int q = 0; q < U / 4; ++q) {
+        if (base + 4 * q < cnt) {
+          const uint4 r4 = rows4(base + 4 * q);
+          const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
 #pragma unroll
-        for (int jy = 0; jy < YPL; ++jy)
+          for (int t = 0; t < 4; ++t)
 #pragma unroll
-          for (int k = 0; k < NV; ++k) {
-            const bool use = ok && d[jy] != 0.f && k < vn[jy];
-            ev[u][jy][k] = ld<COH>(J.e + (use ? rw + vt0[jy] + k : 0));
-          }
+            for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+              for (int k = 0; k < NV; ++k)
+                if (d[jy] != 0.f && vok[jy][k])
+                  ev[4 * q + t][jy][k] = ldo(J.e, rr[t] + vtb[jy][k]);
+        }
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = base + u * P;
-        if (i < nc) {
-          const float ka = sm.ka[i];
-          const int rwi = sm.row[i];
+      for (int q = 0; q < U / 4; ++q) {
+        if (base + 4 * q < cnt) {
+          const uint4 r4 = rows4(base + 4 * q);
+          const float4 a4 = ka4(base + 4 * q);
+          const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
+          const float aa[4] = {a4.x, a4.y, a4.z, a4.w};
 #pragma unroll
-          for (int jy = 0; jy < YPL; ++jy)
+          for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int k = 0; k < NV; ++k)
-              if (d[jy] != 0.f && k < vn[jy])
-                // the atomic form's roundings: product first, then one add (no fma)
-                st<COH>(J.e + rwi + vt0[jy] + k,
-                        __fadd_rn(ev[u][jy][k], -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy])));
+            for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+              for (int k = 0; k < NV; ++k)
+                if (d[jy] != 0.f && vok[jy][k])
+                  *pto(J.e, rr[t] + vtb[jy][k]) = enew(ev[4 * q + t][jy][k], aa[t], jy, k);
         }
       }
     }
@@ -763,7 +809,7 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef*
   __shared__ IcdSmem<YPL> sm;
   if ((int)blockIdx.x >= nlines) return;
   const LineRef ref = lines[blockIdx.x];
-  icd_line<YPL, NV, FW, true>(J, ref, first, zs_on, acc, sm);
+  icd_line<YPL, NV, FW, true>(J, ref, first, zs_on, acc, sm, icd_col(J.col_slots));
 }
 
 // One conflict-free step of every job of a batch that has kernel shape
@@ -771,9 +817,9 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef*
 // entry e with wg_off[e] <= g < wg_off[e] + nlines[e].  Jobs never share memory,
 // so one launch needs no ordering between them.
 template <int YPL, int NV, bool PF>
-__global__ __launch_bounds__(kIcdBlock) void k_icd_batch_step(const DevJob* __restrict__ jobs,
-                                                              const BatchEntry* __restrict__ ent,
-                                                              int e0, int ne)
+__global__ __launch_bounds__(kIcdBlock) __attribute__((amdgpu_waves_per_eu(icd_waves_per_eu<YPL, NV>(PF))))
+void k_icd_batch_step(const DevJob* __restrict__ jobs,
+                      const BatchEntry* __restrict__ ent, int e0, int ne, int slots)
 {
   __shared__ IcdSmem<YPL> sm;
   const int g = blockIdx.x;
@@ -789,15 +835,16 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_batch_step(const DevJob* __re
   if (i < 0 || i >= E.nlines) return;
   const DevJob J = jobs[__builtin_amdgcn_readfirstlane(E.job)];
   const LineRef ref = E.lines[i];
-  icd_line<YPL, NV, false, PF>(J, ref, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm);
+  icd_line<YPL, NV, false, PF>(J, ref, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm,
+                               icd_col(slots));
 }
 
 // One launch of the request server: entries travel by value in the kernel
 // arguments (no host-to-device copy per launch), each naming a job slot and a
 // contiguous run of its current sweep's schedule.
 template <int YPL, int NV, bool PF>
-__global__ __launch_bounds__(kIcdBlock) void k_icd_tick(const DevJob* __restrict__ jobs,
-                                                        const TickArgs args)
+__global__ __launch_bounds__(kIcdBlock) __attribute__((amdgpu_waves_per_eu(icd_waves_per_eu<YPL, NV>(PF))))
+void k_icd_tick(const DevJob* __restrict__ jobs, const TickArgs args)
 {
   __shared__ IcdSmem<YPL> sm;
   const int g = blockIdx.x;
@@ -818,78 +865,8 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_tick(const DevJob* __restrict
   if (i < 0 || i >= nl) return;
   const DevJob J = jobs[job];
   const LineRef ref = J.sched[l0 + i];
-  icd_line<YPL, NV, false, PF>(J, ref, flags & 1, (flags >> 1) & 1, J.acc, sm);
-}
-
-// One ICD sweep of one job in a single launch.  Workgroups claim the sweep's lines in
-// schedule order from a device counter; a line of step k starts once every line of
-// step k - 1 has finished (a per-step completion count), which makes the sweep the
-// same sequence of conflict-free steps as one launch per step.  Claiming in order
-// means a line only ever waits for lines claimed before it by running workgroups, so
-// the waits always end; a wait longer than the deadline sets *fail and every
-// workgroup leaves.  Error sinogram and volume go through device-scope accesses.
-struct SweepSync {
-  unsigned int claim;  // next line
-  unsigned int fail;   // 1: a wait timed out
-};
-
-template <int YPL, int NV, bool FW>
-__global__ __launch_bounds__(kIcdBlock) void k_icd_sweep(DevJob J, const LineRef* __restrict__ lines,
-                                                         const int* __restrict__ step_ptr,
-                                                         int nsteps, unsigned int* done,
-                                                         SweepSync* sync, int first, int zs_on,
-                                                         unsigned long long deadline_ticks)
-{
-  __shared__ IcdSmem<YPL> sm;
-  __shared__ int s_q, s_k;
-  const int nl = step_ptr[nsteps];
-  for (;;) {
-    if (threadIdx.x == 0) {
-      int q = (int)__hip_atomic_fetch_add(&sync->claim, 1u, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-      int k = 0;
-      if (q < nl) {
-        int lo = 0, hi = nsteps - 1;  // the step that holds line q
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (step_ptr[mid] <= q) lo = mid;
-          else hi = mid - 1;
-        }
-        k = lo;
-      }
-      if (q < nl && k > 0) {
-        const unsigned int need = (unsigned int)(step_ptr[k] - step_ptr[k - 1]);
-        // poll sparingly: every poll is a device-scope load of one shared line
-        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-        unsigned int polls = 0;
-        while (__hip_atomic_load(done + (k - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-               need) {
-          // two bounds: the real-time clock and the poll count (each poll sleeps ~512
-          // clocks, so 2^23 polls are seconds) in case the clock does not advance
-          if ((++polls & 15u) == 0u &&
-              (__hip_atomic_load(&sync->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-               __builtin_amdgcn_s_memrealtime() - t0 > deadline_ticks || polls > (1u << 23))) {
-            __hip_atomic_store(&sync->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            q = nl;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(8);
-        }
-      }
-      s_q = q;
-      s_k = k;
-    }
-    __syncthreads();
-    const int q = s_q, k = s_k;
-    if (q >= nl) break;
-    const LineRef ref = lines[q];
-    icd_line<YPL, NV, FW, true, true>(J, ref, first, zs_on, J.acc, sm);
-    // every store of this line has completed (device scope) before the count moves
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0)
-      __hip_atomic_fetch_add(done + k, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  icd_line<YPL, NV, false, PF>(J, ref, flags & 1, (flags >> 1) & 1, J.acc, sm,
+                               icd_col(args.slots));
 }
 
 // jobs[slot] = J: a job descriptor written in stream order.
@@ -1262,17 +1239,18 @@ template <int YPL>
 static void launch_step_nv(const DevJob& J, const LineRef* lines, int nlines, int first, int zs,
                            SweepAccum* acc, hipStream_t s)
 {
+  const size_t dyn = icd_col_bytes(J.col_slots);
   if (J.wfull)  // bright field: per-entry weights and selector (one row per slice)
-    hipLaunchKernelGGL((k_icd_step<YPL, 1, true>), dim3(nlines), dim3(kIcdBlock), 0, s, J, lines,
+    hipLaunchKernelGGL((k_icd_step<YPL, 1, true>), dim3(nlines), dim3(kIcdBlock), dyn, s, J, lines,
                        nlines, first, zs, acc);
   else if (J.vlr_max <= 1)
-    hipLaunchKernelGGL((k_icd_step<YPL, 1>), dim3(nlines), dim3(kIcdBlock), 0, s, J, lines, nlines,
+    hipLaunchKernelGGL((k_icd_step<YPL, 1>), dim3(nlines), dim3(kIcdBlock), dyn, s, J, lines, nlines,
                        first, zs, acc);
   else if (J.vlr_max == 2)
-    hipLaunchKernelGGL((k_icd_step<YPL, 2>), dim3(nlines), dim3(kIcdBlock), 0, s, J, lines, nlines,
+    hipLaunchKernelGGL((k_icd_step<YPL, 2>), dim3(nlines), dim3(kIcdBlock), dyn, s, J, lines, nlines,
                        first, zs, acc);
   else
-    hipLaunchKernelGGL((k_icd_step<YPL, 4>), dim3(nlines), dim3(kIcdBlock), 0, s, J, lines, nlines,
+    hipLaunchKernelGGL((k_icd_step<YPL, 4>), dim3(nlines), dim3(kIcdBlock), dyn, s, J, lines, nlines,
                        first, zs, acc);
 }
 
@@ -1297,17 +1275,18 @@ hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, in
 constexpr int kFullDeviceWg = 512;
 
 hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const BatchEntry* ent,
-                                 int e0, int ne, int nwg, hipStream_t s)
+                                 int e0, int ne, int nwg, int slots, hipStream_t s)
 {
   if (ne <= 0 || nwg <= 0) return hipSuccess;
+  const size_t dyn = icd_col_bytes(slots);
   const bool pf = nwg <= kFullDeviceWg;
 #define MBIR_BATCH(Y, N)                                                                      \
   if (pf)                                                                                     \
-    hipLaunchKernelGGL((k_icd_batch_step<Y, N, true>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, \
-                       ent, e0, ne);                                                          \
+    hipLaunchKernelGGL((k_icd_batch_step<Y, N, true>), dim3(nwg), dim3(kIcdBlock), dyn, s,    \
+                       jobs, ent, e0, ne, slots);                                             \
   else                                                                                        \
-    hipLaunchKernelGGL((k_icd_batch_step<Y, N, false>), dim3(nwg), dim3(kIcdBlock), 0, s,     \
-                       jobs, ent, e0, ne);
+    hipLaunchKernelGGL((k_icd_batch_step<Y, N, false>), dim3(nwg), dim3(kIcdBlock), dyn, s,   \
+                       jobs, ent, e0, ne, slots);
   if (ypl == 1 && nv == 1) {
     MBIR_BATCH(1, 1)
   } else if (ypl == 1) {
@@ -1325,12 +1304,15 @@ hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& 
                            hipStream_t s)
 {
   if (args.n <= 0 || args.n > kTickMax || nwg <= 0) return hipSuccess;
+  const size_t dyn = icd_col_bytes(args.slots);
   const bool pf = nwg <= kFullDeviceWg;
 #define MBIR_TICK(Y, N)                                                                        \
   if (pf)                                                                                      \
-    hipLaunchKernelGGL((k_icd_tick<Y, N, true>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args); \
+    hipLaunchKernelGGL((k_icd_tick<Y, N, true>), dim3(nwg), dim3(kIcdBlock), dyn, s, jobs,      \
+                       args);                                                                  \
   else                                                                                         \
-    hipLaunchKernelGGL((k_icd_tick<Y, N, false>), dim3(nwg), dim3(kIcdBlock), 0, s, jobs, args);
+    hipLaunchKernelGGL((k_icd_tick<Y, N, false>), dim3(nwg), dim3(kIcdBlock), dyn, s, jobs,     \
+                       args);
   if (ypl == 1 && nv == 1) {
     MBIR_TICK(1, 1)
   } else if (ypl == 1) {
@@ -1342,54 +1324,6 @@ hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& 
   }
 #undef MBIR_TICK
   return hipGetLastError();
-}
-
-hipError_t launch_icd_sweep(const DevJob& J, const LineRef* lines, const int* step_ptr,
-                            int nsteps, int nlines, unsigned int* done, void* sync, int first,
-                            int zs, int grid, hipStream_t s)
-{
-  if (nlines <= 0 || nsteps <= 0) return hipSuccess;
-  int ypl, nv;
-  icd_shape(J, J.vlr_max, &ypl, &nv);
-  const unsigned long long deadline = 400000000ull;  // 4 s of the 100 MHz real-time clock
-  SweepSync* sy = (SweepSync*)sync;
-  const int g = grid < nlines ? grid : nlines;
-#define MBIR_SWEEP(Y, N, F)                                                                    \
-  hipLaunchKernelGGL((k_icd_sweep<Y, N, F>), dim3(g), dim3(kIcdBlock), 0, s, J, lines, step_ptr, \
-                     nsteps, done, sy, first, zs, deadline)
-  if (J.wfull) {
-    if (nv != 1 || ypl > 2) return hipErrorInvalidValue;
-    if (ypl == 1) MBIR_SWEEP(1, 1, true);
-    else MBIR_SWEEP(2, 1, true);
-  } else if (ypl == 1 && nv == 1) {
-    MBIR_SWEEP(1, 1, false);
-  } else if (ypl == 1 && nv == 2) {
-    MBIR_SWEEP(1, 2, false);
-  } else if (ypl == 2 && nv == 1) {
-    MBIR_SWEEP(2, 1, false);
-  } else if (ypl == 2 && nv == 2) {
-    MBIR_SWEEP(2, 2, false);
-  } else {
-    return hipErrorInvalidValue;
-  }
-#undef MBIR_SWEEP
-  return hipGetLastError();
-}
-
-int icd_sweep_blocks_per_cu(int ypl, int nv, bool fw)
-{
-  int n = 0;
-  hipError_t e = hipErrorInvalidValue;
-  if (fw)
-    e = ypl == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<1, 1, true>, kIcdBlock, 0)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<2, 1, true>, kIcdBlock, 0);
-  else if (ypl == 1)
-    e = nv == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<1, 1, false>, kIcdBlock, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<1, 2, false>, kIcdBlock, 0);
-  else
-    e = nv == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<2, 1, false>, kIcdBlock, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_icd_sweep<2, 2, false>, kIcdBlock, 0);
-  return e == hipSuccess && n > 0 ? n : 1;
 }
 
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, size_t n, hipStream_t s);

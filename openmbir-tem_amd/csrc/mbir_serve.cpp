@@ -334,6 +334,7 @@ void mbir_server::run_loop()
     for (int shape = 0; shape < 4 && !bad; ++shape) {
       const int ypl = 1 + (shape >> 1), nv = 1 + (shape & 1);
       args.n = 0;
+      args.slots = 4;
       int wg = 0;
       std::vector<Req*> last;  // jobs whose sweep ends with this launch
       auto flush = [&]() -> bool {
@@ -352,6 +353,7 @@ void mbir_server::run_loop()
         }
         last.clear();
         args.n = 0;
+        args.slots = 4;
         wg = 0;
         return true;
       };
@@ -360,6 +362,7 @@ void mbir_server::run_loop()
           continue;
         mbir_job* j = r->job;
         TickEntry& E = args.e[args.n++];
+        args.slots = std::max(args.slots, j->J.col_slots);
         E.job = r->slot;
         E.line0 = j->step_ptr[r->step];
         E.nlines = j->step_ptr[r->step + 1] - E.line0;
