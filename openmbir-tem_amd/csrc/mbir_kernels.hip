@@ -160,6 +160,35 @@ __device__ inline void group_reduce(int G, float& s1, float& s2, float& lo, floa
   if (G >= 16) dpp_step<0x140>(s1, s2, lo, hi); // row_mirror
 }
 
+// Sum over the 64 lanes of a wave in a fixed order by DPP moves (no LDS): prefix
+// sums within rows of 16 (row_shr 1, 2, 4, 8), then row_bcast 15 and 31; the total
+// ends in lane 63.  Every lane of the wave must take part.
+template <int CTRL, int ROWS>
+__device__ inline int dpp_upd(int v)
+{
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+__device__ inline float wave_sum(float v)
+{
+  v += __int_as_float(dpp_upd<0x111, 0xF>(__float_as_int(v)));
+  v += __int_as_float(dpp_upd<0x112, 0xF>(__float_as_int(v)));
+  v += __int_as_float(dpp_upd<0x114, 0xF>(__float_as_int(v)));
+  v += __int_as_float(dpp_upd<0x118, 0xF>(__float_as_int(v)));
+  v += __int_as_float(dpp_upd<0x142, 0xA>(__float_as_int(v)));
+  v += __int_as_float(dpp_upd<0x143, 0xC>(__float_as_int(v)));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ inline int wave_sum(int v)
+{
+  v += dpp_upd<0x111, 0xF>(v);
+  v += dpp_upd<0x112, 0xF>(v);
+  v += dpp_upd<0x114, 0xF>(v);
+  v += dpp_upd<0x118, 0xF>(v);
+  v += dpp_upd<0x142, 0xA>(v);
+  v += dpp_upd<0x143, 0xC>(v);
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -307,11 +336,12 @@ __global__ void k_column_weights(DevJob J, float* ka, float* kw, int nnz)
 //      across waves);
 //   5. the voxel solve of each parity runs on lane groups: the 26 neighbour
 //      terms of one voxel are spread over G lanes and combined by DPP moves;
-//   6. e -= I0 a vlr dx: no two lines of a conflict-free step share a detector
-//      bin, so the first theta batch's entries are stored from the values read
-//      for theta; further entries by fp32 atomics (one job alone) or a plain
-//      read-modify-write (full batches), all rounding alike;
-//   7. per-slice statistics reduced by one wave, one set of global atomics.
+//   6. sweep statistics: per-wave shares formed in the solve, summed by one thread
+//      in a fixed order, one set of global atomics per line;
+//   7. e -= I0 a vlr dx: fp32 atomics (one job alone) or a plain read-modify-write
+//      (full batches); no two lines of a conflict-free step share a detector bin,
+//      so both give the same e, and both round alike;
+
 // Byte-offset access through a uniform base pointer: base in scalar registers plus
 // a 32-bit offset in one vector register (global load/store with a scalar base).
 __device__ inline float ldo(const float* base, uint32_t boff)
@@ -334,7 +364,11 @@ struct IcdSmem {
   double T1[NY], T2[NY];
   // theta partial sums per wave and slice (FP64)
   double pd1[kIcdWaves][NY], pd2[kIcdWaves][NY];
-  double su[NY], sv[NY];      // per slice: ROI |x_new - x_old|, |x_old|
+  // per wave: ROI sums of |x_new - x_old| and |x_old| (fp32, at most 256 non-negative
+  // terms per line), and the counts zero-skipped + 2^10 updated + 2^20 detector rows
+  // of the updated slices
+  float wsu[kIcdWaves], wsv[kIcdWaves];
+  int wcnt[kIcdWaves];
   unsigned char vst[NY];      // per slice: 1 updated, 2 zero-skipped
   unsigned char vns[NY];      // per slice: detector rows (VLR size)
 };
@@ -603,6 +637,8 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   __syncthreads();  // E
 
   // ---- 5. voxel solve on lane groups, even slices then odd slices ----
+  float wsu = 0.f, wsv = 0.f;  // this wave's statistics (uniform)
+  int wcnt = 0;
   for (int phase = 0; phase < 2; ++phase) {
     const int vpp = (Ny + 1 - phase) / 2;  // slices of this parity
     int G = 16;                            // lanes per voxel (one DPP row at most)
@@ -632,10 +668,11 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
     }
     group_reduce(G, s1, s2, lo, hi);
     const bool anypos = hi > 0.f;
+    float su = 0.f, sv = 0.f;  // this lane's voxel (group leaders only)
+    int vst = 0, nrow = 0;
     if (active && gl == 0) {
       float xnew = v;
-      double su = 0.0, sv = 0.0;
-      unsigned char vst = 2;
+      vst = 2;
       if (zs_active && v == 0.f && !anypos) {
         // zero-skipped (:311-338)
       } else {
@@ -653,21 +690,51 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
         xnew = (float)nv;
         J.x[lbase + y] = xnew;
         vst = 1;
+        nrow = sm.vns[y];
         if (roi) {
-          su = fabs((double)xnew - (double)v);
-          sv = fabs((double)v);
+          su = fabsf(xnew - v);
+          sv = fabsf(v);
         }
       }
-      sm.su[y] = su;
-      sm.sv[y] = sv;
-      sm.vst[y] = vst;
+      sm.vst[y] = (unsigned char)vst;
       sm.x[y] = xnew;
       sm.dx[y] = xnew - v;
+    }
+    // this wave's share of the sweep statistics (:412-421): the group leaders' values
+    // summed in a fixed order, counts by ballots
+    wsu += wave_sum(su);
+    wsv += wave_sum(sv);
+    const int nr = NV == 1 ? (int)__popcll(__ballot(vst == 1)) : wave_sum(nrow);
+    wcnt += (int)__popcll(__ballot(vst == 2)) + ((int)__popcll(__ballot(vst == 1)) << 10) +
+            (nr << 20);
+    if (phase == 1 && lane == 0) {
+      sm.wsu[wave] = wsu;
+      sm.wsv[wave] = wsv;
+      sm.wcnt[wave] = wcnt;
     }
     __syncthreads();  // F, G
   }
 
-  // ---- 6. e -= I0 a vlr dx (:423-449): product first, then one add (no fma), in
+  // ---- 6. sweep statistics: the per-wave shares (visible since the last solve
+  // barrier) summed in a fixed order, one set of global atomics per line; issued
+  // before the error-sinogram update where the line's latency counts (PF), after it
+  // where the device is full ----
+  auto line_stats = [&]() {
+    if (wave == 0) {
+      const float su = wave_sum(lane < kIcdWaves ? sm.wsu[lane] : 0.f);
+      const float sv = wave_sum(lane < kIcdWaves ? sm.wsv[lane] : 0.f);
+      const int cn = wave_sum(lane < kIcdWaves ? sm.wcnt[lane] : 0);
+      const int nz = cn & 1023, nu = (cn >> 10) & 1023, nn = nc * (cn >> 20);
+      if (lane == 0 && su != 0.f) atomicAdd(&acc->sum_abs_update, (double)su);
+      if (lane == 0 && sv != 0.f) atomicAdd(&acc->sum_abs_value, (double)sv);
+      if (lane == 0 && nz) atomicAdd(&acc->zero_skipped, (unsigned long long)nz);
+      if (lane == 0 && nu) atomicAdd(&acc->updated, (unsigned long long)nu);
+      if (lane == 0 && nn) atomicAdd(&acc->nnz_updated, (unsigned long long)nn);
+    }
+  };
+  if constexpr (PF) line_stats();
+
+  // ---- 7. e -= I0 a vlr dx (:423-449): product first, then one add (no fma), in
   // every form ----
   float d[YPL];
   bool upd[YPL];
@@ -771,35 +838,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
     }
   }
 
-  // ---- 7. sweep statistics (:412-421): the first wave reduces the per-slice
-  // values (visible since the last solve barrier), one set of global atomics per line ----
-  if (wave == 0) {
-    double su = 0.0, sv = 0.0;
-    int nz = 0, nu = 0, nn = 0;
-    for (int y = lane; y < Ny; y += 64) {
-      const int f = sm.vst[y];
-      su += sm.su[y];
-      sv += sm.sv[y];
-      nz += f == 2;
-      nu += f == 1;
-      nn += f == 1 ? nc * (int)sm.vns[y] : 0;
-    }
-    for (int m = 32; m > 0; m >>= 1) {
-      su += __shfl_xor(su, m);
-      sv += __shfl_xor(sv, m);
-      nz += __shfl_xor(nz, m);
-      nu += __shfl_xor(nu, m);
-      nn += __shfl_xor(nn, m);
-    }
-    if (lane == 0) {
-      if (su != 0.0) atomicAdd(&acc->sum_abs_update, su);
-      if (sv != 0.0) atomicAdd(&acc->sum_abs_value, sv);
-      if (nz) atomicAdd(&acc->zero_skipped, (unsigned long long)nz);
-      if (nu) atomicAdd(&acc->updated, (unsigned long long)nu);
-      if (nn) atomicAdd(&acc->nnz_updated, (unsigned long long)nn);
-    }
-  }
-  __syncthreads();  // H: LDS may be reused by the next line
+  if constexpr (!PF) line_stats();
 }
 
 template <int YPL, int NV, bool FW = false>

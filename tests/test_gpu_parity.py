@@ -95,6 +95,22 @@ def test_sweep_cost_monotone_and_error_consistency(gpu_ctx):
     assert np.max(np.abs(job.error_sinogram() - e_ref)) <= 2e-4 * max(1.0, np.abs(y).max())
 
 
+def test_first_sweep_statistics_match_oracle(gpu_ctx):
+    """The stop-test sums of the first sweep do not depend on the update order: every
+    visited voxel's old value is the initial one, and nothing is zero-skipped.  The
+    GPU's per-line reductions (fp32 within a line, FP64 across lines) must give the
+    oracle's FP64 sum of |x_old| over the ROI and its visit count."""
+    g, ang, vol, y = small_problem()
+    job = _job(gpu_ctx, g, ang, y, default_recon=0.3)
+    eng = O.Engine(g, ang, y, O.params(), default_recon=0.3)
+    _, st = job.sweep(0, 0)
+    _, ost = eng.sweep(0, 0)
+    assert st.zero_skipped == 0 and ost[2] == 0
+    assert st.updated == int(ost[3])
+    assert st.sum_abs_value == pytest.approx(ost[1], rel=1e-6)
+    assert st.sum_abs_update > 0 and ost[0] > 0
+
+
 def test_converged_volume_matches_oracle(gpu_ctx):
     g, ang, vol, y = small_problem()
     eng = _engine(g, ang, y)
