@@ -33,6 +33,9 @@ SOURCES = [
     (os.path.join(CSRC, "mbir_serve.cpp"), ["-x", "hip"] + DEVICE),
     # host adapter with the reference engine's C++ interface (no device code)
     (os.path.join(ENGINE, "HAADF_ReconstructionEngine.cpp"), ["-x", "c++"]),
+    # the engine's files (MRC tilt series in, MRC volume and binary outputs out)
+    (os.path.join(ENGINE, "MBIRIO.cpp"), ["-x", "c++"]),
+    (os.path.join(ENGINE, "mbir_io_capi.cpp"), ["-x", "c++"]),
 ]
 CLI = os.path.join(HERE, "engine_cli")
 

@@ -5,8 +5,11 @@
 // nuisance initialisation, the reconstruction loop, then the outputs.
 #include "HAADF_ReconstructionEngine.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+
+#include "MBIRIO.h"
 
 namespace mbirtem {
 
@@ -25,6 +28,108 @@ void HAADF_ReconstructionEngine::setCancel(bool v)
   if (v && m_Job) mbir_job_cancel(m_Job);
 }
 
+// The tilt series named by TomoInputs::sinoFile into the sinogram
+// (HAADF/HAADF_ReconstructionEngine_Extra.cpp:43-79, MRC files only: the reference's
+// raw .bin reader reads nothing).
+int HAADF_ReconstructionEngine::readInputData()
+{
+  TomoInputs& T = *m_TomoInputs;
+  Sinogram& S = *m_Sinogram;
+  io::TiltSeriesSelect sel;
+  sel.useSubvolume = T.useSubvolume;
+  sel.xStart = T.xStart; sel.xEnd = T.xEnd;
+  sel.yStart = T.yStart; sel.yEnd = T.yEnd;
+  sel.zStart = T.zStart; sel.zEnd = T.zEnd;
+  sel.interpolateFactor = T.interpolateFactor;
+  sel.excludedViews.assign(T.excludedViews.begin(), T.excludedViews.end());
+  sel.tilts = T.tilts;
+  io::TiltSeries ts;
+  const int rc = io::readMRCTiltSeries(T.sinoFile, sel, &ts);
+  if (rc < 0) return rc;
+  if (ts.N_theta > 65535 || ts.N_r > 65535 || ts.N_t > 65535) return -1;
+  S.N_theta = (uint16_t)ts.N_theta;
+  S.N_r = (uint16_t)ts.N_r;
+  S.N_t = (uint16_t)ts.N_t;
+  if (ts.delta_r > 0) S.delta_r = ts.delta_r;
+  if (ts.delta_t > 0) S.delta_t = ts.delta_t;
+  if (!(S.delta_r > 0)) S.delta_r = 1;
+  if (!(S.delta_t > 0)) S.delta_t = 1;
+  S.counts = std::move(ts.counts);
+  S.angles = std::move(ts.angles);
+  T.xStart = (uint16_t)ts.xStart; T.xEnd = (uint16_t)ts.xEnd;
+  T.yStart = (uint16_t)ts.yStart; T.yEnd = (uint16_t)ts.yEnd;
+  T.fileXSize = (uint16_t)ts.fileX; T.fileYSize = (uint16_t)ts.fileY; T.fileZSize = (uint16_t)ts.fileZ;
+  T.goodViews = ts.goodViews;
+  return 0;
+}
+
+// Initial gains, offsets and variances from the files named in TomoInputs, one double
+// per used view (IOFilters/GainsOffsetsReader.cpp:64-230): a file that cannot be
+// opened leaves the defaults, a short one is an error.
+int HAADF_ReconstructionEngine::readNuisanceInputs()
+{
+  TomoInputs& T = *m_TomoInputs;
+  HAADF_ForwardModel& F = *m_ForwardModel;
+  const size_t n = m_Sinogram->N_theta;
+  struct In {
+    const std::string* path;
+    std::vector<Real_t>* dst;
+  } ins[3] = {{&T.gainsInputFile, &F.InitialGain}, {&T.offsetsInputFile, &F.InitialOffset},
+              {&T.varianceInputFile, &F.InitialVariance}};
+  for (const In& in : ins) {
+    if (in.path->empty()) continue;
+    std::vector<Real_t> v(n);
+    const int rc = io::readDoubles(*in.path, v.data(), n);
+    if (rc == -1) continue;
+    if (rc < 0) return -1;
+    *in.dst = std::move(v);
+    if (in.dst == &F.InitialOffset) T.useDefaultOffset = false;
+  }
+  return 0;
+}
+
+// The files HAADF_ReconstructionEngine::execute() writes after the reconstruction
+// (HAADF/HAADF_ReconstructionEngine.cpp:836-902): nuisance parameters, the
+// reconstructed sinogram and raw volume in tempDir, the MRC volume over the x range
+// that lies over the detector.  Only files whose names are set are written.
+void HAADF_ReconstructionEngine::writeOutputs(mbir_job* job, const std::vector<float>& vol)
+{
+  const TomoInputs& T = *m_TomoInputs;
+  const Sinogram& S = *m_Sinogram;
+  const Geometry& G = *m_Geometry;
+  const HAADF_ForwardModel& F = *m_ForwardModel;
+  const AdvancedParameters& A = *m_AdvParams;
+  auto note = [&](int rc, const char* what) {
+    if (rc < 0) fail(-1, std::string("Error writing ") + what);
+  };
+  if (A.JOINT_ESTIMATION) {
+    if (!T.gainsOutputFile.empty()) note(io::writeDoubles(T.gainsOutputFile, F.I_0.data(), S.N_theta), "gains");
+    if (!T.offsetsOutputFile.empty()) note(io::writeDoubles(T.offsetsOutputFile, F.Mu.data(), S.N_theta), "offsets");
+  }
+  if (A.NOISE_ESTIMATION && !T.varianceOutputFile.empty())
+    note(io::writeDoubles(T.varianceOutputFile, F.Alpha.data(), S.N_theta), "variances");
+  if (!T.tempDir.empty()) {
+    const size_t M = (size_t)S.N_theta * S.N_r * S.N_t;
+    std::vector<float> e(M);
+    if (mbir_job_get_error_sinogram(job, e.data()) < 0) return fail(-100, "error sinogram readback failed");
+    note(io::writeReconstructedSinogram(T.tempDir + "/ReconstructedSinogram.bin", S.N_theta, S.N_r,
+                                        S.N_t, S.counts.data(), e.data(), F.I_0.data(),
+                                        F.Mu.data()),
+         "ReconstructedSinogram.bin");
+    note(io::writeRawVolume(T.tempDir + "/ReconstructedObject.bin", vol.data(), G.N_z, G.N_x, G.N_y),
+         "ReconstructedObject.bin");
+  }
+  if (!T.mrcOutputFile.empty()) {
+    int c0 = 0, c1 = G.N_x;
+    // voxel size in detector units: the multiple times delta_r
+    // (GenericFilters/InitialReconstructionInitializer.cpp:109)
+    io::originalXDims(G.N_x, G.x0, T.delta_xz * S.delta_r, S.N_r, S.delta_r, &c0, &c1);
+    // the reference widens the range by one column here (:894)
+    c1 = std::min<int>(c1 + 1, G.N_x);
+    note(io::writeMRCVolume(T.mrcOutputFile, vol.data(), G.N_z, G.N_x, G.N_y, c0, c1, G.LengthX), "MRC");
+  }
+}
+
 void HAADF_ReconstructionEngine::execute()
 {
   m_ErrorCondition = 0;
@@ -41,6 +146,10 @@ void HAADF_ReconstructionEngine::execute()
 
   Sinogram& S = *m_Sinogram;
   TomoInputs& T = *m_TomoInputs;
+  if (S.counts.empty() && !T.sinoFile.empty()) {
+    const int rd = readInputData();
+    if (rd < 0) return fail(rd, "Error reading Input Sinogram Data file");
+  }
   const size_t M = (size_t)S.N_theta * S.N_r * S.N_t;
   if (S.N_theta == 0 || S.counts.size() != M || S.angles.size() != S.N_theta)
     return fail(-1, "Error reading Input Sinogram Data");
@@ -68,6 +177,17 @@ void HAADF_ReconstructionEngine::execute()
 
   std::vector<float> counts(M);
   for (size_t i = 0; i < M; ++i) counts[i] = (float)S.counts[i];
+  if (!T.initialReconFile.empty()) {
+    // HAADF/HAADF_ReconstructionEngine_Extra.cpp:182-222
+    const std::string& f = T.initialReconFile;
+    const std::string ext = f.size() > 4 ? f.substr(f.size() - 4) : std::string();
+    if (ext == ".mrc") return fail(-1, "We are not dealing with mrc volume files.");
+    if (ext != ".bin") return fail(-1, "Could not find a compatible reader for the initial reconstruction data file.");
+    T.initialRecon.assign(V, 0.0);
+    if (io::readRawVolume(f, T.initialRecon.data(), g.N_z, g.N_x, g.N_y, T.InterpFlag) < 0)
+      return fail(-1, "Error reading Initial Reconstruction Data from File");
+  }
+  if (readNuisanceInputs() < 0) return fail(-1, "Error Reading Gains from File");
   std::vector<float> init;
   if (!T.initialRecon.empty()) {
     if (T.initialRecon.size() != V) return fail(-1, "Error reading Initial Reconstruction Data");
@@ -138,6 +258,7 @@ void HAADF_ReconstructionEngine::execute()
       rc = -100;
     } else {
       G.Object.assign(vol.begin(), vol.end());
+      writeOutputs(job, vol);
     }
   }
   {
@@ -148,7 +269,6 @@ void HAADF_ReconstructionEngine::execute()
   mbir_destroy(ctx);
   if (rc == -999) return fail(-999, "cancelled");
   if (rc < 0) return fail(rc, "reconstruction failed");
-  m_ErrorCondition = 0;
 }
 
 }  // namespace mbirtem

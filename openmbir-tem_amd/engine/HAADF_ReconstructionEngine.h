@@ -11,8 +11,12 @@
 // Error codes follow the reference: 0 ok, -1 bad input, -2 detector response,
 // -999 cancelled (HAADF/HAADF_ReconstructionEngine.cpp:369-393,401,522-528);
 // -100 and below are device failures of this engine.
-// Files are not read or written here: the caller fills the sinogram (counts and
-// tilt angles in degrees) and reads back Geometry::Object and the nuisance arrays.
+// Inputs come either as arrays (the caller fills the sinogram's counts and tilt
+// angles in degrees) or as the reference's files (TomoInputs::sinoFile, an MRC tilt
+// series; initialReconFile; gains/offsets/varianceInputFile); after the run the
+// engine writes the files the reference writes when their names are set
+// (mrcOutputFile, gains/offsets/varianceOutputFile, and ReconstructedSinogram.bin /
+// ReconstructedObject.bin in tempDir), see MBIRIO.h.
 #pragma once
 
 #include <atomic>
@@ -61,12 +65,27 @@ struct TomoInputs {
   Real_t defaultOffset = 0;
   Real_t defaultVariance = 1;
   uint32_t seed = 12345;  // update order (the reference seeds from the clock)
+  // the reference's file fields (Reconstruction/ReconstructionStructures.h:103-160)
+  uint8_t InterpFlag = 0;  // 1: initialReconFile is half size, replicated 2 x 2 x 2
+  bool useSubvolume = false;
+  uint16_t xStart = 0, xEnd = 0, yStart = 0, yEnd = 0, zStart = 0, zEnd = 0;
+  uint16_t fileXSize = 0, fileYSize = 0, fileZSize = 0;
+  std::vector<float> tilts;             // degrees, per used view (empty: from the file)
+  std::string sinoFile;                 // MRC tilt series
+  std::string initialReconFile;         // .bin volume, y x z order
+  std::string gainsInputFile, offsetsInputFile, varianceInputFile;
+  std::string tempDir;                  // ReconstructedSinogram.bin, ReconstructedObject.bin
+  std::string gainsOutputFile, offsetsOutputFile, varianceOutputFile;
+  std::string mrcOutputFile;
+  std::vector<uint8_t> excludedViews;   // file sections left out
+  std::vector<int> goodViews;           // file sections used (set when read from a file)
 };
 
 struct AdvancedParameters {
   unsigned int JOINT_ESTIMATION = 1;
   unsigned int ZERO_SKIPPING = 1;
   unsigned int NOISE_MODEL = 1;
+  unsigned int NOISE_ESTIMATION = 1;  // write the variances (HAADF/HAADF_ForwardModel.cpp:62)
 };
 
 class HAADF_ForwardModel {
@@ -123,6 +142,9 @@ class HAADF_ReconstructionEngine {
 
  private:
   void fail(int code, const std::string& msg);
+  int readInputData();
+  int readNuisanceInputs();
+  void writeOutputs(mbir_job* job, const std::vector<float>& vol);
 
   TomoInputsPtr m_TomoInputs;
   SinogramPtr m_Sinogram;

@@ -1,14 +1,22 @@
-// engine_cli -- run HAADF_ReconstructionEngine::execute() on raw binary inputs.
+// engine_cli -- run HAADF_ReconstructionEngine::execute() on raw binary inputs or on an
+// MRC tilt series.
 //
 //   engine_cli <sino.f64> <angles.f64> N_r N_t N_theta thickness sigma_x p
 //              outer inner joint noise seed <out_prefix>
 // sino.f64 is [N_theta][N_r][N_t] little-endian FP64, angles.f64 the N_theta tilts in
 // degrees.  Writes <out_prefix>.vol.f64 ([N_z][N_x][N_y]), <out_prefix>.nuis.f64 (I_0,
 // Mu, Alpha) and <out_prefix>.txt (dimensions, sweeps, error code, costs).
+//
+//   engine_cli --mrc <tilts.mrc> thickness sigma_x p outer inner joint noise seed <out_dir>
+// reads the tilt series (tilt angles and pixel size from its FEI extended header) and
+// lets the engine write the reference's outputs into out_dir: recon.mrc, gains.bin,
+// offsets.bin, variances.bin, ReconstructedSinogram.bin, ReconstructedObject.bin, and
+// run.txt as above.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iomanip>
+#include <string>
 #include <vector>
 
 #include "HAADF_ReconstructionEngine.h"
@@ -24,8 +32,56 @@ static std::vector<double> read_f64(const char* path, size_t n)
   return v;
 }
 
+static int run_mrc(int argc, char** argv)
+{
+  if (argc != 12) {
+    std::fprintf(stderr, "usage: %s --mrc tilts.mrc thickness sigma_x p outer inner joint noise "
+                         "seed out_dir\n", argv[0]);
+    return 2;
+  }
+  const std::string dir = argv[11];
+  auto sino = std::make_shared<Sinogram>();
+  auto ti = std::make_shared<TomoInputs>();
+  HAADF_ReconstructionEngine::InitializeTomoInputs(ti);
+  ti->sinoFile = argv[2];
+  ti->LengthZ = std::atof(argv[3]);
+  ti->SigmaX = std::atof(argv[4]);
+  ti->p = std::atof(argv[5]);
+  ti->NumOuterIter = (uint16_t)std::atoi(argv[6]);
+  ti->NumIter = (uint16_t)std::atoi(argv[7]);
+  ti->StopThreshold = 1e-4;
+  ti->seed = (uint32_t)std::atoi(argv[10]);
+  ti->tempDir = dir;
+  ti->mrcOutputFile = dir + "/recon.mrc";
+  ti->gainsOutputFile = dir + "/gains.bin";
+  ti->offsetsOutputFile = dir + "/offsets.bin";
+  ti->varianceOutputFile = dir + "/variances.bin";
+  auto adv = std::make_shared<AdvancedParameters>();
+  adv->JOINT_ESTIMATION = (unsigned)std::atoi(argv[8]);
+  adv->NOISE_MODEL = (unsigned)std::atoi(argv[9]);
+  auto geom = std::make_shared<Geometry>();
+  auto fm = HAADF_ForwardModel::New();
+  auto eng = HAADF_ReconstructionEngine::New();
+  eng->setTomoInputs(ti);
+  eng->setSinogram(sino);
+  eng->setGeometry(geom);
+  eng->setAdvParams(adv);
+  eng->setForwardModel(fm);
+  eng->execute();
+  std::ofstream txt(dir + "/run.txt");
+  txt << std::setprecision(17);
+  txt << "error " << eng->getErrorCondition() << "\n";
+  txt << "dims " << geom->N_z << " " << geom->N_x << " " << geom->N_y << "\n";
+  txt << "sinogram " << sino->N_theta << " " << sino->N_r << " " << sino->N_t << "\n";
+  txt << "sweeps " << eng->getSweeps() << "\n";
+  for (double c : eng->getCosts()) txt << "cost " << c << "\n";
+  std::printf("engine_cli: error %d, %d sweeps\n", eng->getErrorCondition(), eng->getSweeps());
+  return eng->getErrorCondition() == 0 ? 0 : 1;
+}
+
 int main(int argc, char** argv)
 {
+  if (argc > 1 && std::string(argv[1]) == "--mrc") return run_mrc(argc, argv);
   if (argc != 15) {
     std::fprintf(stderr, "usage: %s sino.f64 angles.f64 N_r N_t N_theta thickness sigma_x p "
                          "outer inner joint noise seed out_prefix\n", argv[0]);

@@ -96,8 +96,43 @@ class ServerStats(C.Structure):
 
 JOB_QUEUED, JOB_SETUP, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(5)
 
+class MrcInfo(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("mode", C.c_int32),
+                ("next", C.c_int32), ("nlabels", C.c_int32), ("fei", C.c_int32),
+                ("amin", C.c_float), ("amax", C.c_float), ("amean", C.c_float)]
+
+
+class TiltSelect(C.Structure):
+    _fields_ = [("use_subvolume", C.c_int32), ("x_start", C.c_int32), ("x_end", C.c_int32),
+                ("y_start", C.c_int32), ("y_end", C.c_int32), ("z_start", C.c_int32),
+                ("z_end", C.c_int32), ("interpolate_factor", C.c_double),
+                ("excluded_views", C.POINTER(C.c_int32)), ("n_excluded", C.c_int32),
+                ("tilts", _F), ("n_tilts", C.c_int32)]
+
+
+class TiltDims(C.Structure):
+    _fields_ = [("N_theta", C.c_int32), ("N_r", C.c_int32), ("N_t", C.c_int32),
+                ("x_start", C.c_int32), ("x_end", C.c_int32), ("y_start", C.c_int32),
+                ("y_end", C.c_int32), ("delta_r", C.c_double), ("delta_t", C.c_double)]
+
+
+_I32P = C.POINTER(C.c_int32)
+
 _SIG = {
     "mbir_version": (C.c_int32, []),
+    # files (include/mbir_io.h)
+    "mbir_mrc_read_info": (C.c_int32, [C.c_char_p, C.POINTER(MrcInfo), _F, _F]),
+    "mbir_mrc_read_tilt_series": (C.c_int32, [C.c_char_p, C.POINTER(TiltSelect),
+                                              C.POINTER(TiltDims), _D, _D, _I32P]),
+    "mbir_mrc_write_volume": (C.c_int32, [C.c_char_p, _F, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_int32, C.c_int32, C.c_double]),
+    "mbir_original_x_dims": (C.c_int32, [C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                         C.c_double, _I32P, _I32P]),
+    "mbir_write_reconstructed_sinogram": (C.c_int32, [C.c_char_p, C.c_int32, C.c_int32,
+                                                      C.c_int32, _D, _F, _D, _D]),
+    "mbir_write_raw_volume": (C.c_int32, [C.c_char_p, _F, C.c_int32, C.c_int32, C.c_int32]),
+    "mbir_read_raw_volume": (C.c_int32, [C.c_char_p, _D, C.c_int32, C.c_int32, C.c_int32,
+                                         C.c_int32]),
     "mbir_create": (C.c_int32, [C.POINTER(Opts), C.POINTER(C.c_void_p)]),
     "mbir_destroy": (C.c_int32, [C.c_void_p]),
     "mbir_geometry_init": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
@@ -439,3 +474,76 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---- files (include/mbir_io.h) ----
+
+def mrc_read_info(path):
+    """Header fields and, for an FEI-style extended header, per-section tilt and pixel size."""
+    info = MrcInfo()
+    _chk("mbir_mrc_read_info", lib().mbir_mrc_read_info(path.encode(), C.byref(info), None, None))
+    tilts = pix = None
+    if info.fei:
+        tilts = np.zeros(info.nz, np.float32)
+        pix = np.zeros(info.nz, np.float32)
+        _chk("mbir_mrc_read_info",
+             lib().mbir_mrc_read_info(path.encode(), C.byref(info), _fptr(tilts), _fptr(pix)))
+    return info, tilts, pix
+
+
+def mrc_read_tilt_series(path, use_subvolume=False, x=(0, 0), y=(0, 0), z=(0, 0),
+                         interpolate_factor=1.0, excluded=(), tilts=None):
+    """(counts [N_theta][N_r][N_t], angles (deg), good views, TiltDims) of an MRC tilt series."""
+    ex = (C.c_int32 * max(1, len(excluded)))(*excluded)
+    tl = None if tilts is None else np.ascontiguousarray(tilts, np.float32)
+    sel = TiltSelect(1 if use_subvolume else 0, x[0], x[1], y[0], y[1], z[0], z[1],
+                     interpolate_factor, ex, len(excluded), _fptr(tl),
+                     0 if tl is None else len(tl))
+    d = TiltDims()
+    _chk("mbir_mrc_read_tilt_series",
+         lib().mbir_mrc_read_tilt_series(path.encode(), C.byref(sel), C.byref(d), None, None, None))
+    counts = np.zeros((d.N_theta, d.N_r, d.N_t))
+    ang = np.zeros(d.N_theta)
+    good = np.zeros(d.N_theta, np.int32)
+    _chk("mbir_mrc_read_tilt_series",
+         lib().mbir_mrc_read_tilt_series(path.encode(), C.byref(sel), C.byref(d), _dptr(counts),
+                                         _dptr(ang), good.ctypes.data_as(_I32P)))
+    return counts, ang, good, d
+
+
+def mrc_write_volume(path, vol, x0=0, x1=None, length_x=0.0):
+    v = np.ascontiguousarray(vol, np.float32)
+    nz, nx, ny = v.shape
+    _chk("mbir_mrc_write_volume",
+         lib().mbir_mrc_write_volume(path.encode(), _fptr(v), nz, nx, ny, x0,
+                                     nx if x1 is None else x1, length_x))
+
+
+def original_x_dims(N_x, x0, delta_xz, N_r, delta_r):
+    s, e = C.c_int32(), C.c_int32()
+    _chk("mbir_original_x_dims",
+         lib().mbir_original_x_dims(N_x, x0, delta_xz, N_r, delta_r, C.byref(s), C.byref(e)))
+    return s.value, e.value
+
+
+def write_reconstructed_sinogram(path, counts, e, I0, mu):
+    c = np.ascontiguousarray(counts, np.float64)
+    ef = np.ascontiguousarray(e, np.float32)
+    i0 = np.ascontiguousarray(I0, np.float64)
+    m = np.ascontiguousarray(mu, np.float64)
+    nt, nr, ntt = c.shape
+    _chk("mbir_write_reconstructed_sinogram",
+         lib().mbir_write_reconstructed_sinogram(path.encode(), nt, nr, ntt, _dptr(c), _fptr(ef),
+                                                 _dptr(i0), _dptr(m)))
+
+
+def write_raw_volume(path, vol):
+    v = np.ascontiguousarray(vol, np.float32)
+    _chk("mbir_write_raw_volume", lib().mbir_write_raw_volume(path.encode(), _fptr(v), *v.shape))
+
+
+def read_raw_volume(path, shape, interp=0):
+    out = np.zeros(shape)
+    _chk("mbir_read_raw_volume",
+         lib().mbir_read_raw_volume(path.encode(), _dptr(out), *shape, interp))
+    return out

@@ -1,4 +1,4 @@
-"""The C-ABI library loads and exports every entry point include/mbir_tem.h declares
+"""The C-ABI library loads and exports every entry point include/*.h declares
 (no device calls: this runs without a GPU)."""
 import ctypes as C
 import os
@@ -13,7 +13,9 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    text = open(os.path.join(REPO, "include", "mbir_tem.h")).read()
+    inc = os.path.join(REPO, "include")
+    text = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc))
+                   if f.endswith(".h"))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(mbir_[a-z0-9_]+)\s*\(", text)))
 

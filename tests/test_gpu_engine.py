@@ -36,6 +36,53 @@ def run_cli(y, ang, n_r, n_t, thickness, sigma, q, outer, inner, joint, noise, s
         return vol, nu, costs
 
 
+def test_adapter_reads_and_writes_reference_files(gpu_ctx, tmp_path):
+    """The engine read from an MRC tilt series (FEI tilt angles and pixel size) and
+    writing the reference's outputs: the MRC volume (x range over the detector,
+    sections reversed), gains / offsets / variances, ReconstructedObject.bin and
+    ReconstructedSinogram.bin must hold exactly the C-ABI's result for the same
+    problem given as arrays."""
+    from test_io import make_tilt_series, parse_mrc
+
+    rng = np.random.default_rng(4)
+    nv = 25
+    g, ang, vol, y = small_problem(n_views=nv, gains=rng.uniform(0.9, 1.1, nv),
+                                   offsets=rng.uniform(-0.03, 0.03, nv))
+    ang32 = ang.astype(np.float32)
+    y32 = y.astype(np.float32)
+    mrc = str(tmp_path / "tilts.mrc")
+    make_tilt_series(mrc, y32.transpose(0, 2, 1), mode=2, tilts=ang32, pixel=1e-9)
+    d = float(np.float32(1e-9)) * 1e9  # detector pitch the reader derives
+    th = g.LengthZ / 2
+    out = str(tmp_path)
+    r = subprocess.run([CLI, "--mrc", mrc, str(th), "0.1", "1.2", "4", "3", "1", "1", "7", out],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    geom = capi.geometry_init(g.N_r, g.N_t, ang32.astype(np.float64), th, delta_r=d, delta_t=d)
+    job = capi.Job(gpu_ctx, geom, ang32.astype(np.float64), y32.astype(np.float64), seed=7,
+                   num_outer=4, num_inner=3, joint_estimation=1, noise_model=1,
+                   stop_threshold=1e-4)
+    job.run()
+    x = job.volume()
+    I0, mu, alpha = job.nuisance()
+    nz, nx, ny = x.shape
+    h, data = parse_mrc(os.path.join(out, "recon.mrc"))
+    c0, c1 = capi.original_x_dims(nx, geom.x0, geom.delta_xz, g.N_r, d)
+    c1 = min(c1 + 1, nx)
+    assert (h["nx"], h["ny"], h["nz"]) == (c1 - c0, ny, nz)
+    for k in range(nz):
+        np.testing.assert_array_equal(data[k], x[nz - 1 - k, c0:c1, :].T)
+    np.testing.assert_array_equal(np.fromfile(os.path.join(out, "gains.bin"), "<f8"), I0)
+    np.testing.assert_array_equal(np.fromfile(os.path.join(out, "offsets.bin"), "<f8"), mu)
+    np.testing.assert_array_equal(np.fromfile(os.path.join(out, "variances.bin"), "<f8"), alpha)
+    raw = np.fromfile(os.path.join(out, "ReconstructedObject.bin"), "<f8").reshape(ny, nx, nz)
+    np.testing.assert_array_equal(raw, x.transpose(2, 1, 0).astype(np.float64))
+    e = job.error_sinogram().astype(np.float64)
+    want = (y32.astype(np.float64) - e) * I0[:, None, None] + mu[:, None, None]
+    got = np.fromfile(os.path.join(out, "ReconstructedSinogram.bin"), "<f8").reshape(want.shape)
+    np.testing.assert_array_equal(got, want)
+
+
 def test_adapter_execute_matches_oracle(gpu_ctx):
     """30 outer x 10 inner with joint gain/offset and noise estimation.  The adapter
     must give exactly the C-ABI's result for the same problem and seed.  Against the
