@@ -35,6 +35,7 @@ SOURCES = [
     (os.path.join(ENGINE, "HAADF_ReconstructionEngine.cpp"), ["-x", "c++"]),
     # the engine's files (MRC tilt series in, MRC volume and binary outputs out)
     (os.path.join(ENGINE, "MBIRIO.cpp"), ["-x", "c++"]),
+    (os.path.join(ENGINE, "HAADF_MultiResolutionReconstruction.cpp"), ["-x", "c++"]),
     (os.path.join(ENGINE, "mbir_io_capi.cpp"), ["-x", "c++"]),
 ]
 CLI = os.path.join(HERE, "engine_cli")

@@ -16,7 +16,7 @@ constexpr int kIcdBlock = 512;        // threads per ICD line workgroup (8 wave6
 constexpr int kIcdWaves = kIcdBlock / 64;
 constexpr int kMaxNy = 256;           // voxels per line handled by one workgroup
 constexpr int kMaxSlots = 2048;       // N_theta * NB per column
-constexpr int kMaxNB = 8;             // detector bins per view per column
+constexpr int kMaxNB = 12;            // detector bins per view per column (voxels up to 5.5 delta_r)
 constexpr int kVlrCap = 4;            // detector rows per voxel slice
 
 // One scheduled line: its id and its column's place in the column table, so a

@@ -12,6 +12,12 @@
 // lets the engine write the reference's outputs into out_dir: recon.mrc, gains.bin,
 // offsets.bin, variances.bin, ReconstructedSinogram.bin, ReconstructedObject.bin, and
 // run.txt as above.
+//
+//   engine_cli --multires <tilts.mrc> levels thickness sigma_x diffuseness outer inner
+//              stop seed <temp_dir> <out.mrc>
+// runs HAADF_MultiResolutionReconstruction (coarse to fine, levels >= 1) and writes
+// <temp_dir>/levels.txt: one line per level with its voxel size multiple, sigma_x, q,
+// stop threshold, inner iterations, interpolation flag, geometry and sweeps.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -19,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "HAADF_MultiResolutionReconstruction.h"
 #include "HAADF_ReconstructionEngine.h"
 
 using namespace mbirtem;
@@ -79,9 +86,46 @@ static int run_mrc(int argc, char** argv)
   return eng->getErrorCondition() == 0 ? 0 : 1;
 }
 
+static int run_multires(int argc, char** argv)
+{
+  if (argc != 13) {
+    std::fprintf(stderr, "usage: %s --multires tilts.mrc levels thickness sigma_x diffuseness "
+                         "outer inner stop seed temp_dir out.mrc\n", argv[0]);
+    return 2;
+  }
+  auto mr = HAADF_MultiResolutionReconstruction::New();
+  mr->InputFile = argv[2];
+  mr->NumberResolutions = std::atoi(argv[3]);
+  mr->SampleThickness = (float)std::atof(argv[4]);
+  mr->SigmaX = (float)std::atof(argv[5]);
+  mr->MRFShapeParameter = (float)std::atof(argv[6]);
+  mr->OuterIterations = std::atoi(argv[7]);
+  mr->InnerIterations = std::atoi(argv[8]);
+  mr->StopThreshold = (float)std::atof(argv[9]);
+  mr->Seed = (uint32_t)std::atoi(argv[10]);
+  mr->TempDir = argv[11];
+  mr->OutputFile = argv[12];
+  mr->execute();
+  std::ofstream txt(std::string(argv[11]) + "/levels.txt");
+  txt << std::setprecision(17);
+  txt << "error " << mr->getErrorCondition() << "\n";
+  const auto& ins = mr->getLevelInputs();
+  for (size_t i = 0; i < ins.size(); ++i) {
+    const Geometry& g = mr->getLevelGeometry()[i];
+    txt << "level " << i << " delta " << ins[i].delta_xz << " sigma " << ins[i].SigmaX << " q "
+        << ins[i].p << " stop " << ins[i].StopThreshold << " inner " << ins[i].NumIter
+        << " interp " << (int)ins[i].InterpFlag << " dims " << g.N_z << " " << g.N_x << " "
+        << g.N_y << " sweeps " << mr->getLevelSweeps()[i] << " dir " << ins[i].tempDir << "\n";
+  }
+  std::printf("engine_cli: multires error %d (%s)\n", mr->getErrorCondition(),
+              mr->getErrorMessage().c_str());
+  return mr->getErrorCondition() == 0 ? 0 : 1;
+}
+
 int main(int argc, char** argv)
 {
   if (argc > 1 && std::string(argv[1]) == "--mrc") return run_mrc(argc, argv);
+  if (argc > 1 && std::string(argv[1]) == "--multires") return run_multires(argc, argv);
   if (argc != 15) {
     std::fprintf(stderr, "usage: %s sino.f64 angles.f64 N_r N_t N_theta thickness sigma_x p "
                          "outer inner joint noise seed out_prefix\n", argv[0]);

@@ -83,6 +83,85 @@ def test_adapter_reads_and_writes_reference_files(gpu_ctx, tmp_path):
     np.testing.assert_array_equal(got, want)
 
 
+def test_multiresolution_driver(gpu_ctx, tmp_path):
+    """Two levels (voxels 2 then 1 detector pitch) through the driver: the per-level
+    parameters follow the reference's rules (sigma_x scaled by
+    2^((levels-1-i)(1-3/q)), one inner sweep after the first level, interpolation
+    from the second level on), and each level's result is exactly the engine's for
+    that level's inputs: level 0 from the tilt series alone, level 1 from level 0's
+    volume replicated 2 x 2 x 2 and its gains, offsets and variances."""
+    from test_io import make_tilt_series, parse_mrc
+
+    rng = np.random.default_rng(9)
+    nv = 31
+    g, ang, vol, y = small_problem(n_r=64, n_t=12, n_views=nv, thickness=16,
+                                   gains=rng.uniform(0.9, 1.1, nv),
+                                   offsets=rng.uniform(-0.03, 0.03, nv))
+    ang32 = ang.astype(np.float32)
+    y32 = y.astype(np.float32)
+    mrc = str(tmp_path / "tilts.mrc")
+    make_tilt_series(mrc, y32.transpose(0, 2, 1), mode=2, tilts=ang32, pixel=1e-9)
+    d = float(np.float32(1e-9)) * 1e9
+    tmp = str(tmp_path / "work")
+    out = str(tmp_path / "final.mrc")
+    r = subprocess.run([CLI, "--multires", mrc, "2", "16", "0.1", "0.2", "3", "4", "0.001", "5",
+                        tmp, out], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = open(os.path.join(tmp, "levels.txt")).read().split("\n")
+    assert lines[0] == "error 0"
+    lev = []
+    for l in lines[1:3]:
+        f = l.split()
+        lev.append(dict(delta=float(f[3]), sigma=float(f[5]), q=float(f[7]), stop=float(f[9]),
+                        inner=int(f[11]), interp=int(f[13]), dims=tuple(int(v) for v in f[15:18]),
+                        dir=f[-1]))
+    q = float(np.float32(0.2) + np.float32(1))
+    sig = float(np.float32(0.1))
+    assert lev[0]["q"] == q and lev[1]["q"] == q
+    assert lev[0]["sigma"] == pytest.approx(2 ** (1 - 3 / q) * sig, rel=1e-12)
+    assert lev[1]["sigma"] == pytest.approx(sig, rel=1e-12)
+    assert (lev[0]["delta"], lev[1]["delta"]) == (2.0, 1.0)
+    assert (lev[0]["inner"], lev[1]["inner"]) == (4, 1)
+    assert (lev[0]["interp"], lev[1]["interp"]) == (0, 1)
+    assert lev[0]["dir"].endswith("/2x") and lev[1]["dir"].endswith("/1x")
+    assert all(2 * a == b for a, b in zip(lev[0]["dims"], lev[1]["dims"]))
+    th = float(np.float32(16))
+    kw = dict(num_outer=3, joint_estimation=1, noise_model=1, seed=5)
+    # level 0, as the engine runs it from the tilt series
+    g0 = capi.geometry_init(g.N_r, g.N_t, ang32.astype(np.float64), th, delta_r=d, delta_t=d,
+                            delta_xz_in=2.0, interpolate_factor=2.0)
+    j0 = capi.Job(gpu_ctx, g0, ang32.astype(np.float64), y32, sigma_x=lev[0]["sigma"],
+                  mrf_q=q, stop_threshold=lev[0]["stop"], num_inner=4, **kw)
+    j0.run()
+    x0 = j0.volume()
+    nz0, nx0, ny0 = x0.shape
+    assert (nz0, nx0, ny0) == lev[0]["dims"]
+    raw0 = np.fromfile(os.path.join(lev[0]["dir"], "ReconstructedObject.bin"), "<f8")
+    np.testing.assert_array_equal(raw0.reshape(ny0, nx0, nz0), x0.transpose(2, 1, 0))
+    I0, mu, alpha = j0.nuisance()
+    np.testing.assert_array_equal(
+        np.fromfile(os.path.join(lev[0]["dir"], "FinalGainParameters.bin"), "<f8"), I0)
+    # level 1 from level 0's volume replicated 2 x 2 x 2 and its nuisance parameters
+    up = np.repeat(np.repeat(np.repeat(x0, 2, 0), 2, 1), 2, 2)
+    g1 = capi.geometry_init(g.N_r, g.N_t, ang32.astype(np.float64), th, delta_r=d, delta_t=d,
+                            delta_xz_in=1.0, interpolate_factor=2.0)
+    j1 = capi.Job(gpu_ctx, g1, ang32.astype(np.float64), y32, sigma_x=lev[1]["sigma"],
+                  mrf_q=q, stop_threshold=lev[1]["stop"], num_inner=1, init_volume=up,
+                  init_gain=I0, init_offset=mu, init_variance=alpha, **kw)
+    j1.run()
+    x1 = j1.volume()
+    h, data = parse_mrc(out)
+    nz, nx, ny = x1.shape
+    c0, c1 = capi.original_x_dims(nx, g1.x0, g1.delta_xz, g.N_r, d)
+    c1 = min(c1 + 1, nx)
+    assert (h["nx"], h["ny"], h["nz"]) == (c1 - c0, ny, nz)
+    for k in range(nz):
+        np.testing.assert_array_equal(data[k], x1[nz - 1 - k, c0:c1, :].T)
+    # the coarse-to-fine result is a reconstruction of the phantom
+    rel = np.sqrt(np.mean((x1 - vol) ** 2)) / np.sqrt(np.mean(vol ** 2))
+    assert rel < 0.35, rel
+
+
 def test_adapter_execute_matches_oracle(gpu_ctx):
     """30 outer x 10 inner with joint gain/offset and noise estimation.  The adapter
     must give exactly the C-ABI's result for the same problem and seed.  Against the
