@@ -99,7 +99,9 @@ typedef struct mbir_problem {
   /* sweep kernels: 0 = one launch per conflict-free step with the job's own kernel
    * (plain per-job GPU ICD); 1 = the batched step kernel (a batch of one job);
    * 2 = reserved for a whole sweep in one persistent launch (refused with -3: not
-   * enabled in this build) */
+   * enabled in this build); 3 = the launches of mode 0 captured once in a HIP graph
+   * and replayed every sweep (each step reads its lines from a fixed slot of a padded
+   * schedule uploaded before the replay; same results as mode 0) */
   int32_t kernel_mode;
   /* Bright-field mode (BrightField/BFForwardModel.cpp, BFReconstructionEngine.cpp:575-872):
    * counts are transmitted counts; the measurement is -log(counts + bf_offset)

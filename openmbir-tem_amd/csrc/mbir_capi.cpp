@@ -623,6 +623,72 @@ int run_sweep_steps(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
   return finish_sweep(j, ms, st);
 }
 
+// kernel_mode 3: the step launches of a sweep replayed from a HIP graph.  The graph
+// is captured once per job (again when the job descriptor or the schedule's shape
+// changes); a sweep uploads its permuted schedule into the padded slots, clears the
+// accumulators with a kernel and replays the graph.  The first sweep (zero skipping
+// off) runs with direct launches.
+int run_sweep_graph(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
+{
+  if (outer == 0 && inner == 0) return run_sweep_steps(j, outer, inner, st);
+  hipStream_t s = j->stream;
+  build_schedule(j, false);
+  const int nsteps = (int)j->step_ptr.size() - 1;
+  int width = 1;
+  for (int k = 0; k < nsteps; ++k) width = std::max(width, j->step_ptr[k + 1] - j->step_ptr[k]);
+  const bool stale = !j->graph_exec || nsteps != j->graph_steps || width != j->graph_lines ||
+                     std::memcmp(&j->graph_J, &j->J, sizeof(DevJob)) != 0;
+  const size_t slots = (size_t)nsteps * width;
+  if (j->gsched.n < slots) {
+    j->gsched.release();
+    HIP_TRY(j->gsched.alloc(slots));
+  }
+  if (stale) {
+    if (j->graph_exec) (void)hipGraphExecDestroy(j->graph_exec);
+    if (j->graph) (void)hipGraphDestroy(j->graph);
+    j->graph_exec = nullptr;
+    j->graph = nullptr;
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t le = hipSuccess;
+    for (int k = 0; k < nsteps && le == hipSuccess; ++k)
+      le = launch_icd_step(j->J, j->gsched.p + (size_t)k * width, width, 1, 1,
+                           j->prob.zero_skipping, j->acc.p, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(s, &g);
+    HIP_TRY(le);
+    HIP_TRY(ce);
+    j->graph = g;
+    HIP_TRY(hipGraphInstantiate(&j->graph_exec, g, nullptr, nullptr, 0));
+    j->graph_J = j->J;
+    j->graph_steps = nsteps;
+    j->graph_lines = width;
+  }
+  j->gsched_h.assign(slots, LineRef{-1, 0, 0, 0});
+  for (int k = 0; k < nsteps; ++k)
+    for (int i = j->step_ptr[k]; i < j->step_ptr[k + 1]; ++i)
+      j->gsched_h[(size_t)k * width + (i - j->step_ptr[k])] = j->sched_h[i];
+  HIP_TRY(hipMemcpyAsync(j->gsched.p, j->gsched_h.data(), slots * sizeof(LineRef),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_fill_f64((double*)j->acc.p, 0.0, sizeof(SweepAccum) / sizeof(double), s));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (j->timing) {
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
+  }
+  HIP_TRY(hipGraphLaunch(j->graph_exec, s));
+  if (j->timing) HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0;
+  if (j->timing) {
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  return finish_sweep(j, ms, st);
+}
+
 // Batched sweep: launch k runs step k of every job that has one (each job's
 // steps are its own conflict-free groups; jobs never share memory), so one
 // launch carries the lines of many jobs and the per-launch cost is shared.
@@ -746,6 +812,7 @@ int run_sweep(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
   // finish on the device (its step waits never saw the completion counts) and was
   // taken out; kernel_mode 2 is refused
   if (j->prob.kernel_mode == 2) return MBIR_ERR_UNSUPPORTED;
+  if (j->prob.kernel_mode == 3) return run_sweep_graph(j, outer, inner, st);
   if (j->prob.kernel_mode == 1) {
     mbir_sweep_stats tmp{};
     std::vector<mbir_job*> one{j};

@@ -86,6 +86,14 @@ struct mbir_job {
   std::vector<LineRef> sched_h;
   std::vector<int> step_ptr;
   std::vector<int> cptr_h;  // column-table offsets of all lines (host copy)
+  // kernel_mode 3: the sweep's step launches as a HIP graph over a padded schedule
+  // (step k's lines at gsched[k * graph_lines], unused entries line = -1)
+  DevBuf<LineRef> gsched;
+  std::vector<LineRef> gsched_h;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  DevJob graph_J{};
+  int graph_steps = 0, graph_lines = 0;
   std::atomic<int> cancel{0};
   int timing = 0;
   int sweep_counter = 0;
@@ -99,6 +107,9 @@ struct mbir_job {
     vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
     mask.release(); acc.release(); col_ptr.release(); col_row.release();
     col_a.release(); col_ka.release(); col_kw.release(); wfull.release(); alpha_b.release();
+    gsched.release();
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };

@@ -848,6 +848,7 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef*
   __shared__ IcdSmem<YPL> sm;
   if ((int)blockIdx.x >= nlines) return;
   const LineRef ref = lines[blockIdx.x];
+  if (__builtin_amdgcn_readfirstlane(ref.line) < 0) return;  // padding of a graph's step slot
   icd_line<YPL, NV, FW, true>(J, ref, first, zs_on, acc, sm, icd_col(J.col_slots));
 }
 
