@@ -37,6 +37,8 @@ SOURCES = [
     (os.path.join(ENGINE, "MBIRIO.cpp"), ["-x", "c++"]),
     (os.path.join(ENGINE, "HAADF_MultiResolutionReconstruction.cpp"), ["-x", "c++"]),
     (os.path.join(ENGINE, "mbir_io_capi.cpp"), ["-x", "c++"]),
+    # initial-parameter estimators (per-view reductions on the host)
+    (os.path.join(ENGINE, "mbir_estimate.cpp"), ["-x", "c++"]),
 ]
 CLI = os.path.join(HERE, "engine_cli")
 

@@ -10,6 +10,7 @@
 #include <cstring>
 
 #include "MBIRIO.h"
+#include "mbir_estimate.h"
 
 namespace mbirtem {
 
@@ -188,6 +189,14 @@ void HAADF_ReconstructionEngine::execute()
       return fail(-1, "Error reading Initial Reconstruction Data from File");
   }
   if (readNuisanceInputs() < 0) return fail(-1, "Error Reading Gains from File");
+  if (!T.useDefaultOffset && m_ForwardModel->InitialOffset.size() != S.N_theta) {
+    // offsets neither given nor defaulted: least-squares fit of the view means
+    // (HAADF/HAADF_ForwardModel.cpp:327-345, GenericFilters/ComputeInitialOffsets.cpp)
+    m_ForwardModel->InitialOffset.assign(S.N_theta, 0.0);
+    if (mbir_initial_offsets(S.counts.data(), S.N_theta, S.N_r, S.N_t, S.angles.data(),
+                             m_ForwardModel->InitialOffset.data(), nullptr) < 0)
+      return fail(-1, "Error initializing Input Offsets Data file");
+  }
   std::vector<float> init;
   if (!T.initialRecon.empty()) {
     if (T.initialRecon.size() != V) return fail(-1, "Error reading Initial Reconstruction Data");

@@ -120,6 +120,11 @@ _I32P = C.POINTER(C.c_int32)
 
 _SIG = {
     "mbir_version": (C.c_int32, []),
+    # estimators (include/mbir_estimate.h)
+    "mbir_initial_offsets": (C.c_int32, [_D, C.c_int32, C.c_int32, C.c_int32, _D, _D, _D]),
+    "mbir_sigma_x_estimate": (C.c_int32, [_D, C.c_int32, C.c_int32, C.c_int32, _D, C.c_int32,
+                                          C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                          C.c_double, C.c_int32, C.c_double, _D]),
     # files (include/mbir_io.h)
     "mbir_mrc_read_info": (C.c_int32, [C.c_char_p, C.POINTER(MrcInfo), _F, _F]),
     "mbir_mrc_read_tilt_series": (C.c_int32, [C.c_char_p, C.POINTER(TiltSelect),
@@ -547,3 +552,28 @@ def read_raw_volume(path, shape, interp=0):
     _chk("mbir_read_raw_volume",
          lib().mbir_read_raw_volume(path.encode(), _dptr(out), *shape, interp))
     return out
+
+
+# ---- estimators (include/mbir_estimate.h) ----
+
+def initial_offsets(counts, angles_deg):
+    """(offsets per view, (a, b)) of the least-squares fit mean[th] ~ a / cos + b."""
+    c = np.ascontiguousarray(counts, np.float64)
+    ang = np.ascontiguousarray(angles_deg, np.float64)
+    out = np.zeros(c.shape[0])
+    fit = np.zeros(2)
+    _chk("mbir_initial_offsets",
+         lib().mbir_initial_offsets(_dptr(c), *c.shape, _dptr(ang), _dptr(out), _dptr(fit)))
+    return out, tuple(fit)
+
+
+def sigma_x_estimate(counts, angles_deg, thickness, target_gain=1.0, x=(-1, -1), y=(-1, -1),
+                     bf_offset=None):
+    c = np.ascontiguousarray(counts, np.float64)
+    ang = np.ascontiguousarray(angles_deg, np.float64)
+    s = C.c_double()
+    _chk("mbir_sigma_x_estimate",
+         lib().mbir_sigma_x_estimate(_dptr(c), *c.shape, _dptr(ang), x[0], x[1], y[0], y[1],
+                                     target_gain, thickness, 0 if bf_offset is None else 1,
+                                     0.0 if bf_offset is None else bf_offset, C.byref(s)))
+    return s.value

@@ -185,3 +185,39 @@ def test_raw_volume_layout_and_interpolated_read(tmp_path):
                 assert up[z, x, y] == np.float64(vol[z // 2, x // 2, y // 2])
     with pytest.raises(capi.MbirError):
         capi.read_raw_volume(p, (5, 6, 2))
+
+
+def test_initial_offsets_least_squares():
+    # per-view means a / cos(theta) + b + noise; the fit must be the least-squares
+    # solution (numpy's solver on the same design matrix)
+    rng = np.random.default_rng(5)
+    ang = np.linspace(-60, 60, 21)
+    g = 1 / np.cos(np.radians(ang))
+    means = 0.7 * g + 0.25 + rng.normal(0, 0.01, ang.size)
+    counts = means[:, None, None] + rng.normal(0, 0.05, (ang.size, 6, 4))
+    counts -= counts.mean(axis=(1, 2), keepdims=True) - means[:, None, None]
+    off, (a, b) = capi.initial_offsets(counts, ang)
+    A = np.stack([g, np.ones_like(g)], 1)
+    (ra, rb), *_ = np.linalg.lstsq(A, counts.mean(axis=(1, 2)), rcond=None)
+    assert a == pytest.approx(ra, rel=1e-10) and b == pytest.approx(rb, rel=1e-10)
+    assert np.all(off == b)
+    with pytest.raises(capi.MbirError) as ei:  # one distinct angle: no fit
+        capi.initial_offsets(counts[:3], np.zeros(3))
+    assert ei.value.code == -2
+
+
+def test_sigma_x_estimate_hand_values():
+    # view 0 (0 deg): image rows [0, 2], [0, 2] -> mean 1, mean |dev| 1
+    # view 1 (60 deg): [1, 1], [1, 5] -> mean 2, mean |dev| 1.5
+    # thickness 2, gain 1: (1 * 1 / 2 + 1.5 * 0.5 / 2) / 2 = 0.4375
+    v0 = np.array([[0.0, 2.0], [0.0, 2.0]])
+    v1 = np.array([[1.0, 1.0], [1.0, 5.0]])
+    counts = np.stack([v0.T, v1.T])  # [theta][r = column][t = row]
+    assert capi.sigma_x_estimate(counts, [0.0, 60.0], 2.0) == pytest.approx(0.4375, rel=1e-12)
+    # gain divides, a sub-image selects: columns [1, 2) of view 0 are constant
+    assert capi.sigma_x_estimate(counts, [0.0, 60.0], 2.0, target_gain=2.0) == \
+        pytest.approx(0.4375 / 2, rel=1e-12)
+    assert capi.sigma_x_estimate(counts[:1], [0.0], 1.0, x=(1, 2)) == 0.0
+    # bright field: deviations of log(counts + offset): e^1, e^3 -> 1, 3 -> 1
+    bf = np.array([[[np.e - 0.5, np.e ** 3 - 0.5]]])
+    assert capi.sigma_x_estimate(bf, [0.0], 1.0, bf_offset=0.5) == pytest.approx(1.0, rel=1e-12)
