@@ -188,6 +188,29 @@ def server_runs(ctx, inputs, arrivals):
     return res, st
 
 
+def tune_on(ctx, inputs, k=16):
+    """Autotune the batched line kernels' form on the first k problems (each job runs
+    two sweeps, one per form); returns the choice and the timings for the JSON line."""
+    from mbirtem import capi
+    jobs = [capi.Job(ctx, w["geom"], w["angles"], w["sinogram"], **dict(w["job_kw"], kernel_mode=1))
+            for w in inputs[:k]]
+    rep = capi.autotune(ctx, jobs, sweeps=1)
+    for j in jobs:
+        j.close()
+    names = ["<=256", "<=512", "<=1024", "<=2048", ">2048"]
+    out = {}
+    for a in range(2):
+        for b in range(2):
+            for q in range(5):
+                if rep.launches[a][b][q][0] and rep.launches[a][b][q][1]:
+                    out[f"ypl{a + 1}_nv{b + 1}_wg{names[q]}"] = {
+                        "prefetch": int(rep.prefetch[a][b][q]),
+                        "us_per_wg": [round(rep.us_per_wg[a][b][q][0], 4),
+                                      round(rep.us_per_wg[a][b][q][1], 4)]}
+    log(f"autotune on {len(jobs)} slab problems: {out}")
+    return out
+
+
 def slab_figures(ctx, n, rate, big_frac, only_batched=False):
     """batched_slabs (all n problems at once) and served (Poisson arrivals) figures."""
     import numpy as np
@@ -222,8 +245,9 @@ def slab_figures(ctx, n, rate, big_frac, only_batched=False):
             log(f"slabs: {k + 1} plain runs done ({time.time() - t0:.1f}s)")
     log(f"slabs: plain per-job runs {time.time() - t0:.1f}s")
 
-    # batched: every problem submitted at t = 0
+    # batched: every problem submitted at t = 0, with the kernel forms autotuned first
     inputs = [cache[s.name] for s in specs]
+    tuning = tune_on(ctx, inputs)
     res, st = server_runs(ctx, inputs, [0.0] * n)
     p = [plain[s.name] for s in specs]
     for a, b in zip(res, p):
@@ -246,6 +270,7 @@ def slab_figures(ctx, n, rate, big_frac, only_batched=False):
         "final_rmse_equal_to_plain": True, "sweeps_total": int(sum(r["sweeps"] for r in res)),
         "launches": int(st.launches), "entries_per_launch": st.entries / max(1, st.launches),
         "peak_resident": int(st.peak_resident), "peak_device_gb": st.peak_reserved_bytes / 1e9,
+        "autotuned_forms": tuning,
     }
     log(f"batched slabs: {batched['value']:.4g} visits/s vs plain {batched['plain_value']:.4g} "
         f"({batched['vs_plain']:.2f}x), p99 {batched['time_to_converged_p99_s']:.2f}s vs "

@@ -164,6 +164,28 @@ int32_t mbir_job_destroy(mbir_job* job);
  * Returns 1 = continue, 0 = converged (stop test met), negative on error. */
 int32_t mbir_job_sweep(mbir_job* job, int32_t outer, int32_t inner, mbir_sweep_stats* stats);
 
+/* Form of the batched line kernels (both forms compute identical results): per kernel
+ * shape [N_y > 64][rows per slice > 1] and launch-size bucket (workgroups <= 256,
+ * <= 512, <= 1024, <= 2048, more) 1 = the prefetching form (shorter line latency),
+ * 0 = the form without prefetch (fewer registers).  No reference counterpart: the
+ * reference has no batched kernels. */
+typedef struct mbir_tuning {
+  int32_t prefetch[2][2][5];
+  double us_per_wg[2][2][5][2];  /* autotuning: mean launch time per workgroup [form], 0 = none */
+  int32_t launches[2][2][5][2];  /* autotuning: launches timed [form] */
+} mbir_tuning;
+
+/* Autotuner (no reference counterpart): times `sweeps` batched sweeps of the jobs
+ * with each form, launch by launch, and keeps for every shape and size bucket the
+ * faster form in ctx (used by mbir_batch_sweep and servers created afterwards).  The
+ * jobs advance by 2 x sweeps sweeps (results as with any form).  out may be NULL. */
+int32_t mbir_autotune(mbir_ctx* ctx, mbir_job* const* jobs, int32_t n, const int32_t* outer,
+                      const int32_t* inner, int32_t sweeps, mbir_tuning* out);
+/* The context's current choice (default: prefetching up to 512 workgroups), and
+ * setting a stored one. */
+int32_t mbir_get_tuning(mbir_ctx* ctx, mbir_tuning* out);
+int32_t mbir_set_tuning(mbir_ctx* ctx, const mbir_tuning* t);
+
 /* Batched sweep: one ICD sweep of each of n jobs of ctx (no reference counterpart:
  * the reference runs one job per process).  Launch k runs conflict-free step k of
  * every job that has one, so the launches carry many jobs' lines at once; jobs stay

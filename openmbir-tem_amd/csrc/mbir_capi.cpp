@@ -781,8 +781,25 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
         slots = std::max(slots, jobs[ent[f].job]->J.col_slots);
         wg += ent[f++].nlines;
       }
+      int pf = ctx->tune.choose(ent[e].ypl, ent[e].nv, wg);
+      mbir_ctx::Probe* pr = ctx->probe;
+      hipEvent_t a = nullptr, b = nullptr;
+      if (pr) {
+        pf = pr->force;
+        HIP_TRY(hipEventCreate(&a));
+        HIP_TRY(hipEventCreate(&b));
+        HIP_TRY(hipEventRecord(a, s));
+      }
       HIP_TRY(launch_icd_batch_step(ent[e].ypl, ent[e].nv, ctx->djobs.p, ctx->ents.p, e, f - e,
-                                    wg, slots, s));
+                                    wg, slots, pf, s));
+      if (pr) {
+        HIP_TRY(hipEventRecord(b, s));
+        pr->ev.push_back(a);
+        pr->ev.push_back(b);
+        pr->ypl.push_back(ent[e].ypl);
+        pr->nv.push_back(ent[e].nv);
+        pr->nwg.push_back(wg);
+      }
       e = f;
     }
   }
@@ -903,6 +920,83 @@ extern "C" int32_t mbir_batch_sweep(mbir_ctx* ctx, mbir_job* const* jobs, int32_
   if (hipSetDevice(ctx->device) != hipSuccess) return MBIR_ERR_HIP;
   std::vector<mbir_job*> all(jobs, jobs + n);
   return run_batch(ctx, all, outer, inner, stats);
+}
+
+namespace {
+
+void tuning_out(const LineTuning& t, mbir_tuning* out)
+{
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int k = 0; k < kTuneBuckets; ++k) out->prefetch[a][b][k] = t.pf[a][b][k];
+}
+
+}  // namespace
+
+extern "C" int32_t mbir_get_tuning(mbir_ctx* ctx, mbir_tuning* out)
+{
+  if (!ctx || !out) return MBIR_ERR_INPUT;
+  std::memset(out, 0, sizeof(*out));
+  tuning_out(ctx->tune, out);
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_set_tuning(mbir_ctx* ctx, const mbir_tuning* t)
+{
+  if (!ctx || !t) return MBIR_ERR_INPUT;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int k = 0; k < kTuneBuckets; ++k) ctx->tune.pf[a][b][k] = t->prefetch[a][b][k] ? 1 : 0;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_autotune(mbir_ctx* ctx, mbir_job* const* jobs, int32_t n,
+                                 const int32_t* outer, const int32_t* inner, int32_t sweeps,
+                                 mbir_tuning* out)
+{
+  if (!ctx || !jobs || n <= 0 || sweeps <= 0) return MBIR_ERR_INPUT;
+  if (hipSetDevice(ctx->device) != hipSuccess) return MBIR_ERR_HIP;
+  std::vector<mbir_job*> all(jobs, jobs + n);
+  double ms[2][2][kTuneBuckets][2] = {};
+  double wgs[2][2][kTuneBuckets][2] = {};
+  int cnt[2][2][kTuneBuckets][2] = {};
+  int rc = MBIR_OK;
+  for (int form = 1; form >= 0 && rc >= 0; --form) {
+    mbir_ctx::Probe pr;
+    pr.force = form;
+    ctx->probe = &pr;
+    for (int sw = 0; sw < sweeps && rc >= 0; ++sw) rc = run_batch(ctx, all, outer, inner, nullptr);
+    ctx->probe = nullptr;
+    for (size_t i = 0; i < pr.nwg.size(); ++i) {
+      float t = 0;
+      if (rc >= 0 && hipEventElapsedTime(&t, pr.ev[2 * i], pr.ev[2 * i + 1]) == hipSuccess) {
+        const int a = pr.ypl[i] > 1 ? 1 : 0, b = pr.nv[i] > 1 ? 1 : 0, k = tune_bucket(pr.nwg[i]);
+        ms[a][b][k][form] += t;
+        wgs[a][b][k][form] += pr.nwg[i];
+        cnt[a][b][k][form]++;
+      }
+    }
+    for (hipEvent_t e : pr.ev) (void)hipEventDestroy(e);
+  }
+  if (rc < 0) return rc;
+  LineTuning t = ctx->tune;
+  if (out) std::memset(out, 0, sizeof(*out));
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int k = 0; k < kTuneBuckets; ++k) {
+        double us[2] = {0, 0};
+        for (int f = 0; f < 2; ++f)
+          if (cnt[a][b][k][f] > 0) us[f] = 1e3 * ms[a][b][k][f] / wgs[a][b][k][f];
+        if (cnt[a][b][k][0] > 0 && cnt[a][b][k][1] > 0) t.pf[a][b][k] = us[1] <= us[0] ? 1 : 0;
+        if (out)
+          for (int f = 0; f < 2; ++f) {
+            out->us_per_wg[a][b][k][f] = us[f];
+            out->launches[a][b][k][f] = cnt[a][b][k][f];
+          }
+      }
+  ctx->tune = t;
+  if (out) tuning_out(t, out);
+  return MBIR_OK;
 }
 
 extern "C" int32_t mbir_job_nuisance_update(mbir_job* job)

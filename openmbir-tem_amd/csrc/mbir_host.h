@@ -53,6 +53,13 @@ struct mbir_ctx {
   hipStream_t stream = nullptr;  // stream of batched launches
   DevBuf<DevJob> djobs;          // job table of a batched sweep
   DevBuf<BatchEntry> ents;       // entry table of a batched sweep
+  mbir::LineTuning tune;         // form of the batched line kernels per shape and size
+  // autotuning: when set, batched sweeps force this form (0 / 1) and time each launch
+  struct Probe {
+    int force = -1;
+    std::vector<hipEvent_t> ev;       // two per launch
+    std::vector<int> ypl, nv, nwg;    // per launch
+  }* probe = nullptr;
   ~mbir_ctx()
   {
     djobs.release(); ents.release();

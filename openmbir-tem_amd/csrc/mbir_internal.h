@@ -113,6 +113,28 @@ struct TickArgs {
   TickEntry e[kTickMax];
 };
 
+// Choice between the two forms of the batched line kernels (identical results):
+// 1 = prefetching (shorter line latency), 0 = without prefetch (fewer registers), per
+// kernel shape (ypl 1|2, nv 1|2) and launch size (workgroups) bucket.
+constexpr int kTuneBuckets = 5;
+inline int tune_bucket(int nwg)
+{
+  return nwg <= 256 ? 0 : nwg <= 512 ? 1 : nwg <= 1024 ? 2 : nwg <= 2048 ? 3 : 4;
+}
+struct LineTuning {
+  int8_t pf[2][2][kTuneBuckets];
+  LineTuning()
+  {
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < kTuneBuckets; ++k) pf[a][b][k] = k <= 1 ? 1 : 0;  // <= 512: prefetch
+  }
+  int choose(int ypl, int nv, int nwg) const
+  {
+    return pf[ypl > 1 ? 1 : 0][nv > 1 ? 1 : 0][tune_bucket(nwg)];
+  }
+};
+
 // ---- launchers (all asynchronous on `s`) ----
 hipError_t launch_line_nnz(const DevJob& J, int* line_nnz, float* x, hipStream_t s);
 hipError_t launch_forward_project(const DevJob& J, const float* vol, double* out, hipStream_t s);
@@ -123,7 +145,7 @@ void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv);
 // one step of several jobs of one kernel shape: entries [e0, e0 + ne) of ent,
 // nwg workgroups in total
 hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const BatchEntry* ent,
-                                 int e0, int ne, int nwg, int slots, hipStream_t s);
+                                 int e0, int ne, int nwg, int slots, int pf, hipStream_t s);
 hipError_t launch_build_columns(const DevJob& J, int* row, float* a, hipStream_t s);
 hipError_t launch_column_weights(const DevJob& J, float* ka, float* kw, int nnz, hipStream_t s);
 hipError_t launch_export_column(const DevJob& J, int z, int x, float* vals, int* rows,
@@ -140,7 +162,7 @@ hipError_t launch_cost(const DevJob& J, const double* alpha, int noise_model, do
                        hipStream_t s);
 // one launch of the request server: kernel shape (ypl, nv), nwg workgroups
 hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& args, int nwg,
-                           hipStream_t s);
+                           int pf, hipStream_t s);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, size_t n, hipStream_t s);
 // jobs[slot] = J, ordered on s (the descriptor travels as a kernel argument)
 hipError_t launch_set_devjob(DevJob* jobs, int slot, const DevJob& J, hipStream_t s);

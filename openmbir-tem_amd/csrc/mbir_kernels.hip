@@ -1310,16 +1310,13 @@ hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, in
   return hipGetLastError();
 }
 
-// Launches that fill the device take the higher-occupancy form of the line update,
-// smaller ones the shorter-latency form (both compute the same values).
-constexpr int kFullDeviceWg = 512;
-
+// The caller picks the form of the line update (pf: prefetching; both compute the same
+// values) from its tuning table.
 hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const BatchEntry* ent,
-                                 int e0, int ne, int nwg, int slots, hipStream_t s)
+                                 int e0, int ne, int nwg, int slots, int pf, hipStream_t s)
 {
   if (ne <= 0 || nwg <= 0) return hipSuccess;
   const size_t dyn = icd_col_bytes(slots);
-  const bool pf = nwg <= kFullDeviceWg;
 #define MBIR_BATCH(Y, N)                                                                      \
   if (pf)                                                                                     \
     hipLaunchKernelGGL((k_icd_batch_step<Y, N, true>), dim3(nwg), dim3(kIcdBlock), dyn, s,    \
@@ -1341,11 +1338,10 @@ hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const Batc
 }
 
 hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& args, int nwg,
-                           hipStream_t s)
+                           int pf, hipStream_t s)
 {
   if (args.n <= 0 || args.n > kTickMax || nwg <= 0) return hipSuccess;
   const size_t dyn = icd_col_bytes(args.slots);
-  const bool pf = nwg <= kFullDeviceWg;
 #define MBIR_TICK(Y, N)                                                                        \
   if (pf)                                                                                      \
     hipLaunchKernelGGL((k_icd_tick<Y, N, true>), dim3(nwg), dim3(kIcdBlock), dyn, s, jobs,      \

@@ -80,6 +80,7 @@ DevJob tick_desc(const mbir_job* j)
 
 struct mbir_server {
   mbir_ctx* ctx = nullptr;
+  mbir::LineTuning tune;  // the context's kernel-form choice when the server was created
   int max_jobs = 64;
   uint64_t budget = 0, reserved = 0, peak_reserved = 0;
   int resident = 0;  // jobs holding a slot (in setup, running or finished but unreleased)
@@ -339,7 +340,9 @@ void mbir_server::run_loop()
       std::vector<Req*> last;  // jobs whose sweep ends with this launch
       auto flush = [&]() -> bool {
         if (args.n == 0) return true;
-        if (launch_icd_tick(ypl, nv, djobs.p, args, wg, stream) != hipSuccess) return false;
+        if (launch_icd_tick(ypl, nv, djobs.p, args, wg, tune.choose(ypl, nv, wg), stream) !=
+            hipSuccess)
+          return false;
         n_launch++;
         n_entries += args.n;
         n_wg += wg;
@@ -415,6 +418,7 @@ extern "C" int32_t mbir_server_create(mbir_ctx* ctx, const mbir_server_opts* o, 
   if (hipSetDevice(ctx->device) != hipSuccess) return MBIR_ERR_HIP;
   auto* s = new mbir_server();
   s->ctx = ctx;
+  s->tune = ctx->tune;
   int threads = 2;
   if (o) {
     if (o->max_jobs > 0) s->max_jobs = std::min(o->max_jobs, kMaxServerJobs);

@@ -118,8 +118,23 @@ class TiltDims(C.Structure):
 
 _I32P = C.POINTER(C.c_int32)
 
+
+class Tuning(C.Structure):
+    _fields_ = [("prefetch", C.c_int32 * 5 * 2 * 2), ("us_per_wg", C.c_double * 2 * 5 * 2 * 2),
+                ("launches", C.c_int32 * 2 * 5 * 2 * 2)]
+
+    def table(self):
+        """prefetch as an array [ypl>1][nv>1][bucket]"""
+        return np.array([[[self.prefetch[a][b][k] for k in range(5)] for b in range(2)]
+                         for a in range(2)])
+
 _SIG = {
     "mbir_version": (C.c_int32, []),
+    "mbir_autotune": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32,
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
+                                  C.POINTER(Tuning)]),
+    "mbir_get_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
+    "mbir_set_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
     # estimators (include/mbir_estimate.h)
     "mbir_initial_offsets": (C.c_int32, [_D, C.c_int32, C.c_int32, C.c_int32, _D, _D, _D]),
     "mbir_sigma_x_estimate": (C.c_int32, [_D, C.c_int32, C.c_int32, C.c_int32, _D, C.c_int32,
@@ -222,6 +237,28 @@ def batch_sweep(ctx, jobs, outer=None, inner=None):
     st = (SweepStats * n)()
     _chk("mbir_batch_sweep", lib().mbir_batch_sweep(ctx.h, hs, n, o, i, st))
     return list(st)
+
+
+def autotune(ctx, jobs, outer=None, inner=None, sweeps=1):
+    """Time each form of the batched line kernels on these jobs and keep the faster per
+    shape and launch-size bucket in ctx; returns the Tuning report."""
+    n = len(jobs)
+    hs = (C.c_void_p * n)(*[j.h.value for j in jobs])
+    o = (C.c_int32 * n)(*(outer if outer is not None else [0] * n))
+    i = (C.c_int32 * n)(*(inner if inner is not None else [0] * n))
+    t = Tuning()
+    _chk("mbir_autotune", lib().mbir_autotune(ctx.h, hs, n, o, i, sweeps, C.byref(t)))
+    return t
+
+
+def get_tuning(ctx):
+    t = Tuning()
+    _chk("mbir_get_tuning", lib().mbir_get_tuning(ctx.h, C.byref(t)))
+    return t
+
+
+def set_tuning(ctx, t):
+    _chk("mbir_set_tuning", lib().mbir_set_tuning(ctx.h, C.byref(t)))
 
 
 def geometry_init(N_r, N_t, angles_deg, thickness, delta_r=1.0, delta_t=1.0, delta_xz_in=1.0,

@@ -70,6 +70,41 @@ def test_batch_result_same_alone_and_in_mix(gpu_ctx):
         np.testing.assert_array_equal(a.error_sinogram(), m.error_sinogram())
 
 
+def test_autotune_keeps_results(gpu_ctx):
+    """The autotuner times both forms of the batched line kernels and keeps the faster
+    per shape and launch-size bucket; since the forms compute identical values, jobs
+    swept during and after tuning end bit for bit where untuned jobs end."""
+    probs = [small_problem(n_r=32, n_t=8, seed=5),
+             small_problem(n_r=48, n_t=16, n_views=41, thickness=12, seed=9)]
+    tuned = [_job(gpu_ctx, *p[:2], p[3], kernel_mode=1, seed=40 + i) for i, p in enumerate(probs)]
+    plain = [_job(gpu_ctx, *p[:2], p[3], kernel_mode=1, seed=40 + i) for i, p in enumerate(probs)]
+    saved = capi.get_tuning(gpu_ctx)
+    try:
+        rep = capi.autotune(gpu_ctx, tuned, inner=[0, 0], sweeps=1)
+        measured = 0
+        for a in range(2):
+            for b in range(2):
+                for k in range(5):
+                    n0, n1 = rep.launches[a][b][k][0], rep.launches[a][b][k][1]
+                    if n0 and n1:
+                        measured += 1
+                        u0, u1 = rep.us_per_wg[a][b][k][0], rep.us_per_wg[a][b][k][1]
+                        assert u0 > 0 and u1 > 0
+                        assert rep.prefetch[a][b][k] == (1 if u1 <= u0 else 0)
+        assert measured >= 1
+        assert (capi.get_tuning(gpu_ctx).table() == rep.table()).all()
+        for _ in range(2):  # as the two tuning sweeps (first-sweep flags: inner 0)
+            capi.batch_sweep(gpu_ctx, plain, inner=[0, 0])
+        for it in range(2, 5):
+            capi.batch_sweep(gpu_ctx, tuned, inner=[it, it])
+            capi.batch_sweep(gpu_ctx, plain, inner=[it, it])
+        for a, b in zip(tuned, plain):
+            np.testing.assert_array_equal(a.volume(), b.volume())
+            np.testing.assert_array_equal(a.error_sinogram(), b.error_sinogram())
+    finally:
+        capi.set_tuning(gpu_ctx, saved)
+
+
 def test_batch_cost_monotone(gpu_ctx):
     probs = [small_problem(seed=s) for s in (31, 32, 33, 34)]
     jobs = [_job(gpu_ctx, *p[:2], p[3], kernel_mode=1, seed=s) for s, p in enumerate(probs)]
