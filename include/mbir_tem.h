@@ -116,6 +116,14 @@ typedef struct mbir_problem {
   double bragg_T;     /* Bragg threshold T (BFReconstructionArgsParser.cpp:176, default 3) */
   double bragg_delta; /* delta of the Bragg rejector (:179, default 0.5) */
   double bf_offset;   /* added to the counts before the log (:182, default 0) */
+  /* bright field: 1 = the reference's NHICD schedule (BrightField/BFConstants.h:51,
+   * BFReconstructionEngine.cpp:641-884): each call updates either the next twentieth
+   * of a seeded random list of all lines (homogeneous) or the lines whose 5 x 5
+   * Hamming-filtered update magnitude exceeds its 95th percentile (non-homogeneous),
+   * alternately, and the stop test compares the ROI update magnitude with the ROI
+   * volume sum every 20 calls; 0 = full random-order sweeps */
+  int32_t bf_nhicd;
+  int32_t pad_;
 } mbir_problem;
 
 /* Per-sweep statistics (updateVoxels, HAADF_ReconstructionEngine_UpdateVoxels.cpp:717-786) */
@@ -163,6 +171,14 @@ int32_t mbir_job_destroy(mbir_job* job);
 /* One ICD sweep: updateVoxels(outer, inner, ...) (…_UpdateVoxels.cpp:594-817).
  * Returns 1 = continue, 0 = converged (stop test met), negative on error. */
 int32_t mbir_job_sweep(mbir_job* job, int32_t outer, int32_t inner, mbir_sweep_stats* stats);
+
+/* NHICD list helpers of the bright-field schedule (host only): the update-magnitude
+ * map [N_z][N_x] filtered with the 5 x 5 Hamming window, zero outside
+ * (BFReconstructionEngine::ComputeVSC, BrightField/BFReconstructionEngine_Extra.cpp:888-925),
+ * and the threshold of the non-homogeneous list, the (n - n/20)-th smallest value
+ * (SetNonHomThreshold, :934-987). */
+int32_t mbir_nhicd_filter(const float* mag, int32_t N_z, int32_t N_x, double* filtered);
+int32_t mbir_nhicd_threshold(const double* values, int32_t n, double* threshold);
 
 /* Form of the batched line kernels (both forms compute identical results): per kernel
  * shape [N_y > 64][rows per slice > 1] and launch-size bucket (workgroups <= 256,

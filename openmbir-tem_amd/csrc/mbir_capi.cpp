@@ -371,6 +371,7 @@ int setup_job(mbir_job* j)
   HIP_TRY(hipMemcpyAsync(j->vlr_n.p, vn.data(), vn.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(j->vlr_val.p, vval.data(), vval.size() * sizeof(float), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(j->mask.p, mask.data(), L, hipMemcpyHostToDevice, s));
+  j->mask_h = mask;
   HIP_TRY(hipMemcpyAsync(j->I0.p, j->I0h.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(j->mu.p, j->muh.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(j->alpha.p, j->alphah.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
@@ -402,10 +403,23 @@ int setup_job(mbir_job* j)
   // one job alone: the atomic e-update is the faster form (the batch and server paths
   // switch to the plain form for conflict-free schedules)
   J.e_atomic = 1;
+  J.magmap = nullptr;
   if (P.bf_mode) {
     J.wfull = j->wfull.p;
     J.e_atomic = 0;  // the selector refresh is a read-modify-write of its own
     set_bragg(j, P.bragg_T);
+    if (P.bf_nhicd) {
+      // NHICD: update magnitudes start at zero; one seeded random list of all lines
+      // (BFReconstructionEngine.cpp:595-608)
+      HIP_TRY(j->magmap.alloc(L));
+      HIP_TRY(launch_fill_f32(j->magmap.p, 0.f, L, s));
+      J.magmap = j->magmap.p;
+      j->nh_all.resize(L);
+      for (size_t l = 0; l < L; ++l) j->nh_all[l] = (int)l;
+      std::mt19937 rng(P.seed * 2654435761u + 17u);
+      std::shuffle(j->nh_all.begin(), j->nh_all.end(), rng);
+      j->bytes += (int64_t)L * 4;
+    }
   }
   // QGGMRF constants (HAADF/HAADF_QGGMRFPriorModel.cpp:10-19), MRF_P = 2
   const double sp = std::pow(P.sigma_x, 2.0), spq = std::pow(P.sigma_x, 2.0 - P.mrf_q);
@@ -621,6 +635,103 @@ int run_sweep_steps(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
     (void)hipEventDestroy(e1);
   }
   return finish_sweep(j, ms, st);
+}
+
+// ---- bright-field NHICD (BrightField/BFReconstructionEngine.cpp:641-884,
+// BFReconstructionEngine_Extra.cpp:476-1273) ----
+
+// 5 x 5 Hamming window of the update-magnitude filter (BFReconstructionEngine.cpp:157-181)
+constexpr double kNhHamming[5][5] = {{0.0013, 0.0086, 0.0159, 0.0086, 0.0013},
+                                     {0.0086, 0.0581, 0.1076, 0.0581, 0.0086},
+                                     {0.0159, 0.1076, 0.1993, 0.1076, 0.0159},
+                                     {0.0086, 0.0581, 0.1076, 0.0581, 0.0086},
+                                     {0.0013, 0.0086, 0.0159, 0.0086, 0.0013}};
+constexpr int kNhParts = 20;  // k_NumHomogeniousIter = k_NumNonHomogeniousIter
+
+// ComputeVSC (Extra.cpp:888-925): the magnitude map filtered with the window, zero
+// outside the volume.
+void nhicd_filter(const float* mag, int Nz, int Nx, double* out)
+{
+  for (int i = 0; i < Nz; ++i)
+    for (int j = 0; j < Nx; ++j) {
+      double f = 0;
+      for (int p = -2; p <= 2; ++p)
+        for (int q = -2; q <= 2; ++q)
+          if (i + p >= 0 && i + p < Nz && j + q >= 0 && j + q < Nx)
+            f += kNhHamming[p + 2][q + 2] * (double)mag[(size_t)(i + p) * Nx + (j + q)];
+      out[(size_t)i * Nx + j] = f;
+    }
+}
+
+// SetNonHomThreshold (Extra.cpp:934-987): the (n - n/20)-th smallest filtered value.
+double nhicd_threshold(std::vector<double> v)
+{
+  const size_t n = v.size();
+  const size_t k = n - n / kNhParts;
+  if (k >= n) return v.empty() ? 0.0 : *std::max_element(v.begin(), v.end());
+  std::nth_element(v.begin(), v.begin() + (long)k, v.end());
+  return v[k];
+}
+
+// Conflict-free steps of a list of lines: classes (z, x mod S), lines of a class in
+// increasing x, classes in a seeded random order; lines with empty columns dropped.
+void build_list_schedule(mbir_job* j, const std::vector<int>& lines)
+{
+  const int Nx = j->g.N_x, S = j->spacing;
+  std::vector<std::vector<int>> cls((size_t)j->g.N_z * S);
+  std::vector<int> sorted(lines);
+  std::sort(sorted.begin(), sorted.end());
+  for (int l : sorted)
+    if (j->nnz_h[l] > 0) cls[(size_t)(l / Nx) * S + (l % Nx) % S].push_back(l);
+  std::vector<int> order;
+  for (size_t c = 0; c < cls.size(); ++c)
+    if (!cls[c].empty()) order.push_back((int)c);
+  std::mt19937 rng(j->prob.seed + 7919u * (uint32_t)j->sweep_counter);
+  std::shuffle(order.begin(), order.end(), rng);
+  j->sched_h.clear();
+  j->step_ptr.assign(1, 0);
+  for (int c : order) {
+    for (int l : cls[c]) j->sched_h.push_back(LineRef{l, j->cptr_h[l], j->nnz_h[l], 0});
+    j->step_ptr.push_back((int)j->sched_h.size());
+  }
+}
+
+// One NHICD call: the lines of `lines` in conflict-free steps, per-step launches.
+int run_list_steps(mbir_job* j, const std::vector<int>& lines, int outer, int inner)
+{
+  hipStream_t s = j->stream;
+  build_list_schedule(j, lines);
+  int rc = prepare_sweep_on(j, s, nullptr);
+  if (rc < 0) return rc;
+  const int nsteps = (int)j->step_ptr.size() - 1;
+  for (int k = 0; k < nsteps; ++k) {
+    const int b = j->step_ptr[k], n = j->step_ptr[k + 1] - b;
+    HIP_TRY(launch_icd_step(j->J, j->sched.p + b, n, outer, inner, j->prob.zero_skipping,
+                            j->acc.p, s));
+  }
+  return finish_sweep(j, 0.0, nullptr);
+}
+
+double roi_volume_sum(mbir_job* j)
+{
+  double* d = j->cost.p + 3;  // the cost buffer's spare fourth double
+  if (launch_fill_f64(d, 0.0, 1, j->stream) != hipSuccess) return -1;
+  if (launch_roi_sum(j->J, d, j->stream) != hipSuccess) return -1;
+  double v = 0;
+  if (hipMemcpyAsync(&v, d, sizeof(double), hipMemcpyDeviceToHost, j->stream) != hipSuccess ||
+      hipStreamSynchronize(j->stream) != hipSuccess)
+    return -1;
+  return v;
+}
+
+int fetch_magmap(mbir_job* j)
+{
+  const size_t L = (size_t)j->g.N_z * j->g.N_x;
+  j->magmap_h.resize(L);
+  HIP_TRY(hipMemcpyAsync(j->magmap_h.data(), j->magmap.p, L * sizeof(float),
+                         hipMemcpyDeviceToHost, j->stream));
+  HIP_TRY(hipStreamSynchronize(j->stream));
+  return MBIR_OK;
 }
 
 // kernel_mode 3: the step launches of a sweep replayed from a HIP graph.  The graph
@@ -933,6 +1044,20 @@ void tuning_out(const LineTuning& t, mbir_tuning* out)
 
 }  // namespace
 
+extern "C" int32_t mbir_nhicd_filter(const float* mag, int32_t N_z, int32_t N_x, double* out)
+{
+  if (!mag || !out || N_z <= 0 || N_x <= 0) return MBIR_ERR_INPUT;
+  nhicd_filter(mag, N_z, N_x, out);
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_nhicd_threshold(const double* v, int32_t n, double* thr)
+{
+  if (!v || !thr || n <= 0) return MBIR_ERR_INPUT;
+  *thr = nhicd_threshold(std::vector<double>(v, v + n));
+  return MBIR_OK;
+}
+
 extern "C" int32_t mbir_get_tuning(mbir_ctx* ctx, mbir_tuning* out)
 {
   if (!ctx || !out) return MBIR_ERR_INPUT;
@@ -1039,6 +1164,12 @@ extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, 
     const double des = j->prob.bragg_T;
     double tmp = num_iter > 1 ? kDefBragg : des;
     set_bragg(j, tmp);
+    // NHICD state (BFReconstructionEngine.cpp:606-635): the call counter, the next
+    // homogeneous part and the ROI volume sum of the last check
+    const bool nhicd = j->prob.bf_nhicd != 0;
+    uint32_t eff = 0;
+    int part = 0;
+    double prev_mag = 0;
     for (int outer = 0; outer < j->prob.num_outer; ++outer) {
       if (outer > 0) {
         num_iter = 1;
@@ -1046,7 +1177,51 @@ extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, 
       }
       for (int inner = 0; inner < num_iter; ++inner) {
         if (num_iter > 1 && inner == 0) set_bragg(j, kDefBragg);
-        status = run_sweep(j, outer, inner, nullptr);
+        if (!nhicd) {
+          status = run_sweep(j, outer, inner, nullptr);
+        } else {
+          if (j->cancel.load()) return MBIR_ERR_CANCELLED;
+          std::vector<int> list;
+          const size_t n = j->nh_all.size();
+          if (eff % 2 == 0) {
+            // homogeneous: the next twentieth of the random list, the last one longer
+            // (:681-712)
+            part %= kNhParts;
+            size_t ne = n / kNhParts;
+            if (part == kNhParts - 1) ne += n - ne * kNhParts;
+            const size_t start = (n / kNhParts) * (size_t)part;
+            list.assign(j->nh_all.begin() + (long)start, j->nh_all.begin() + (long)(start + ne));
+            part++;
+          } else {
+            // non-homogeneous: lines whose filtered update magnitude exceeds the
+            // threshold, in a seeded random order (Extra.cpp:571-591, 990-1025)
+            if ((rc = fetch_magmap(j)) < 0) return rc;
+            std::vector<double> filt(n);
+            nhicd_filter(j->magmap_h.data(), j->g.N_z, j->g.N_x, filt.data());
+            const double thr = nhicd_threshold(filt);
+            for (size_t l = 0; l < n; ++l)
+              if (filt[l] > thr) list.push_back((int)l);
+            std::mt19937 rng(j->prob.seed * 31u + eff);
+            std::shuffle(list.begin(), list.end(), rng);
+          }
+          rc = run_list_steps(j, list, outer, inner);
+          if (rc < 0) return rc;
+          // stop test every 20 calls: ROI update magnitude against the ROI volume sum
+          // of 20 calls before (Extra.cpp:1217-1268; BFReconstructionEngine.cpp:739-750)
+          status = 1;
+          if (eff % kNhParts == 0 && eff > 0 && prev_mag > 0) {
+            if ((rc = fetch_magmap(j)) < 0) return rc;
+            double sum = 0;
+            for (size_t l = 0; l < n; ++l)
+              if (j->mask_h[l] == 1) sum += j->magmap_h[l];
+            if (std::fabs(sum) / std::fabs(prev_mag) < j->stop_threshold) status = 0;
+          }
+          if (eff % kNhParts == 0) {
+            prev_mag = roi_volume_sum(j);
+            if (prev_mag < 0) return MBIR_ERR_HIP;  // the volume is non-negative
+          }
+          eff++;
+        }
         if (status < 0) return status;
         ns++;
         if (status == 0) break;

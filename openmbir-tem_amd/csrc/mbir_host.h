@@ -93,6 +93,12 @@ struct mbir_job {
   std::vector<LineRef> sched_h;
   std::vector<int> step_ptr;
   std::vector<int> cptr_h;  // column-table offsets of all lines (host copy)
+  // bright-field NHICD: per-line update magnitudes, the seeded random list of all
+  // lines, and the ROI mask on the host
+  DevBuf<float> magmap;
+  std::vector<float> magmap_h;
+  std::vector<int> nh_all;
+  std::vector<uint8_t> mask_h;
   // kernel_mode 3: the sweep's step launches as a HIP graph over a padded schedule
   // (step k's lines at gsched[k * graph_lines], unused entries line = -1)
   DevBuf<LineRef> gsched;
@@ -115,6 +121,7 @@ struct mbir_job {
     mask.release(); acc.release(); col_ptr.release(); col_row.release();
     col_a.release(); col_ka.release(); col_kw.release(); wfull.release(); alpha_b.release();
     gsched.release();
+    magmap.release();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (stream) (void)hipStreamDestroy(stream);

@@ -81,6 +81,8 @@ struct DevJob {
                          // 4 P, P = 8 * 64 / TY streams): the line update's dynamic LDS
   uint32_t spare_b;      // M * 4: byte offset of the spare detector row (N_t zeros) that
                          // follows e and wfull, the target of padding column entries
+  float* magmap;         // [N_z * N_x] per-line sum of |x_new - x_old| of the last update
+                         // (NHICD), nullptr = not kept
   const LineRef* sched;  // [N_z * N_x] lines of the current sweep, step by step
   SweepAccum* acc;       // the current sweep's accumulators
 };
@@ -164,6 +166,8 @@ hipError_t launch_cost(const DevJob& J, const double* alpha, int noise_model, do
 hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& args, int nwg,
                            int pf, hipStream_t s);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, size_t n, hipStream_t s);
+// out[0] += sum of the volume over the lines of the ROI mask (FP64); out zeroed by the caller
+hipError_t launch_roi_sum(const DevJob& J, double* out, hipStream_t s);
 // jobs[slot] = J, ordered on s (the descriptor travels as a kernel argument)
 hipError_t launch_set_devjob(DevJob* jobs, int slot, const DevJob& J, hipStream_t s);
 // bright field (BrightField/BFForwardModel.cpp): offset-only joint estimation

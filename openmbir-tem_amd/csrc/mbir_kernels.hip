@@ -367,7 +367,7 @@ struct IcdSmem {
   // per wave: ROI sums of |x_new - x_old| and |x_old| (fp32, at most 256 non-negative
   // terms per line), and the counts zero-skipped + 2^10 updated + 2^20 detector rows
   // of the updated slices
-  float wsu[kIcdWaves], wsv[kIcdWaves];
+  float wsu[kIcdWaves], wsv[kIcdWaves], wmg[kIcdWaves];  // wmg: |x_new - x_old|, all slices
   int wcnt[kIcdWaves];
   unsigned char vst[NY];      // per slice: 1 updated, 2 zero-skipped
   unsigned char vns[NY];      // per slice: detector rows (VLR size)
@@ -637,7 +637,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   __syncthreads();  // E
 
   // ---- 5. voxel solve on lane groups, even slices then odd slices ----
-  float wsu = 0.f, wsv = 0.f;  // this wave's statistics (uniform)
+  float wsu = 0.f, wsv = 0.f, wmg = 0.f;  // this wave's statistics (uniform)
   int wcnt = 0;
   for (int phase = 0; phase < 2; ++phase) {
     const int vpp = (Ny + 1 - phase) / 2;  // slices of this parity
@@ -668,7 +668,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
     }
     group_reduce(G, s1, s2, lo, hi);
     const bool anypos = hi > 0.f;
-    float su = 0.f, sv = 0.f;  // this lane's voxel (group leaders only)
+    float su = 0.f, sv = 0.f, mg = 0.f;  // this lane's voxel (group leaders only)
     int vst = 0, nrow = 0;
     if (active && gl == 0) {
       float xnew = v;
@@ -691,6 +691,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
         J.x[lbase + y] = xnew;
         vst = 1;
         nrow = sm.vns[y];
+        mg = fabsf(xnew - v);
         if (roi) {
           su = fabsf(xnew - v);
           sv = fabsf(v);
@@ -704,12 +705,14 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
     // summed in a fixed order, counts by ballots
     wsu += wave_sum(su);
     wsv += wave_sum(sv);
+    if (J.magmap) wmg += wave_sum(mg);
     const int nr = NV == 1 ? (int)__popcll(__ballot(vst == 1)) : wave_sum(nrow);
     wcnt += (int)__popcll(__ballot(vst == 2)) + ((int)__popcll(__ballot(vst == 1)) << 10) +
             (nr << 20);
     if (phase == 1 && lane == 0) {
       sm.wsu[wave] = wsu;
       sm.wsv[wave] = wsv;
+      sm.wmg[wave] = wmg;
       sm.wcnt[wave] = wcnt;
     }
     __syncthreads();  // F, G
@@ -730,6 +733,10 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       if (lane == 0 && nz) atomicAdd(&acc->zero_skipped, (unsigned long long)nz);
       if (lane == 0 && nu) atomicAdd(&acc->updated, (unsigned long long)nu);
       if (lane == 0 && nn) atomicAdd(&acc->nnz_updated, (unsigned long long)nn);
+      if (J.magmap) {  // the line's update magnitude (BFUpdateYSlice.cpp:171, 303-304)
+        const float mgl = wave_sum(lane < kIcdWaves ? sm.wmg[lane] : 0.f);
+        if (lane == 0) J.magmap[line] = mgl;
+      }
     }
   };
   if constexpr (PF) line_stats();
@@ -1211,6 +1218,20 @@ __global__ void k_fill_f32(float* p, float v, size_t n)
 {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
+// Sum of the volume over the ROI lines (BFReconstructionEngine_Extra.cpp:1216-1232).
+__global__ __launch_bounds__(kBlock) void k_roi_sum(DevJob J, double* out)
+{
+  __shared__ double red[kBlock];
+  double s = 0.0;
+  const int L = J.N_z * J.N_x;
+  for (int l = blockIdx.x; l < L; l += gridDim.x) {
+    if (!J.mask[l]) continue;  // uniform over the block
+    for (int y = threadIdx.x; y < J.N_y; y += blockDim.x) s += J.x[(size_t)l * J.N_y + y];
+  }
+  s = block_sum_d(s, red);
+  if (threadIdx.x == 0 && s != 0.0) atomicAdd(out, s);
+}
+
 __global__ void k_fill_u32(unsigned int* p, unsigned int v, size_t n)
 {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -1367,6 +1388,13 @@ hipError_t launch_fill_u32(unsigned int* p, unsigned int v, size_t n, hipStream_
 hipError_t launch_set_devjob(DevJob* jobs, int slot, const DevJob& J, hipStream_t s)
 {
   hipLaunchKernelGGL(k_set_devjob, dim3(1), dim3(64), 0, s, jobs, slot, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_roi_sum(const DevJob& J, double* out, hipStream_t s)
+{
+  const int L = J.N_z * J.N_x;
+  hipLaunchKernelGGL(k_roi_sum, dim3(std::max(1, std::min(L, 1024))), dim3(kBlock), 0, s, J, out);
   return hipGetLastError();
 }
 

@@ -55,7 +55,7 @@ class Problem(C.Structure):
         ("zero_skipping", C.c_int32), ("seed", C.c_uint32),
         ("schedule", C.c_int32), ("rows_per_step", C.c_int32), ("kernel_mode", C.c_int32),
         ("bf_mode", C.c_int32), ("bragg_T", C.c_double), ("bragg_delta", C.c_double),
-        ("bf_offset", C.c_double),
+        ("bf_offset", C.c_double), ("bf_nhicd", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -130,6 +130,8 @@ class Tuning(C.Structure):
 
 _SIG = {
     "mbir_version": (C.c_int32, []),
+    "mbir_nhicd_filter": (C.c_int32, [_F, C.c_int32, C.c_int32, _D]),
+    "mbir_nhicd_threshold": (C.c_int32, [_D, C.c_int32, _D]),
     "mbir_autotune": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32,
                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
                                   C.POINTER(Tuning)]),
@@ -294,7 +296,7 @@ def make_problem(geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3
                  num_outer=1, num_inner=20, joint_estimation=0, noise_model=0, zero_skipping=1,
                  seed=12345, schedule=0, rows_per_step=1, kernel_mode=0, init_volume=None,
                  default_recon=0.0, init_gain=None, init_offset=None, init_variance=None,
-                 bf_mode=0, bragg_T=3.0, bragg_delta=0.5, bf_offset=0.0):
+                 bf_mode=0, bragg_T=3.0, bragg_delta=0.5, bf_offset=0.0, bf_nhicd=0):
     """mbir_problem for the arrays given; returns (Problem, arrays to keep alive)."""
     ang = np.ascontiguousarray(angles_deg, np.float64)
     cnt = np.ascontiguousarray(counts, np.float32)
@@ -325,6 +327,7 @@ def make_problem(geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3
     p.seed = seed
     p.schedule, p.rows_per_step, p.kernel_mode = schedule, rows_per_step, kernel_mode
     p.bf_mode, p.bragg_T, p.bragg_delta, p.bf_offset = bf_mode, bragg_T, bragg_delta, bf_offset
+    p.bf_nhicd = bf_nhicd
     return p, keep
 
 
@@ -614,3 +617,17 @@ def sigma_x_estimate(counts, angles_deg, thickness, target_gain=1.0, x=(-1, -1),
                                      target_gain, thickness, 0 if bf_offset is None else 1,
                                      0.0 if bf_offset is None else bf_offset, C.byref(s)))
     return s.value
+
+
+def nhicd_filter(mag):
+    m = np.ascontiguousarray(mag, np.float32)
+    out = np.zeros(m.shape)
+    _chk("mbir_nhicd_filter", lib().mbir_nhicd_filter(_fptr(m), m.shape[0], m.shape[1], _dptr(out)))
+    return out
+
+
+def nhicd_threshold(values):
+    v = np.ascontiguousarray(values, np.float64).ravel()
+    t = C.c_double()
+    _chk("mbir_nhicd_threshold", lib().mbir_nhicd_threshold(_dptr(v), v.size, C.byref(t)))
+    return t.value

@@ -109,3 +109,44 @@ def test_bf_run_matches_oracle(gpu_ctx):
     _, mug, alg = job.nuisance()
     np.testing.assert_allclose(mug, mu, atol=2e-3)
     np.testing.assert_allclose(alg, al, rtol=5e-2)
+
+
+def test_bf_nhicd_converges_to_the_full_sweep_minimum(gpu_ctx):
+    """The NHICD schedule (homogeneous twentieths alternating with the filtered top-5 %
+    lists) only reorders the updates.  With one outer iteration the Bragg threshold
+    stays at its "regular" value 5e10, the problem is convex, and NHICD must reach the
+    minimum the full random-order sweeps reach.  Zero skipping is off here: with it,
+    NHICD from a zero volume (as the reference's, BFUpdateYSlice.cpp:228-232) skips
+    from the second call on every zero voxel whose neighbours are zero, so a region
+    that no line of the first twentieth reached stays at zero."""
+    g, ang, vol, counts = bf_problem(bragg_frac=0.03)
+    offs = np.full(len(ang), -np.log(2000.0))
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
+    c32 = counts.astype(np.float32)
+    full = capi.Job(gpu_ctx, geom, ang, c32, init_offset=offs, seed=7, num_outer=1,
+                    num_inner=40, stop_threshold=1e-9, zero_skipping=0, **BF)
+    _, cf = full.run()
+    nh = capi.Job(gpu_ctx, geom, ang, c32, init_offset=offs, seed=7, num_outer=1,
+                  num_inner=1000, stop_threshold=1e-9, zero_skipping=0, bf_nhicd=1, **BF)
+    rs, cn = nh.run()
+    assert rs.sweeps == 1000
+    assert all(b <= a * (1 + 1e-6) for a, b in zip(cn, cn[1:]))  # convex: never up
+    xf, xn = full.volume().astype(np.float64), nh.volume().astype(np.float64)
+    rel = np.sqrt(np.mean((xn - xf) ** 2)) / np.sqrt(np.mean(xf ** 2))
+    assert rel < 1e-2, rel
+    assert cn[-1] == pytest.approx(cf[-1], rel=1e-4)
+
+
+def test_bf_nhicd_stop_rule_every_twenty_calls(gpu_ctx):
+    """The NHICD stop test runs on calls 20, 40, ... (ROI update magnitude of the map
+    against the ROI volume sum taken 20 calls before): a threshold above any ratio
+    ends the run after the 21st call, a zero threshold never."""
+    g, ang, vol, counts = bf_problem()
+    offs = np.full(len(ang), -np.log(2000.0))
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
+    c32 = counts.astype(np.float32)
+    for stop, calls in ((10.0, 21), (0.0, 45)):
+        job = capi.Job(gpu_ctx, geom, ang, c32, init_offset=offs, seed=3, num_outer=1,
+                       num_inner=45, stop_threshold=stop, bf_nhicd=1, **BF)
+        rs, _ = job.run()
+        assert rs.sweeps == calls, (stop, rs.sweeps)
