@@ -178,6 +178,25 @@ __device__ inline float wave_sum(float v)
   v += __int_as_float(dpp_upd<0x143, 0xC>(__float_as_int(v)));
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
+template <int CTRL, int ROWS>
+__device__ inline double dpp_upd(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_upd<CTRL, ROWS>((int)b), hi = dpp_upd<CTRL, ROWS>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double wave_sum(double v)
+{
+  v += dpp_upd<0x111, 0xF>(v);
+  v += dpp_upd<0x112, 0xF>(v);
+  v += dpp_upd<0x114, 0xF>(v);
+  v += dpp_upd<0x118, 0xF>(v);
+  v += dpp_upd<0x142, 0xA>(v);
+  v += dpp_upd<0x143, 0xC>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 __device__ inline int wave_sum(int v)
 {
   v += dpp_upd<0x111, 0xF>(v);
@@ -344,13 +363,43 @@ __global__ void k_column_weights(DevJob J, float* ka, float* kw, int nnz)
 
 // Byte-offset access through a uniform base pointer: base in scalar registers plus
 // a 32-bit offset in one vector register (global load/store with a scalar base).
+// Accesses through pointers typed as global memory (address space 1) compile to
+// global_* instructions.  The job descriptors of the batched kernels are read from
+// device memory, and through their plain (generic) pointers the compiler emits flat
+// accesses, which count against the LDS wait counter too: every LDS wait of the line
+// update would then also wait for all of its loads in flight.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ inline gptr<T> gp(T* p)
+{
+  return (gptr<T>)p;
+}
+template <class T>
+__device__ inline gptr<const T> gp(const T* p)
+{
+  return (gptr<const T>)p;
+}
+
 __device__ inline float ldo(const float* base, uint32_t boff)
 {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + boff);
+  return *(gptr<const float>)((gptr<const char>)base + boff);
 }
-__device__ inline float* pto(float* base, uint32_t boff)
+__device__ inline gptr<float> pto(float* base, uint32_t boff)
 {
-  return reinterpret_cast<float*>(reinterpret_cast<char*>(base) + boff);
+  return (gptr<float>)((gptr<char>)base + boff);
+}
+// A line reference read through a global pointer (field by field: a struct lvalue in
+// another address space does not bind to the copy constructor)
+__device__ inline LineRef load_ref(const LineRef* p)
+{
+  const gptr<const LineRef> q = gp(p);
+  return LineRef{q->line, q->p0, q->nc, q->pad};
+}
+template <class T>
+__device__ inline void atomic_add_g(T* p, T v)
+{
+  (void)__hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Per-line LDS of the line update for lines of at most 64 YPL voxels.  The column
@@ -364,6 +413,7 @@ struct IcdSmem {
   double T1[NY], T2[NY];
   // theta partial sums per wave and slice (FP64)
   double pd1[kIcdWaves][NY], pd2[kIcdWaves][NY];
+  double kk[kIcdWaves];  // per wave: sum over the column of I0 a * I0 a w (theta2 factor)
   // per wave: ROI sums of |x_new - x_old| and |x_old| (fp32, at most 256 non-negative
   // terms per line), and the counts zero-skipped + 2^10 updated + 2^20 detector rows
   // of the updated slices
@@ -417,7 +467,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   const int Ny = J.N_y;
   const int line = __builtin_amdgcn_readfirstlane(ref.line);
   const int z = line / J.N_x, xi = line % J.N_x;
-  const bool roi = J.mask[line] != 0;  // used by the solve
+  const bool roi = gp(J.mask)[line] != 0;  // used by the solve
   const size_t lbase = (size_t)line * Ny;
   const bool zs_active = zs_on && !first;
   const int p0 = __builtin_amdgcn_readfirstlane(ref.p0);
@@ -436,38 +486,39 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   const int ntot = P * C;                                     // <= kMaxSlots
   const int s0 = p * C;
   const int cnt = min(max(nc - s0, 0), C);
-  uint32_t vtb[YPL][NV];  // byte offset of the lane's detector row position t
-  float vv[YPL][NV];      // 0 where the lane has no entry: it then adds nothing
-  bool vok[YPL][NV];
+  // the lane's voxel-line response, loaded here with everything else and unpacked
+  // (vtb, vv, vok) once the column is staged
+  int vt0_[YPL], vn_[YPL];
+  float vval_[YPL][NV];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
     const int y = yl + TY * jy;
-    const bool act = y < Ny;
-    const int yc = act ? y : 0;
-    const int t0 = J.vlr_t0[yc];
-    const int n = act ? J.vlr_n[yc] : 0;
+    const int yc = y < Ny ? y : 0;
+    vt0_[jy] = gp(J.vlr_t0)[yc];
+    vn_[jy] = gp(J.vlr_n)[yc];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      vok[jy][k] = k < n;
-      vv[jy][k] = k < n ? J.vlr_val[yc * kVlrCap + k] : 0.f;
-      vtb[jy][k] = k < n ? (uint32_t)(t0 + k) * 4u : 0u;
-    }
+    for (int k = 0; k < NV; ++k) vval_[jy][k] = gp(J.vlr_val)[yc * kVlrCap + k];
   }
 
   // ---- 2. one round trip: the 3x3 neighbour lines (wave w: line w, wave 0 also
   // line 8, lanes along y), the column table (for LDS) and, PF, the detector rows of
-  // this lane's first batch ----
+  // this lane's first batch.  Every load here is unconditional (addresses clamped
+  // into the arrays, values selected afterwards) and nothing computes on a loaded
+  // value before the last load is issued: a load under a per-lane branch, or one
+  // whose value is used at once, is waited for before the next load goes out ----
   float nbv[2][YPL];
+  bool nbin[2][YPL];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int l = wave + 8 * h;
     const int zz = z + l / 3 - 1, xx = xi + l % 3 - 1;
     const bool in = l < 9 && zz >= 0 && zz < J.N_z && xx >= 0 && xx < J.N_x;
-    const float* src = J.x + ((size_t)(in ? zz : 0) * J.N_x + (in ? xx : 0)) * Ny;
+    const gptr<const float> src = gp(J.x) + ((size_t)(in ? zz : 0) * J.N_x + (in ? xx : 0)) * Ny;
 #pragma unroll
     for (int jy = 0; jy < YPL; ++jy) {
       const int y = lane + 64 * jy;
-      nbv[h][jy] = (in && y < Ny) ? src[y] : 0.f;
+      nbv[h][jy] = src[y < Ny ? y : Ny - 1];
+      nbin[h][jy] = in && y < Ny;
     }
   }
   constexpr int kColIt = kMaxSlots / kIcdBlock;
@@ -475,20 +526,23 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   float ka_[kColIt], kw_[kColIt];
 #pragma unroll
   for (int it = 0; it < kColIt; ++it) {
-    if (it * kIcdBlock < nc) {  // uniform
+    if (it == 0 || it * kIcdBlock < nc) {  // uniform; a column has at least one entry
       const int i = tid + it * kIcdBlock;
       const int ic = p0 + (i < nc ? i : nc - 1);
-      rw_[it] = (uint32_t)J.col_row[ic] * 4u;
-      ka_[it] = J.col_ka[ic];
-      kw_[it] = J.col_kw[ic];
+      rw_[it] = (uint32_t)gp(J.col_row)[ic];
+      ka_[it] = gp(J.col_ka)[ic];
+      kw_[it] = gp(J.col_kw)[ic];
     }
   }
   uint32_t r0[PF ? U : 1];
   if constexpr (PF) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) r0[u] = u < cnt ? (uint32_t)J.col_row[p0 + s0 + u] * 4u : J.spare_b;
+    for (int u = 0; u < U; ++u) r0[u] = (uint32_t)gp(J.col_row)[p0 + min(s0 + u, nc - 1)];
   }
-  const int vn_t = J.vlr_n[tid < Ny ? tid : 0];
+  const int vn_t = gp(J.vlr_n)[tid < Ny ? tid : 0];
+  float vl_t[NV];  // the slice's voxel-line response values (thread tid < N_y)
+#pragma unroll
+  for (int k = 0; k < NV; ++k) vl_t[k] = gp(J.vlr_val)[(tid < Ny ? tid : 0) * kVlrCap + k];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int l = wave + 8 * h;
@@ -496,20 +550,38 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
 #pragma unroll
       for (int jy = 0; jy < YPL; ++jy) {
         const int y = lane + 64 * jy;
-        if (y < Ny) sm.nbl[l][y + 1] = nbv[h][jy];
+        if (y < Ny) sm.nbl[l][y + 1] = nbin[h][jy] ? nbv[h][jy] : 0.f;
       }
       if (lane == 0) sm.nbl[l][0] = 0.f;
       if (lane == 1) sm.nbl[l][Ny + 1] = 0.f;
     }
   }
+  // theta2 of slice y is (sum_k vlr_yk^2) * sum over the column of (I0 a)(I0 a w):
+  // the column part once per line, exact products summed in FP64
+  double kk = 0.0;
 #pragma unroll
   for (int it = 0; it < kColIt; ++it) {
     const int i = tid + it * kIcdBlock;
     if (it * kIcdBlock < ntot && i < ntot) {
       const bool in = i < nc;
-      col.rb[i] = in ? rw_[it] : J.spare_b;
+      col.rb[i] = in ? rw_[it] * 4u : J.spare_b;
       col.ka[i] = in ? ka_[it] : 0.f;
       col.kw[i] = in ? kw_[it] : 0.f;
+      if constexpr (!FW)
+        if (in) kk = __fma_rn((double)ka_[it], (double)kw_[it], kk);
+    }
+  }
+  uint32_t vtb[YPL][NV];  // byte offset of the lane's detector row position t
+  float vv[YPL][NV];      // 0 where the lane has no entry: it then adds nothing
+  bool vok[YPL][NV];
+#pragma unroll
+  for (int jy = 0; jy < YPL; ++jy) {
+    const int n = yl + TY * jy < Ny ? vn_[jy] : 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      vok[jy][k] = k < n;
+      vv[jy][k] = k < n ? vval_[jy][k] : 0.f;
+      vtb[jy][k] = k < n ? (uint32_t)(vt0_[jy] + k) * 4u : 0u;
     }
   }
   // ---- 3. (PF) the first batch's error-sinogram loads, in flight across the barrier ----
@@ -539,55 +611,81 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       }
     }
   };
+  // padding entries (u >= cnt) read the spare row: in bounds, zero gains
   float ev0[PF ? U : 1][YPL][NV], wf0[PF ? U : 1][YPL][NV];
   if constexpr (PF) {
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (((u & ~3)) < cnt) load_entry(r0[u], ev0[u], wf0[u]);
+    for (int u = 0; u < U; ++u) load_entry(u < cnt ? r0[u] * 4u : J.spare_b, ev0[u], wf0[u]);
+  }
+  if constexpr (!FW) {
+    kk = wave_sum(kk);
+    if (lane == 0) sm.kk[wave] = kk;
   }
   __syncthreads();  // A: neighbour lines and column visible
   if (tid < Ny) {  // read after barrier D
     sm.x[tid] = sm.nbl[4][tid + 1];
     sm.vns[tid] = vn_t;
   }
+  double vsq_t = 0.0;  // sum_k vlr^2 of slice tid: theta2 / column factor
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    if (k < vn_t) vsq_t = __fma_rn((double)vl_t[k], (double)vl_t[k], vsq_t);
 
   // ---- 4. theta1 / theta2 (:343-369): every slice of the line (a zero-skipped
-  // slice's sums are simply unused) ----
+  // slice's sums are simply unused).  Plain weights: theta1 of slice y is
+  // sum_k vlr_yk sum_entries (I0 a w) e, the inner sums per lane as exact products
+  // accumulated in FP64 (no compensation needed); theta2 comes from the column
+  // factor above.  Bright field: both per entry, Kahan-compensated fp32. ----
   Kahan t1[YPL], t2[YPL];
+  double a1[YPL][NV];
+#pragma unroll
+  for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) a1[jy][k] = 0.0;
   auto accumulate = [&](int base, const auto& ev, const auto& wf) {
+    if constexpr (!FW) {
 #pragma unroll
-    for (int q = 0; q < U / 4; ++q) {
-      if (base + 4 * q < cnt) {
-        const float4 a4 = ka4(base + 4 * q);
-        float4 w4 = a4;
-        if constexpr (!FW) w4 = kw4(base + 4 * q);
-        const float kav[4] = {a4.x, a4.y, a4.z, a4.w};
-        const float kwv[4] = {w4.x, w4.y, w4.z, w4.w};
+      for (int q = 0; q < U / 4; ++q) {
+        if (base + 4 * q < cnt) {
+          const float4 w4 = kw4(base + 4 * q);
+          const float kwv[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int u = 4 * q + t;
+          for (int t = 0; t < 4; ++t) {
+            const double kd = (double)kwv[t];
 #pragma unroll
-          for (int jy = 0; jy < YPL; ++jy)
+            for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
-            for (int k = 0; k < NV; ++k) {
-            const float pe = kav[t] * vv[jy][k];
-            // per-view weight folded in: pe w = kw vlr.  The bright-field path
-            // carries w per entry, its sign being the Bragg selector: a rejected
-            // entry (w < 0) weighs delta T / (|e| sqrt(|w|)) more
-            // (BrightField/BFForwardModel.cpp:851-892); padding and lanes without
-            // an entry have pe = 0 and add nothing
-            float pw;
-            if constexpr (FW) {
-              const float w = wf[u][jy][k];
-              const float aw = fabsf(w);
-              pw = (w >= 0.f || pe == 0.f)
-                       ? pe * aw
-                       : pe * aw * (J.bragg_dT / (fabsf(ev[u][jy][k]) * sqrtf(aw)));
-            } else {
-              pw = kwv[t] * vv[jy][k];
-            }
-            t2[jy].add(pe * pw);
-            t1[jy].add(ev[u][jy][k] * pw);
+              for (int k = 0; k < NV; ++k)
+                a1[jy][k] = __fma_rn((double)ev[4 * q + t][jy][k], kd, a1[jy][k]);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < U / 4; ++q) {
+        if (base + 4 * q < cnt) {
+          const float4 a4 = ka4(base + 4 * q);
+          const float kav[4] = {a4.x, a4.y, a4.z, a4.w};
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int u = 4 * q + t;
+#pragma unroll
+            for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+              for (int k = 0; k < NV; ++k) {
+                // w per entry, its sign the Bragg selector: a rejected entry (w < 0)
+                // weighs delta T / (|e| sqrt(|w|)) more
+                // (BrightField/BFForwardModel.cpp:851-892); padding and lanes without
+                // an entry have pe = 0 and add nothing
+                const float pe = kav[t] * vv[jy][k];
+                const float w = wf[u][jy][k];
+                const float aw = fabsf(w);
+                const float pw = (w >= 0.f || pe == 0.f)
+                                     ? pe * aw
+                                     : pe * aw * (J.bragg_dT / (fabsf(ev[u][jy][k]) * sqrtf(aw)));
+                t2[jy].add(pe * pw);
+                t1[jy].add(ev[u][jy][k] * pw);
+              }
           }
         }
       }
@@ -608,28 +706,41 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   double d1[YPL], d2[YPL];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
-    d1[jy] = (double)t1[jy].s - (double)t1[jy].c;
-    d2[jy] = (double)t2[jy].s - (double)t2[jy].c;
+    if constexpr (FW) {
+      d1[jy] = (double)t1[jy].s - (double)t1[jy].c;
+      d2[jy] = (double)t2[jy].s - (double)t2[jy].c;
+    } else {
+      d1[jy] = 0.0;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) d1[jy] = __fma_rn((double)vv[jy][k], a1[jy][k], d1[jy]);
+      d2[jy] = 0.0;
+    }
   }
   for (int m = TY; m < 64; m <<= 1)
 #pragma unroll
     for (int jy = 0; jy < YPL; ++jy) {
       d1[jy] += __shfl_xor(d1[jy], m);
-      d2[jy] += __shfl_xor(d2[jy], m);
+      if constexpr (FW) d2[jy] += __shfl_xor(d2[jy], m);
     }
   if (sub == 0)
 #pragma unroll
     for (int jy = 0; jy < YPL; ++jy) {
       sm.pd1[wave][yl + TY * jy] = d1[jy];
-      sm.pd2[wave][yl + TY * jy] = d2[jy];
+      if constexpr (FW) sm.pd2[wave][yl + TY * jy] = d2[jy];
     }
   __syncthreads();  // D
   if (tid < Ny) {
     double T1 = 0.0, T2 = 0.0;
 #pragma unroll
-    for (int w = 0; w < kIcdWaves; ++w) {
-      T1 += sm.pd1[w][tid];
-      T2 += sm.pd2[w][tid];
+    for (int w = 0; w < kIcdWaves; ++w) T1 += sm.pd1[w][tid];
+    if constexpr (FW) {
+#pragma unroll
+      for (int w = 0; w < kIcdWaves; ++w) T2 += sm.pd2[w][tid];
+    } else {
+      double K = 0.0;
+#pragma unroll
+      for (int w = 0; w < kIcdWaves; ++w) K += sm.kk[w];
+      T2 = K * vsq_t;
     }
     sm.T1[tid] = -T1;  // THETA1 *= -1 (:369)
     sm.T2[tid] = T2;
@@ -688,7 +799,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
         nv = nv < dlo ? dlo : (nv > dhi ? dhi : nv);
         if (nv < 0.0) nv = 0.0;  // POSITIVITY_CONSTRAINT (:394-400)
         xnew = (float)nv;
-        J.x[lbase + y] = xnew;
+        gp(J.x)[lbase + y] = xnew;
         vst = 1;
         nrow = sm.vns[y];
         mg = fabsf(xnew - v);
@@ -728,31 +839,33 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       const float sv = wave_sum(lane < kIcdWaves ? sm.wsv[lane] : 0.f);
       const int cn = wave_sum(lane < kIcdWaves ? sm.wcnt[lane] : 0);
       const int nz = cn & 1023, nu = (cn >> 10) & 1023, nn = nc * (cn >> 20);
-      if (lane == 0 && su != 0.f) atomicAdd(&acc->sum_abs_update, (double)su);
-      if (lane == 0 && sv != 0.f) atomicAdd(&acc->sum_abs_value, (double)sv);
-      if (lane == 0 && nz) atomicAdd(&acc->zero_skipped, (unsigned long long)nz);
-      if (lane == 0 && nu) atomicAdd(&acc->updated, (unsigned long long)nu);
-      if (lane == 0 && nn) atomicAdd(&acc->nnz_updated, (unsigned long long)nn);
+      if (lane == 0 && su != 0.f) atomic_add_g(&acc->sum_abs_update, (double)su);
+      if (lane == 0 && sv != 0.f) atomic_add_g(&acc->sum_abs_value, (double)sv);
+      if (lane == 0 && nz) atomic_add_g(&acc->zero_skipped, (unsigned long long)nz);
+      if (lane == 0 && nu) atomic_add_g(&acc->updated, (unsigned long long)nu);
+      if (lane == 0 && nn) atomic_add_g(&acc->nnz_updated, (unsigned long long)nn);
       if (J.magmap) {  // the line's update magnitude (BFUpdateYSlice.cpp:171, 303-304)
         const float mgl = wave_sum(lane < kIcdWaves ? sm.wmg[lane] : 0.f);
-        if (lane == 0) J.magmap[line] = mgl;
+        if (lane == 0) gp(J.magmap)[line] = mgl;
       }
     }
   };
   if constexpr (PF) line_stats();
 
-  // ---- 7. e -= I0 a vlr dx (:423-449): product first, then one add (no fma), in
-  // every form ----
-  float d[YPL];
+  // ---- 7. e -= I0 a (vlr dx) (:423-449): the lane's vlr dx once, then per entry
+  // one product and one add (no fma), in every form ----
+  float d[YPL], vd[YPL][NV];
   bool upd[YPL];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
     const int y = yl + TY * jy;
     d[jy] = y < Ny ? sm.dx[y] : 0.f;
     upd[jy] = y < Ny && sm.vst[y] == 1;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) vd[jy][k] = __fmul_rn(vv[jy][k], d[jy]);
   }
   auto enew = [&](float e, float ka, int jy, int k) {
-    return __fadd_rn(e, -__fmul_rn(__fmul_rn(ka, vv[jy][k]), d[jy]));
+    return __fadd_rn(e, -__fmul_rn(ka, vd[jy][k]));
   };
   // padding entries (zero gains, spare row) are updated like the others: they leave
   // the spare row at zero
@@ -802,8 +915,9 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
 #pragma unroll
               for (int k = 0; k < NV; ++k)
                 if (d[jy] != 0.f && vok[jy][k])
-                  unsafeAtomicAdd(pto(J.e, rr[t] + vtb[jy][k]),
-                                  -__fmul_rn(__fmul_rn(aa[t], vv[jy][k]), d[jy]));
+                  (void)__hip_atomic_fetch_add(pto(J.e, rr[t] + vtb[jy][k]),
+                                               -__fmul_rn(aa[t], vd[jy][k]), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
         }
       }
     } else {
@@ -881,7 +995,7 @@ void k_icd_batch_step(const DevJob* __restrict__ jobs,
   const int i = g - E.wg_off;
   if (i < 0 || i >= E.nlines) return;
   const DevJob J = jobs[__builtin_amdgcn_readfirstlane(E.job)];
-  const LineRef ref = E.lines[i];
+  const LineRef ref = load_ref(E.lines + i);
   icd_line<YPL, NV, false, PF>(J, ref, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm,
                                icd_col(slots));
 }
@@ -911,7 +1025,7 @@ void k_icd_tick(const DevJob* __restrict__ jobs, const TickArgs args)
   const int flags = __builtin_amdgcn_readfirstlane(args.e[lo].flags);
   if (i < 0 || i >= nl) return;
   const DevJob J = jobs[job];
-  const LineRef ref = J.sched[l0 + i];
+  const LineRef ref = load_ref(J.sched + l0 + i);
   icd_line<YPL, NV, false, PF>(J, ref, flags & 1, (flags >> 1) & 1, J.acc, sm,
                                icd_col(args.slots));
 }
