@@ -108,7 +108,8 @@ struct TickEntry {
   int wg_off;
   int flags;  // bit 0: first sweep, bit 1: zero skipping on
 };
-constexpr int kTickMax = 96;  // entries per launch, passed by value as a kernel argument
+constexpr int kTickMax = 48;  // entries per launch, passed by value as a kernel argument (1 kB:
+                               // a larger argument block slows every launch)
 struct TickArgs {
   int n;
   int slots;  // column entries (after padding) of the launch's longest line: dynamic LDS

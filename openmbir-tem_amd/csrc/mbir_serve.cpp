@@ -30,7 +30,8 @@ using namespace mbir;
 namespace {
 
 constexpr int kMaxServerJobs = 256;
-constexpr int kTicksInFlight = 16;  // launches the host may run ahead of the device
+constexpr int kTickEvery = 4;   // ticks per run-ahead event
+constexpr int kTickEvents = 4;  // events in flight: the host runs at most 16 ticks ahead
 
 struct Req {
   uint64_t ticket = 0;
@@ -258,7 +259,7 @@ void mbir_server::run_loop()
 {
   (void)hipSetDevice(ctx->device);
   std::vector<Req*> act;
-  std::vector<hipEvent_t> tick_ev(kTicksInFlight, nullptr);
+  std::vector<hipEvent_t> tick_ev(kTickEvents, nullptr);
   for (auto& e : tick_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
   int64_t tick = 0, n_launch = 0, n_entries = 0, n_wg = 0;
   TickArgs args;
@@ -382,10 +383,15 @@ void mbir_server::run_loop()
       continue;
     }
     if (launched) {
-      // bound how far the host runs ahead, so arrivals and sweep ends are seen promptly
-      hipEvent_t& e = tick_ev[tick % kTicksInFlight];
-      if (tick >= kTicksInFlight) (void)hipEventSynchronize(e);
-      (void)hipEventRecord(e, stream);
+      // bound how far the host runs ahead, so arrivals and sweep ends are seen
+      // promptly: one event every kTickEvery ticks (an event between two launches
+      // delays the second)
+      if (tick % kTickEvery == 0) {
+        const int64_t k = tick / kTickEvery;
+        hipEvent_t& e = tick_ev[k % kTickEvents];
+        if (k >= kTickEvents) (void)hipEventSynchronize(e);
+        (void)hipEventRecord(e, stream);
+      }
       tick++;
       std::lock_guard<std::mutex> lk(mu);
       st.ticks = tick;
