@@ -747,32 +747,53 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   }
   __syncthreads();  // E
 
-  // ---- 5. voxel solve on lane groups, even slices then odd slices ----
+  // ---- 5. voxel solve on lane groups, even slices then odd slices.  G = 16 / YPL
+  // lanes per voxel (a DPP row at most: every slice of one parity fits the
+  // workgroup), each lane holding NS of the 26 neighbours, the same ones in both
+  // phases: their offsets, weights and whether they lie inside the volume in z and x
+  // are set once, and a phase issues all its LDS reads together ----
+  constexpr int G = 16 / YPL, NS = (26 + G - 1) / G;
+  const int vi = tid / G, gl = tid % G;
+  int ndy[NS];
+  bool nok[NS];
+  float nw[NS];
+  const float* nsrc[NS];  // slice 0 of the neighbour line in LDS (this line: sm.x)
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const int n = gl + q * G;
+    const int m = n < 13 ? n : n + 1;  // the 27-point index without the centre
+    const int dy = m / 9 - 1, dz = (m / 3) % 3 - 1, dx = m % 3 - 1;
+    const int l = (dz + 1) * 3 + (dx + 1);
+    ndy[q] = dy;
+    nok[q] = n < 26 && z + dz >= 0 && z + dz < J.N_z && xi + dx >= 0 && xi + dx < J.N_x;
+    nw[q] = filter_weight(dy, dz, dx);
+    nsrc[q] = l == 4 ? sm.x : &sm.nbl[l < 9 ? l : 0][1];
+  }
   float wsu = 0.f, wsv = 0.f, wmg = 0.f;  // this wave's statistics (uniform)
   int wcnt = 0;
   for (int phase = 0; phase < 2; ++phase) {
     const int vpp = (Ny + 1 - phase) / 2;  // slices of this parity
-    int G = 16;                            // lanes per voxel (one DPP row at most)
-    while (G > 1 && G * vpp > kIcdBlock) G >>= 1;
-    const int vi = tid / G, gl = tid % G;
     const int y = 2 * vi + phase;
     const bool active = vi < vpp;
     const float v = active ? sm.x[y] : 0.f;
     float s1 = 0.f, s2 = 0.f, lo = 0.f, hi = 0.f;  // lo/hi include the zeroed centre
     if (active) {
-      for (int n = gl; n < 26; n += G) {
-        const int m = n < 13 ? n : n + 1;  // the 27-point index without the centre
-        const int dy = m / 9 - 1, dz = (m / 3) % 3 - 1, dx = m % 3 - 1;
-        const int l = (dz + 1) * 3 + (dx + 1), yy = y + dy;
-        const bool ex = yy >= 0 && yy < Ny && z + dz >= 0 && z + dz < J.N_z && xi + dx >= 0 &&
-                        xi + dx < J.N_x;
-        float nbv = 0.f;
-        if (ex) nbv = l == 4 ? sm.x[yy] : sm.nbl[l][yy + 1];
-        lo = fminf(lo, nbv);
-        hi = fmaxf(hi, nbv);
-        if (ex) {
-          const float b = filter_weight(dy, dz, dx) * qggmrf_b(v - nbv, J);
-          s1 += b * nbv;
+      float nbv[NS];
+      bool ex[NS];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const int yy = y + ndy[q];
+        ex[q] = nok[q] && yy >= 0 && yy < Ny;
+        const float t = nsrc[q][yy < 0 ? 0 : (yy < Ny ? yy : Ny - 1)];
+        nbv[q] = ex[q] ? t : 0.f;
+      }
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        lo = fminf(lo, nbv[q]);
+        hi = fmaxf(hi, nbv[q]);
+        if (ex[q]) {
+          const float b = nw[q] * qggmrf_b(v - nbv[q], J);
+          s1 += b * nbv[q];
           s2 += b;
         }
       }
