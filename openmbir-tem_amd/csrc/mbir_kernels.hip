@@ -769,13 +769,17 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
     nw[q] = filter_weight(dy, dz, dx);
     nsrc[q] = l == 4 ? sm.x : &sm.nbl[l < 9 ? l : 0][1];
   }
-  float wsu = 0.f, wsv = 0.f, wmg = 0.f;  // this wave's statistics (uniform)
-  int wcnt = 0;
+  // this lane's statistics over both phases (group leaders only): |dx| and |x| in
+  // the ROI, |dx| everywhere, and the counts zero-skipped + 2^10 updated + 2^20
+  // detector rows of the updated slices
+  float lsu = 0.f, lsv = 0.f, lmg = 0.f;
+  int lcnt = 0;
   for (int phase = 0; phase < 2; ++phase) {
     const int vpp = (Ny + 1 - phase) / 2;  // slices of this parity
     const int y = 2 * vi + phase;
     const bool active = vi < vpp;
     const float v = active ? sm.x[y] : 0.f;
+    const double T1 = active ? sm.T1[y] : 0.0, T2 = active ? sm.T2[y] : 0.0;
     float s1 = 0.f, s2 = 0.f, lo = 0.f, hi = 0.f;  // lo/hi include the zeroed centre
     if (active) {
       float nbv[NS];
@@ -808,7 +812,6 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       if (zs_active && v == 0.f && !anypos) {
         // zero-skipped (:311-338)
       } else {
-        const double T1 = sm.T1[y], T2 = sm.T2[y];
         double dlo = lo, dhi = hi;
         if (T2 != 0.0) {
           const double c = (double)v - T1 / T2;
@@ -833,19 +836,21 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       sm.x[y] = xnew;
       sm.dx[y] = xnew - v;
     }
-    // this wave's share of the sweep statistics (:412-421): the group leaders' values
-    // summed in a fixed order, counts by ballots
-    wsu += wave_sum(su);
-    wsv += wave_sum(sv);
-    if (J.magmap) wmg += wave_sum(mg);
-    const int nr = NV == 1 ? (int)__popcll(__ballot(vst == 1)) : wave_sum(nrow);
-    wcnt += (int)__popcll(__ballot(vst == 2)) + ((int)__popcll(__ballot(vst == 1)) << 10) +
-            (nr << 20);
-    if (phase == 1 && lane == 0) {
-      sm.wsu[wave] = wsu;
-      sm.wsv[wave] = wsv;
-      sm.wmg[wave] = wmg;
-      sm.wcnt[wave] = wcnt;
+    lsu += su;
+    lsv += sv;
+    lmg += mg;
+    lcnt += (vst == 2 ? 1 : 0) + (vst == 1 ? (1 << 10) + ((NV == 1 ? 1 : nrow) << 20) : 0);
+    if (phase == 1) {
+      // this wave's share of the sweep statistics (:412-421), summed in a fixed order
+      const float wsu = wave_sum(lsu), wsv = wave_sum(lsv);
+      const float wmg = J.magmap ? wave_sum(lmg) : 0.f;
+      const int wcnt = wave_sum(lcnt);
+      if (lane == 0) {
+        sm.wsu[wave] = wsu;
+        sm.wsv[wave] = wsv;
+        sm.wmg[wave] = wmg;
+        sm.wcnt[wave] = wcnt;
+      }
     }
     __syncthreads();  // F, G
   }
