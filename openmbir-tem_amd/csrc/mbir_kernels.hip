@@ -178,6 +178,21 @@ __device__ inline float wave_sum(float v)
   v += __int_as_float(dpp_upd<0x143, 0xC>(__float_as_int(v)));
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
+// Sum of lanes 0-7 (every other lane 0) by three DPP prefix steps, in lane 7.
+__device__ inline float sum8(float v)
+{
+  v += __int_as_float(dpp_upd<0x111, 0xF>(__float_as_int(v)));
+  v += __int_as_float(dpp_upd<0x112, 0xF>(__float_as_int(v)));
+  v += __int_as_float(dpp_upd<0x114, 0xF>(__float_as_int(v)));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 7));
+}
+__device__ inline int sum8(int v)
+{
+  v += dpp_upd<0x111, 0xF>(v);
+  v += dpp_upd<0x112, 0xF>(v);
+  v += dpp_upd<0x114, 0xF>(v);
+  return __builtin_amdgcn_readlane(v, 7);
+}
 template <int CTRL, int ROWS>
 __device__ inline double dpp_upd(double v)
 {
@@ -861,9 +876,9 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   // where the device is full ----
   auto line_stats = [&]() {
     if (wave == 0) {
-      const float su = wave_sum(lane < kIcdWaves ? sm.wsu[lane] : 0.f);
-      const float sv = wave_sum(lane < kIcdWaves ? sm.wsv[lane] : 0.f);
-      const int cn = wave_sum(lane < kIcdWaves ? sm.wcnt[lane] : 0);
+      const float su = sum8(lane < kIcdWaves ? sm.wsu[lane] : 0.f);
+      const float sv = sum8(lane < kIcdWaves ? sm.wsv[lane] : 0.f);
+      const int cn = sum8(lane < kIcdWaves ? sm.wcnt[lane] : 0);
       const int nz = cn & 1023, nu = (cn >> 10) & 1023, nn = nc * (cn >> 20);
       if (lane == 0 && su != 0.f) atomic_add_g(&acc->sum_abs_update, (double)su);
       if (lane == 0 && sv != 0.f) atomic_add_g(&acc->sum_abs_value, (double)sv);
@@ -871,7 +886,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       if (lane == 0 && nu) atomic_add_g(&acc->updated, (unsigned long long)nu);
       if (lane == 0 && nn) atomic_add_g(&acc->nnz_updated, (unsigned long long)nn);
       if (J.magmap) {  // the line's update magnitude (BFUpdateYSlice.cpp:171, 303-304)
-        const float mgl = wave_sum(lane < kIcdWaves ? sm.wmg[lane] : 0.f);
+        const float mgl = sum8(lane < kIcdWaves ? sm.wmg[lane] : 0.f);
         if (lane == 0) gp(J.magmap)[line] = mgl;
       }
     }
