@@ -273,6 +273,10 @@ typedef struct mbir_server_opts {
   int32_t setup_threads;     /* host threads that set admitted jobs up (0 = 2, at most 8) */
   uint64_t hbm_budget_bytes; /* device bytes for resident jobs (0 = the context's budget,
                                 or 80% of the free device memory) */
+  int32_t long_steps;        /* jobs with at least this many conflict-free steps per sweep
+                                (N_z x spacing) run on a second launch stream, beside the
+                                others (0 = 2048, < 0 = one stream for all) */
+  int32_t long_jobs;         /* jobs on the second stream at once (0 = 4) */
 } mbir_server_opts;
 
 #define MBIR_JOB_QUEUED 0
@@ -298,7 +302,8 @@ typedef struct mbir_server_stats {
   int64_t ticks, launches, entries, workgroups;
   int32_t jobs_submitted, jobs_done, jobs_failed, peak_resident;
   uint64_t budget_bytes, reserved_bytes, peak_reserved_bytes;
-  int32_t fatal_error, pad;
+  int32_t fatal_error;
+  int32_t long_lane_jobs;   /* jobs that ran on the second launch stream */
 } mbir_server_stats;
 
 /* Upper bound of a job's device footprint, used for admission. */

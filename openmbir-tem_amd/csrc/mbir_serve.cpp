@@ -10,6 +10,13 @@
 // same operations in the same order as mbir_job_run, so its result does not
 // depend on what else is running.
 //
+// Two launch lanes: jobs whose sweeps have many steps (at least long_steps
+// conflict-free steps, N_z S) run their ticks on a stream of their own, so that a
+// long job's step is not held behind a full launch of the others: the two lanes'
+// launches run side by side on the device, and the long job, whose chain of steps
+// bounds its latency, keeps close to its own pace.  The lane changes only where a
+// job's launches go, not what they compute.
+//
 // Threads: setup workers admit queued jobs in arrival order (FIFO, no overtaking)
 // when the job fits the budget and a slot is free, and run the job's setup
 // (mbir_job_create) on the job's own stream; one tick thread owns the launch
@@ -32,6 +39,9 @@ namespace {
 constexpr int kMaxServerJobs = 256;
 constexpr int kTickEvery = 4;   // ticks per run-ahead event
 constexpr int kTickEvents = 4;  // events in flight: the host runs at most 16 ticks ahead
+constexpr int kLanes = 2;       // 0: every job; 1: jobs with long sweeps
+constexpr int kLongStepsDefault = 2048;
+constexpr int kLongJobsDefault = 4;
 
 struct Req {
   uint64_t ticket = 0;
@@ -49,6 +59,7 @@ struct Req {
   bool waiting = false;    // the sweep's last step is launched; accumulators in flight
   bool finishing = false;  // final nuisance update in flight
   int ypl = 1, nv = 1;
+  int lane = 0;
   SweepAccum* hacc = nullptr;  // pinned
   LineRef* hsched = nullptr;   // pinned copy of the current sweep's schedule
   hipEvent_t ev = nullptr;
@@ -83,9 +94,10 @@ struct mbir_server {
   mbir_ctx* ctx = nullptr;
   mbir::LineTuning tune;  // the context's kernel-form choice when the server was created
   int max_jobs = 64;
+  int long_steps = kLongStepsDefault, long_jobs = kLongJobsDefault;
   uint64_t budget = 0, reserved = 0, peak_reserved = 0;
   int resident = 0;  // jobs holding a slot (in setup, running or finished but unreleased)
-  hipStream_t stream = nullptr;
+  hipStream_t lane_stream[kLanes] = {nullptr, nullptr};
   DevBuf<DevJob> djobs;
   std::vector<int> free_slots;
   std::mutex mu;
@@ -107,6 +119,7 @@ struct mbir_server {
   {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
+  hipStream_t stream_of(const Req* r) const { return lane_stream[r->lane]; }
   void setup_loop();
   void run_loop();
   int start_sweep(Req* r);
@@ -203,6 +216,7 @@ int mbir_server::start_sweep(Req* r)
   build_schedule(j, true);
   if (!j->sched_h.empty())
     std::memcpy(r->hsched, j->sched_h.data(), j->sched_h.size() * sizeof(LineRef));
+  hipStream_t stream = stream_of(r);
   int rc = prepare_sweep_on(j, stream, r->hsched);
   if (rc < 0) return rc;
   r->step = 0;
@@ -222,6 +236,7 @@ int mbir_server::start_sweep(Req* r)
 int mbir_server::end_sweep(Req* r)
 {
   mbir_job* j = r->job;
+  hipStream_t stream = stream_of(r);
   mbir_sweep_stats ss{};
   const int status = sweep_status(j, *r->hacc, 0.0, &ss);
   r->waiting = false;
@@ -259,15 +274,27 @@ void mbir_server::run_loop()
 {
   (void)hipSetDevice(ctx->device);
   std::vector<Req*> act;
-  std::vector<hipEvent_t> tick_ev(kTickEvents, nullptr);
-  for (auto& e : tick_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+  // per lane: run-ahead events (one every kTickEvery ticks) and the tick count
+  hipEvent_t tick_ev[kLanes][kTickEvents];
+  int64_t lane_ticks[kLanes] = {0, 0};
+  for (auto& l : tick_ev)
+    for (auto& e : l) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
   int64_t tick = 0, n_launch = 0, n_entries = 0, n_wg = 0;
+  int long_resident = 0;
   TickArgs args;
   auto fail_all = [&](int code) {
     std::lock_guard<std::mutex> lk(mu);
     fatal = code;
     for (Req* r : act) finish(r, MBIR_JOB_FAILED, code);
     act.clear();
+    long_resident = 0;
+  };
+  // whether lane l may launch another tick: the host stays at most
+  // kTickEvery * kTickEvents ticks ahead of the device on each lane
+  auto lane_open = [&](int l) {
+    const int64_t t = lane_ticks[l];
+    if (t % kTickEvery != 0 || t / kTickEvery < kTickEvents) return true;
+    return hipEventQuery(tick_ev[l][(t / kTickEvery) % kTickEvents]) != hipErrorNotReady;
   };
   for (;;) {
     {
@@ -284,6 +311,11 @@ void mbir_server::run_loop()
         r->t_start = now();
         r->num_iter = r->job->prob.num_inner;
         icd_shape(r->job->J, r->job->J.vlr_max, &r->ypl, &r->nv);
+        // the lane: jobs with long sweeps (conflict-free steps per sweep) go to lane 1
+        const int64_t spp = (int64_t)r->job->g.N_z * r->job->spacing;
+        r->lane = (long_steps > 0 && spp >= long_steps && long_resident < long_jobs) ? 1 : 0;
+        long_resident += r->lane;
+        st.long_lane_jobs += r->lane;
         act.push_back(r);
       }
       st.peak_resident = std::max<int32_t>(st.peak_resident, (int32_t)act.size());
@@ -292,7 +324,7 @@ void mbir_server::run_loop()
       lk.unlock();
       bool bad = false;
       for (Req* r : fresh) {
-        if (launch_set_devjob(djobs.p, r->slot, tick_desc(r->job), stream) != hipSuccess ||
+        if (launch_set_devjob(djobs.p, r->slot, tick_desc(r->job), stream_of(r)) != hipSuccess ||
             start_sweep(r) < 0)
           bad = true;
       }
@@ -313,6 +345,7 @@ void mbir_server::run_loop()
       }
       if (r->finishing) {
         r->finishing = false;
+        long_resident -= r->lane;
         std::lock_guard<std::mutex> lk(mu);
         finish(r, MBIR_JOB_DONE, 0);
         continue;
@@ -321,6 +354,7 @@ void mbir_server::run_loop()
         bad = true;
         break;
       }
+      if (r->state == MBIR_JOB_FAILED) long_resident -= r->lane;
     }
     if (bad) {
       fail_all(MBIR_ERR_HIP);
@@ -331,83 +365,99 @@ void mbir_server::run_loop()
                                return r->state == MBIR_JOB_DONE || r->state == MBIR_JOB_FAILED;
                              }),
               act.end());
-    // one tick: the next step of every job that has one, one launch per kernel shape
-    bool launched = false;
-    for (int shape = 0; shape < 4 && !bad; ++shape) {
-      const int ypl = 1 + (shape >> 1), nv = 1 + (shape & 1);
-      args.n = 0;
-      args.slots = 4;
-      int wg = 0;
-      std::vector<Req*> last;  // jobs whose sweep ends with this launch
-      auto flush = [&]() -> bool {
-        if (args.n == 0) return true;
-        if (launch_icd_tick(ypl, nv, djobs.p, args, wg, tune.choose(ypl, nv, wg), stream) !=
-            hipSuccess)
-          return false;
-        n_launch++;
-        n_entries += args.n;
-        n_wg += wg;
-        launched = true;
-        for (Req* r : last) {
-          if (hipMemcpyAsync(r->hacc, r->job->acc.p, sizeof(SweepAccum), hipMemcpyDeviceToHost,
-                             stream) != hipSuccess ||
-              hipEventRecord(r->ev, stream) != hipSuccess)
-            return false;
-          r->waiting = true;
-        }
-        last.clear();
+    // one tick per lane that may run ahead: the next step of every job of the lane
+    // that has one, one launch per kernel shape
+    bool launched_any = false, held = false;
+    for (int lane = 0; lane < kLanes && !bad; ++lane) {
+      bool has = false;
+      for (Req* r : act)
+        if (r->lane == lane && !r->waiting && !r->finishing && r->step < r->nsteps) has = true;
+      if (!has) continue;
+      if (!lane_open(lane)) {
+        held = true;
+        continue;
+      }
+      hipStream_t stream = lane_stream[lane];
+      bool launched = false;
+      for (int shape = 0; shape < 4 && !bad; ++shape) {
+        const int ypl = 1 + (shape >> 1), nv = 1 + (shape & 1);
         args.n = 0;
         args.slots = 4;
-        wg = 0;
-        return true;
-      };
-      for (Req* r : act) {
-        if (r->ypl != ypl || r->nv != nv || r->waiting || r->finishing || r->step >= r->nsteps)
-          continue;
-        mbir_job* j = r->job;
-        TickEntry& E = args.e[args.n++];
-        args.slots = std::max(args.slots, j->J.col_slots);
-        E.job = r->slot;
-        E.line0 = j->step_ptr[r->step];
-        E.nlines = j->step_ptr[r->step + 1] - E.line0;
-        E.wg_off = wg;
-        E.flags = ((r->outer == 0 && r->inner == 0) ? 1 : 0) | (j->prob.zero_skipping ? 2 : 0);
-        wg += E.nlines;
-        if (++r->step == r->nsteps) last.push_back(r);
-        if (args.n == kTickMax && !flush()) bad = true;
+        int wg = 0;
+        std::vector<Req*> last;  // jobs whose sweep ends with this launch
+        auto flush = [&]() -> bool {
+          if (args.n == 0) return true;
+          if (launch_icd_tick(ypl, nv, djobs.p, args, wg, tune.choose(ypl, nv, wg), stream) !=
+              hipSuccess)
+            return false;
+          n_launch++;
+          n_entries += args.n;
+          n_wg += wg;
+          launched = true;
+          for (Req* r : last) {
+            if (hipMemcpyAsync(r->hacc, r->job->acc.p, sizeof(SweepAccum), hipMemcpyDeviceToHost,
+                               stream) != hipSuccess ||
+                hipEventRecord(r->ev, stream) != hipSuccess)
+              return false;
+            r->waiting = true;
+          }
+          last.clear();
+          args.n = 0;
+          args.slots = 4;
+          wg = 0;
+          return true;
+        };
+        for (Req* r : act) {
+          if (r->lane != lane || r->ypl != ypl || r->nv != nv || r->waiting || r->finishing ||
+              r->step >= r->nsteps)
+            continue;
+          mbir_job* j = r->job;
+          TickEntry& E = args.e[args.n++];
+          args.slots = std::max(args.slots, j->J.col_slots);
+          E.job = r->slot;
+          E.line0 = j->step_ptr[r->step];
+          E.nlines = j->step_ptr[r->step + 1] - E.line0;
+          E.wg_off = wg;
+          E.flags = ((r->outer == 0 && r->inner == 0) ? 1 : 0) | (j->prob.zero_skipping ? 2 : 0);
+          wg += E.nlines;
+          if (++r->step == r->nsteps) last.push_back(r);
+          if (args.n == kTickMax && !flush()) bad = true;
+        }
+        if (!bad && !flush()) bad = true;
       }
-      if (!bad && !flush()) bad = true;
+      if (launched) {
+        // the lane's run-ahead event, one every kTickEvery ticks (an event between
+        // two launches delays the second)
+        const int64_t t = lane_ticks[lane];
+        if (t % kTickEvery == 0) (void)hipEventRecord(tick_ev[lane][(t / kTickEvery) % kTickEvents], stream);
+        lane_ticks[lane]++;
+        tick++;
+        launched_any = true;
+      }
     }
     if (bad) {
       fail_all(MBIR_ERR_HIP);
       continue;
     }
-    if (launched) {
-      // bound how far the host runs ahead, so arrivals and sweep ends are seen
-      // promptly: one event every kTickEvery ticks (an event between two launches
-      // delays the second)
-      if (tick % kTickEvery == 0) {
-        const int64_t k = tick / kTickEvery;
-        hipEvent_t& e = tick_ev[k % kTickEvents];
-        if (k >= kTickEvents) (void)hipEventSynchronize(e);
-        (void)hipEventRecord(e, stream);
-      }
-      tick++;
+    if (launched_any) {
       std::lock_guard<std::mutex> lk(mu);
       st.ticks = tick;
       st.launches = n_launch;
       st.entries = n_entries;
       st.workgroups = n_wg;
+    } else if (held) {
+      // every lane with work is at its run-ahead bound: let the device catch up
+      std::this_thread::yield();
     } else {
-      // nothing to launch: wait for the oldest outstanding sweep end
+      // nothing to launch: wait for the oldest outstanding sweep end (of any lane)
+      bool any_wait = false;
       for (Req* r : act)
-        if (r->waiting || r->finishing) {
-          (void)hipEventSynchronize(r->ev);
-          break;
-        }
+        if (r->waiting || r->finishing) any_wait = true;
+      if (any_wait) std::this_thread::yield();
     }
   }
-  for (auto& e : tick_ev) (void)hipEventDestroy(e);
+  for (auto& l : tick_ev)
+    for (auto& e : l) (void)hipEventDestroy(e);
 }
 
 // ---------------------------------------------------------------------------
@@ -430,6 +480,8 @@ extern "C" int32_t mbir_server_create(mbir_ctx* ctx, const mbir_server_opts* o, 
     if (o->max_jobs > 0) s->max_jobs = std::min(o->max_jobs, kMaxServerJobs);
     if (o->setup_threads > 0) threads = std::min(o->setup_threads, 8);
     s->budget = o->hbm_budget_bytes;
+    if (o->long_steps != 0) s->long_steps = o->long_steps;  // < 0: one lane
+    if (o->long_jobs > 0) s->long_jobs = o->long_jobs;
   }
   if (s->budget == 0) s->budget = ctx->budget;
   if (s->budget == 0) {
@@ -440,8 +492,12 @@ extern "C" int32_t mbir_server_create(mbir_ctx* ctx, const mbir_server_opts* o, 
     }
     s->budget = (uint64_t)(0.8 * (double)fr);
   }
-  if (hipStreamCreate(&s->stream) != hipSuccess || s->djobs.alloc(s->max_jobs) != hipSuccess) {
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+  // default-priority streams, one per lane
+  if (hipStreamCreate(&s->lane_stream[0]) != hipSuccess ||
+      hipStreamCreate(&s->lane_stream[1]) != hipSuccess ||
+      s->djobs.alloc(s->max_jobs) != hipSuccess) {
+    for (hipStream_t& q : s->lane_stream)
+      if (q) (void)hipStreamDestroy(q);
     s->djobs.release();
     delete s;
     return MBIR_ERR_HIP;
@@ -631,12 +687,12 @@ extern "C" int32_t mbir_server_destroy(mbir_server* s)
     s->cv_run.notify_all();
   }
   s->run_thr.join();
-  (void)hipStreamSynchronize(s->stream);
+  for (hipStream_t q : s->lane_stream) (void)hipStreamSynchronize(q);
   std::vector<uint64_t> tickets;
   for (auto& kv : s->reqs) tickets.push_back(kv.first);
   for (uint64_t t : tickets) mbir_server_release(s, t);
   s->djobs.release();
-  (void)hipStreamDestroy(s->stream);
+  for (hipStream_t q : s->lane_stream) (void)hipStreamDestroy(q);
   delete s;
   return MBIR_OK;
 }

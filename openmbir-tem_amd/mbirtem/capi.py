@@ -74,7 +74,8 @@ class RunStats(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("max_jobs", C.c_int32), ("setup_threads", C.c_int32),
-                ("hbm_budget_bytes", C.c_uint64)]
+                ("hbm_budget_bytes", C.c_uint64), ("long_steps", C.c_int32),
+                ("long_jobs", C.c_int32)]
 
 
 class JobReport(C.Structure):
@@ -91,7 +92,7 @@ class ServerStats(C.Structure):
                 ("jobs_done", C.c_int32), ("jobs_failed", C.c_int32),
                 ("peak_resident", C.c_int32), ("budget_bytes", C.c_uint64),
                 ("reserved_bytes", C.c_uint64), ("peak_reserved_bytes", C.c_uint64),
-                ("fatal_error", C.c_int32), ("pad", C.c_int32)]
+                ("fatal_error", C.c_int32), ("long_lane_jobs", C.c_int32)]
 
 
 JOB_QUEUED, JOB_SETUP, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(5)
@@ -465,11 +466,12 @@ def problem_device_bytes(geom, angles_deg, counts, **kw):
 class Server:
     """Request server (mbir_server_*): jobs run concurrently with continuous batching."""
 
-    def __init__(self, ctx, max_jobs=0, setup_threads=0, hbm_budget_bytes=0):
+    def __init__(self, ctx, max_jobs=0, setup_threads=0, hbm_budget_bytes=0, long_steps=0,
+                 long_jobs=0):
         self.ctx = ctx
         self.h = C.c_void_p()
         self._geom = {}
-        o = ServerOpts(max_jobs, setup_threads, hbm_budget_bytes)
+        o = ServerOpts(max_jobs, setup_threads, hbm_budget_bytes, long_steps, long_jobs)
         _chk("mbir_server_create", lib().mbir_server_create(ctx.h, C.byref(o), C.byref(self.h)))
 
     def submit(self, geom, angles_deg, counts, **kw):

@@ -82,6 +82,33 @@ def test_server_results_equal_alone(gpu_ctx):
     srv.close()
 
 
+def test_server_two_lanes_results_equal_alone(gpu_ctx):
+    """Jobs on the second launch stream (long sweeps; here the first two admitted,
+    by a threshold of one step) run beside the others and still end bit for bit as
+    alone."""
+    cases = _cases()
+    refs = [_alone(gpu_ctx, c) for c in cases]
+    srv = capi.Server(gpu_ctx, max_jobs=8, long_steps=1, long_jobs=2)
+    tickets = [srv.submit(_geom(g, ang), ang, y, **kw) for (g, ang, y, kw) in cases]
+    for t, ref in zip(tickets, refs):
+        rep = srv.wait(t, timeout=60.0)
+        assert rep is not None, "job did not finish"
+        _check_same(srv.job(t), ref, rep)
+        srv.release(t)
+    st = srv.stats()
+    assert st.jobs_done == len(cases) and st.jobs_failed == 0 and st.fatal_error == 0
+    assert 1 <= st.long_lane_jobs <= len(cases)
+    srv.close()
+    # one stream for all (long_steps < 0): no job on the second
+    srv = capi.Server(gpu_ctx, max_jobs=8, long_steps=-1)
+    t = srv.submit(_geom(cases[0][0], cases[0][1]), cases[0][1], cases[0][2], **cases[0][3])
+    rep = srv.wait(t, timeout=60.0)
+    _check_same(srv.job(t), refs[0], rep)
+    srv.release(t)
+    assert srv.stats().long_lane_jobs == 0
+    srv.close()
+
+
 def test_server_budget_admission(gpu_ctx):
     cases = _cases()[:4]
     need = [capi.problem_device_bytes(_geom(g, ang), ang, y, **kw) for g, ang, y, kw in cases]
