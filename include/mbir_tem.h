@@ -237,6 +237,14 @@ int32_t mbir_job_get_column(mbir_job* job, int32_t z, int32_t x, uint32_t* index
 
 int32_t mbir_job_get_volume(mbir_job* job, float* out);
 int32_t mbir_job_set_volume(mbir_job* job, const float* volume); /* also re-derives e */
+/* The device-resident multi-resolution hand-over: fine's volume becomes coarse's
+ * replicated 2 x 2 x 2 (coarse sizes = fine's N_z/2, N_x/2, N_y/2; the last plane of an
+ * odd size and lines whose column is empty stay 0, as after a setup from the same
+ * volume), then e is re-derived.  Both jobs on one device.  No counterpart in
+ * the reference, which hands levels over through files: it replaces the level file
+ * write (HAADF/HAADF_MultiResolutionReconstruction.cpp:211-229) and its interpolating
+ * read (GenericFilters/InitialReconstructionBinReader.cpp:103-130). */
+int32_t mbir_job_upsample_volume(mbir_job* fine, const mbir_job* coarse);
 int32_t mbir_job_get_error_sinogram(mbir_job* job, float* out);
 int32_t mbir_job_get_nuisance(mbir_job* job, double* I0, double* mu, double* alpha);
 int32_t mbir_job_get_geometry(mbir_job* job, mbir_geometry* out);

@@ -1386,6 +1386,32 @@ extern "C" int32_t mbir_job_set_volume(mbir_job* j, const float* vol)
   return MBIR_OK;
 }
 
+// The device-resident form of the multi-resolution hand-over: the coarse level's
+// volume replicated 2 x 2 x 2 into this job's volume as the reference's reader does
+// with the level file (HAADF/HAADF_MultiResolutionReconstruction.cpp:211-229,
+// GenericFilters/InitialReconstructionBinReader.cpp:103-130), then e re-derived.
+extern "C" int32_t mbir_job_upsample_volume(mbir_job* fine, const mbir_job* coarse)
+{
+  if (!fine || !coarse || fine == coarse) return MBIR_ERR_INPUT;
+  if (fine->ctx->device != coarse->ctx->device) return MBIR_ERR_INPUT;
+  const mbir_geometry& f = fine->g;
+  const mbir_geometry& c = coarse->g;
+  if (c.N_z != f.N_z / 2 || c.N_x != f.N_x / 2 || c.N_y != f.N_y / 2 || c.N_z < 1 || c.N_x < 1 ||
+      c.N_y < 1)
+    return MBIR_ERR_INPUT;
+  const size_t M = (size_t)f.N_theta * f.N_r * f.N_t;
+  hipStream_t s = fine->stream;
+  HIP_TRY(hipStreamSynchronize(coarse->stream));  // the coarse volume is final
+  HIP_TRY(launch_upsample2(coarse->x.p, fine->x.p, fine->line_nnz.p, f.N_z, f.N_x, f.N_y, s));
+  HIP_TRY(fine->yest.alloc(M));
+  HIP_TRY(launch_fill_f64(fine->yest.p, 0.0, M, s));
+  HIP_TRY(launch_forward_project(fine->J, fine->x.p, fine->yest.p, s));
+  HIP_TRY(launch_init_error(fine->J, fine->yest.p, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  fine->yest.release();
+  return MBIR_OK;
+}
+
 extern "C" int32_t mbir_job_get_error_sinogram(mbir_job* j, float* out)
 {
   if (!j || !out) return MBIR_ERR_INPUT;

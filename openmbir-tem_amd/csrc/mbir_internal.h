@@ -142,6 +142,10 @@ struct LineTuning {
 hipError_t launch_line_nnz(const DevJob& J, int* line_nnz, float* x, hipStream_t s);
 hipError_t launch_forward_project(const DevJob& J, const float* vol, double* out, hipStream_t s);
 hipError_t launch_init_error(const DevJob& J, const double* yest, hipStream_t s);
+// fine volume (sizes N_z, N_x, N_y) from the coarse one of sizes N/2, replicated 2 x 2 x 2;
+// lines with line_nnz 0 stay 0
+hipError_t launch_upsample2(const float* coarse, float* fine, const int* line_nnz, int Nz, int Nx,
+                            int Ny, hipStream_t s);
 hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, int outer, int inner,
                            int zero_skipping, SweepAccum* acc, hipStream_t s);
 void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv);

@@ -1417,6 +1417,36 @@ hipError_t launch_forward_project(const DevJob& J, const float* vol, double* out
   return hipGetLastError();
 }
 
+// fine[z][x][y] = coarse[z/2][x/2][y/2] for z < 2 (N_z/2), x < 2 (N_x/2), y < 2 (N_y/2),
+// 0 elsewhere (the last plane of an odd size); sizes are the fine volume's
+// (GenericFilters/InitialReconstructionBinReader.cpp:103-130).  Lines whose column is
+// empty stay 0, as the setup leaves them (k_line_nnz).
+__global__ __launch_bounds__(kBlock) void k_upsample2(const float* __restrict__ coarse,
+                                                      float* __restrict__ fine,
+                                                      const int* __restrict__ line_nnz, int Nz,
+                                                      int Nx, int Ny)
+{
+  const int ze = Nz / 2, xe = Nx / 2, ye = Ny / 2;
+  const size_t V = (size_t)Nz * Nx * Ny;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < V;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int y = (int)(i % Ny), x = (int)((i / Ny) % Nx), z = (int)(i / ((size_t)Ny * Nx));
+    float v = 0.f;
+    if (z < 2 * ze && x < 2 * xe && y < 2 * ye && line_nnz[(size_t)z * Nx + x] > 0)
+      v = coarse[((size_t)(z >> 1) * xe + (x >> 1)) * ye + (y >> 1)];
+    fine[i] = v;
+  }
+}
+
+hipError_t launch_upsample2(const float* coarse, float* fine, const int* line_nnz, int Nz, int Nx,
+                            int Ny, hipStream_t s)
+{
+  const size_t V = (size_t)Nz * Nx * Ny;
+  hipLaunchKernelGGL(k_upsample2, dim3(grid_for(V)), dim3(kBlock), 0, s, coarse, fine, line_nnz,
+                     Nz, Nx, Ny);
+  return hipGetLastError();
+}
+
 hipError_t launch_init_error(const DevJob& J, const double* yest, hipStream_t s)
 {
   const size_t M = (size_t)J.N_theta * J.N_r * J.N_t;

@@ -34,8 +34,22 @@ void HAADF_MultiResolutionReconstruction::execute()
   std::vector<std::string> temps;
   TomoInputs prev;
   const int L = NumberResolutions;
+  // device-resident hand-over: the previous level's finished job (volume on the
+  // device) and its nuisance parameters instead of the level files
+  const bool dev = DeviceHandover && L > 1;
+  mbir_job* prevJob = nullptr;
+  mbir_ctx* prevCtx = nullptr;
+  std::vector<Real_t> prevI0, prevMu, prevAlpha;
+  auto dropPrev = [&]() {
+    if (prevJob) mbir_job_destroy(prevJob);
+    if (prevCtx) mbir_destroy(prevCtx);
+    prevJob = nullptr;
+    prevCtx = nullptr;
+  };
   for (int i = 0; i < L; ++i) {
+    const bool last = i == L - 1;
     if (m_Cancel.load()) {
+      dropPrev();
       m_ErrorCondition = -999;
       return;
     }
@@ -59,21 +73,32 @@ void HAADF_MultiResolutionReconstruction::execute()
       else d << f;
       T.tempDir = TempDir + "/" + d.str() + "x";
     }
-    std::error_code ec;
-    fs::create_directories(T.tempDir, ec);
-    if (ec) {
-      m_ErrorCondition = -1;
-      m_ErrorMessage = "Could not create path: " + T.tempDir;
-      return;
+    if (dev && i > 0) {  // no level files: the coarse job hands over on the device
+      T.gainsInputFile.clear();
+      T.offsetsInputFile.clear();
+      T.varianceInputFile.clear();
+      T.initialReconFile.clear();
     }
-    if (i == L - 1) T.mrcOutputFile = OutputFile;  // only the last level writes the volume
-    T.gainsOutputFile = T.tempDir + "/FinalGainParameters.bin";
-    T.offsetsOutputFile = T.tempDir + "/FinalOffsetParameters.bin";
-    T.varianceOutputFile = T.tempDir + "/FinalVariances.bin";
-    for (const char* f : {"/FinalGainParameters.bin", "/FinalOffsetParameters.bin",
-                          "/FinalVariances.bin", "/ReconstructedSinogram.bin",
-                          "/ReconstructedObject.bin"})
-      temps.push_back(T.tempDir + f);
+    if (dev && !last) {
+      T.tempDir.clear();  // intermediate levels write nothing
+    } else {
+      std::error_code ec;
+      fs::create_directories(T.tempDir, ec);
+      if (ec) {
+        dropPrev();
+        m_ErrorCondition = -1;
+        m_ErrorMessage = "Could not create path: " + T.tempDir;
+        return;
+      }
+      if (last) T.mrcOutputFile = OutputFile;  // only the last level writes the volume
+      T.gainsOutputFile = T.tempDir + "/FinalGainParameters.bin";
+      T.offsetsOutputFile = T.tempDir + "/FinalOffsetParameters.bin";
+      T.varianceOutputFile = T.tempDir + "/FinalVariances.bin";
+      for (const char* f : {"/FinalGainParameters.bin", "/FinalOffsetParameters.bin",
+                            "/FinalVariances.bin", "/ReconstructedSinogram.bin",
+                            "/ReconstructedObject.bin"})
+        temps.push_back(T.tempDir + f);
+    }
     // iterations, prior and stop rule per level (:325-349)
     T.NumOuterIter = (uint16_t)OuterIterations;
     T.NumIter = (uint16_t)(i == 0 ? InnerIterations : 1);
@@ -117,10 +142,30 @@ void HAADF_MultiResolutionReconstruction::execute()
     eng->setGeometry(geom);
     eng->setAdvParams(AdvParams);
     eng->setForwardModel(fm);
+    if (dev && i > 0) {
+      // what the level files would carry: the nuisance files exist only for the
+      // estimated parameters (gains and offsets with joint estimation, variances
+      // with noise estimation), and an offsets file turns the default offset off
+      if (AdvParams->JOINT_ESTIMATION) {
+        fm->InitialGain = prevI0;
+        fm->InitialOffset = prevMu;
+        T.useDefaultOffset = false;
+      }
+      if (AdvParams->NOISE_ESTIMATION) fm->InitialVariance = prevAlpha;
+      eng->setInitialJob(prevJob);
+    }
+    if (dev && !last) eng->setKeepJob(true);
     m_Current = eng;
     if (m_Cancel.load()) eng->setCancel(true);
     eng->execute();
     m_Current.reset();
+    dropPrev();  // the level's job has taken the coarse volume (or failed)
+    if (dev && !last && eng->getErrorCondition() >= 0) {
+      prevJob = eng->takeJob(&prevCtx);
+      prevI0 = fm->I_0;
+      prevMu = fm->Mu;
+      prevAlpha = fm->Alpha;
+    }
     m_LevelInputs.push_back(T);
     Geometry gcopy = *geom;
     gcopy.Object.clear();
@@ -132,8 +177,9 @@ void HAADF_MultiResolutionReconstruction::execute()
       return;
     }
     prev = T;
-    temps.push_back(T.tempDir);  // directories last: removed once empty
+    if (!T.tempDir.empty()) temps.push_back(T.tempDir);  // directories last: removed once empty
   }
+  dropPrev();
   if (DeleteTempFiles) {
     for (const std::string& p : temps) {
       std::error_code ec;

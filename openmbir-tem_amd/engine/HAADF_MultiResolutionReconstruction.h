@@ -4,7 +4,8 @@
 // the final resolution, starts from the previous level's volume replicated 2 x 2 x 2
 // and its gains, offsets and variances, and only the last level writes the MRC
 // volume.  Levels hand over through the files the engine writes into one temporary
-// directory per level ("4x", "2x", "1x" ...), as the reference's do.
+// directory per level ("4x", "2x", "1x" ...), as the reference's do, or on the device
+// (DeviceHandover).
 #pragma once
 #include <atomic>
 #include <memory>
@@ -47,6 +48,11 @@ class HAADF_MultiResolutionReconstruction {
   std::vector<uint8_t> ViewMasks;        // excluded file sections
   AdvancedParametersPtr AdvParams;
   uint32_t Seed = 12345;
+  // Hand the levels over on the device (the finished coarse job's volume replicated
+  // 2 x 2 x 2 into the next level's job, the nuisance parameters in memory) instead
+  // of through the level files; intermediate levels then write no files.  Not in the
+  // reference; off by default.
+  bool DeviceHandover = false;
 
   // Runs every level; 0 on success, else the failing level's error code.
   void execute();

@@ -14,7 +14,11 @@
 
 namespace mbirtem {
 
-HAADF_ReconstructionEngine::~HAADF_ReconstructionEngine() = default;
+HAADF_ReconstructionEngine::~HAADF_ReconstructionEngine()
+{
+  if (m_KeptJob) mbir_job_destroy(m_KeptJob);  // kept for a next level that never came
+  if (m_KeptCtx) mbir_destroy(m_KeptCtx);
+}
 
 void HAADF_ReconstructionEngine::fail(int code, const std::string& msg)
 {
@@ -131,6 +135,15 @@ void HAADF_ReconstructionEngine::writeOutputs(mbir_job* job, const std::vector<f
   }
 }
 
+mbir_job* HAADF_ReconstructionEngine::takeJob(mbir_ctx** ctx)
+{
+  mbir_job* j = m_KeptJob;
+  if (ctx) *ctx = m_KeptCtx;
+  m_KeptJob = nullptr;
+  m_KeptCtx = nullptr;
+  return j;
+}
+
 void HAADF_ReconstructionEngine::execute()
 {
   m_ErrorCondition = 0;
@@ -244,6 +257,11 @@ void HAADF_ReconstructionEngine::execute()
     mbir_destroy(ctx);
     return fail(rc, rc == -2 ? "Error Calling function detectorResponse" : "job setup failed");
   }
+  if (m_InitialJob && (rc = mbir_job_upsample_volume(job, m_InitialJob)) < 0) {
+    mbir_job_destroy(job);
+    mbir_destroy(ctx);
+    return fail(-1, "Error reading Initial Reconstruction Data from the coarse level");
+  }
   {
     std::lock_guard<std::mutex> lk(m_JobMutex);
     m_Job = job;
@@ -274,8 +292,13 @@ void HAADF_ReconstructionEngine::execute()
     std::lock_guard<std::mutex> lk(m_JobMutex);
     m_Job = nullptr;
   }
-  mbir_job_destroy(job);
-  mbir_destroy(ctx);
+  if (m_KeepJob && rc == 0 && m_ErrorCondition == 0) {
+    m_KeptJob = job;
+    m_KeptCtx = ctx;
+  } else {
+    mbir_job_destroy(job);
+    mbir_destroy(ctx);
+  }
   if (rc == -999) return fail(-999, "cancelled");
   if (rc < 0) return fail(rc, "reconstruction failed");
 }

@@ -134,6 +134,16 @@ class HAADF_ReconstructionEngine {
   int getSweeps() const { return m_Sweeps; }
   const std::vector<Real_t>& getCosts() const { return m_Costs; }
 
+  // Device-resident multi-resolution hand-over (no counterpart in the reference,
+  // whose levels hand over through files): start from a finished coarse job's volume
+  // replicated 2 x 2 x 2 on the device (mbir_job_upsample_volume) instead of
+  // TomoInputs::initialReconFile, and keep this run's job for the next level.
+  void setInitialJob(mbir_job* coarse) { m_InitialJob = coarse; }
+  void setKeepJob(bool v) { m_KeepJob = v; }
+  // The kept job and its context (after a successful execute() with setKeepJob(true));
+  // the caller destroys both (mbir_job_destroy, then mbir_destroy).
+  mbir_job* takeJob(mbir_ctx** ctx);
+
   static void InitializeTomoInputs(TomoInputsPtr v) { *v = TomoInputs(); }
   static void InitializeAdvancedParams(AdvancedParametersPtr v) { *v = AdvancedParameters(); }
 
@@ -157,6 +167,10 @@ class HAADF_ReconstructionEngine {
   std::atomic<bool> m_Cancel{false};
   std::mutex m_JobMutex;
   mbir_job* m_Job = nullptr;
+  mbir_job* m_InitialJob = nullptr;
+  bool m_KeepJob = false;
+  mbir_job* m_KeptJob = nullptr;
+  mbir_ctx* m_KeptCtx = nullptr;
   int m_Sweeps = 0;
   std::vector<Real_t> m_Costs;
 };

@@ -14,8 +14,9 @@
 // run.txt as above.
 //
 //   engine_cli --multires <tilts.mrc> levels thickness sigma_x diffuseness outer inner
-//              stop seed <temp_dir> <out.mrc>
-// runs HAADF_MultiResolutionReconstruction (coarse to fine, levels >= 1) and writes
+//              stop seed <temp_dir> <out.mrc> [device_handover 0|1]
+// runs HAADF_MultiResolutionReconstruction (coarse to fine, levels >= 1; the levels
+// hand over through files, or on the device with device_handover 1) and writes
 // <temp_dir>/levels.txt: one line per level with its voxel size multiple, sigma_x, q,
 // stop threshold, inner iterations, interpolation flag, geometry and sweeps.
 #include <cstdio>
@@ -88,9 +89,9 @@ static int run_mrc(int argc, char** argv)
 
 static int run_multires(int argc, char** argv)
 {
-  if (argc != 13) {
+  if (argc != 13 && argc != 14) {
     std::fprintf(stderr, "usage: %s --multires tilts.mrc levels thickness sigma_x diffuseness "
-                         "outer inner stop seed temp_dir out.mrc\n", argv[0]);
+                         "outer inner stop seed temp_dir out.mrc [device_handover 0|1]\n", argv[0]);
     return 2;
   }
   auto mr = HAADF_MultiResolutionReconstruction::New();
@@ -105,6 +106,7 @@ static int run_multires(int argc, char** argv)
   mr->Seed = (uint32_t)std::atoi(argv[10]);
   mr->TempDir = argv[11];
   mr->OutputFile = argv[12];
+  mr->DeviceHandover = argc == 14 && std::atoi(argv[13]) != 0;
   mr->execute();
   std::ofstream txt(std::string(argv[11]) + "/levels.txt");
   txt << std::setprecision(17);

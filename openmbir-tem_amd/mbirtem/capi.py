@@ -175,6 +175,7 @@ _SIG = {
     "mbir_job_get_column": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _U, _F, C.c_int32]),
     "mbir_job_get_volume": (C.c_int32, [C.c_void_p, _F]),
     "mbir_job_set_volume": (C.c_int32, [C.c_void_p, _F]),
+    "mbir_job_upsample_volume": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "mbir_job_get_error_sinogram": (C.c_int32, [C.c_void_p, _F]),
     "mbir_job_get_nuisance": (C.c_int32, [C.c_void_p, _D, _D, _D]),
     "mbir_job_get_geometry": (C.c_int32, [C.c_void_p, C.POINTER(Geometry)]),
@@ -418,6 +419,10 @@ class Job:
     def set_volume(self, vol):
         v = np.ascontiguousarray(vol, np.float32)
         _chk("mbir_job_set_volume", lib().mbir_job_set_volume(self.h, _fptr(v)))
+
+    def upsample_from(self, coarse):
+        """This job's volume from a coarse job's, replicated 2 x 2 x 2 on the device."""
+        _chk("mbir_job_upsample_volume", lib().mbir_job_upsample_volume(self.h, coarse.h))
 
     def error_sinogram(self):
         out = np.zeros(self.sshape, np.float32)

@@ -162,6 +162,63 @@ def test_multiresolution_driver(gpu_ctx, tmp_path):
     assert rel < 0.35, rel
 
 
+def test_upsample_volume_on_device(gpu_ctx):
+    """mbir_job_upsample_volume gives the fine job exactly the state that a setup
+    from the host-replicated coarse volume gives (volume, with lines of empty columns
+    at zero, and error sinogram), and rejects jobs whose sizes do not pair."""
+    g, ang, vol, y = small_problem(n_r=64, n_t=12, n_views=31, thickness=16, seed=21)
+    y32 = y.astype(np.float32)
+    th = float(g.LengthZ) / 2
+    g0 = capi.geometry_init(g.N_r, g.N_t, ang, th, delta_xz_in=2.0, interpolate_factor=2.0)
+    g1 = capi.geometry_init(g.N_r, g.N_t, ang, th, delta_xz_in=1.0, interpolate_factor=2.0)
+    kw = dict(sigma_x=0.1, num_inner=3, seed=4)
+    j0 = capi.Job(gpu_ctx, g0, ang, y32, **kw)
+    j0.run()
+    x0 = j0.volume()
+    assert x0.max() > 0
+    up = np.repeat(np.repeat(np.repeat(x0, 2, 0), 2, 1), 2, 2)
+    ref = capi.Job(gpu_ctx, g1, ang, y32, init_volume=up, **kw)
+    j1 = capi.Job(gpu_ctx, g1, ang, y32, **kw)
+    j1.upsample_from(j0)
+    np.testing.assert_array_equal(j1.volume(), ref.volume())
+    np.testing.assert_array_equal(j1.error_sinogram(), ref.error_sinogram())
+    with pytest.raises(capi.MbirError):
+        j0.upsample_from(j1)  # coarse larger than fine
+    with pytest.raises(capi.MbirError):
+        j1.upsample_from(j1)
+    for j in (j0, j1, ref):
+        j.close()
+
+
+def test_multiresolution_device_handover_equals_files(gpu_ctx, tmp_path):
+    """The driver with levels handed over on the device ends with the same volume
+    file as with the level files, and writes no intermediate level directory."""
+    from test_io import make_tilt_series
+
+    rng = np.random.default_rng(10)
+    nv = 31
+    g, ang, vol, y = small_problem(n_r=64, n_t=12, n_views=nv, thickness=16,
+                                   gains=rng.uniform(0.9, 1.1, nv),
+                                   offsets=rng.uniform(-0.03, 0.03, nv))
+    mrc = str(tmp_path / "tilts.mrc")
+    make_tilt_series(mrc, y.astype(np.float32).transpose(0, 2, 1), mode=2,
+                     tilts=ang.astype(np.float32), pixel=1e-9)
+    outs = {}
+    for dev in (0, 1):
+        tmp = str(tmp_path / ("work%d" % dev))
+        out = str(tmp_path / ("final%d.mrc" % dev))
+        r = subprocess.run([CLI, "--multires", mrc, "2", "16", "0.1", "0.2", "3", "4", "0.001",
+                            "5", tmp, out, str(dev)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        lines = open(os.path.join(tmp, "levels.txt")).read().split("\n")
+        assert lines[0] == "error 0"
+        outs[dev] = (open(out, "rb").read(), [l.split()[19] for l in lines[1:3]], tmp)
+    assert outs[1][0] == outs[0][0]  # the final MRC, header and data, byte for byte
+    assert outs[1][1] == outs[0][1]  # sweeps per level
+    assert os.path.isdir(os.path.join(outs[0][2], "2x"))
+    assert not os.path.exists(os.path.join(outs[1][2], "2x"))
+
+
 def test_adapter_execute_matches_oracle(gpu_ctx):
     """30 outer x 10 inner with joint gain/offset and noise estimation.  The adapter
     must give exactly the C-ABI's result for the same problem and seed.  Against the
