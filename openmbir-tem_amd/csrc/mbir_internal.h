@@ -169,7 +169,7 @@ hipError_t launch_cost(const DevJob& J, const double* alpha, int noise_model, do
                        hipStream_t s);
 // one launch of the request server: kernel shape (ypl, nv), nwg workgroups
 hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& args, int nwg,
-                           int pf, hipStream_t s);
+                           int pf, hipStream_t s, int fw = 0);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, size_t n, hipStream_t s);
 // out[0] += sum of the volume over the lines of the ROI mask (FP64); out zeroed by the caller
 hipError_t launch_roi_sum(const DevJob& J, double* out, hipStream_t s);

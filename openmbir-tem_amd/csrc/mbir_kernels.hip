@@ -1043,9 +1043,10 @@ void k_icd_batch_step(const DevJob* __restrict__ jobs,
 
 // One launch of the request server: entries travel by value in the kernel
 // arguments (no host-to-device copy per launch), each naming a job slot and a
-// contiguous run of its current sweep's schedule.
-template <int YPL, int NV, bool PF>
-__global__ __launch_bounds__(kIcdBlock) __attribute__((amdgpu_waves_per_eu(icd_waves_per_eu<YPL, NV>(PF))))
+// contiguous run of its current sweep's schedule.  FW: bright-field jobs (per-entry
+// weights and selector, one detector row per slice).
+template <int YPL, int NV, bool PF, bool FW = false>
+__global__ __launch_bounds__(kIcdBlock) __attribute__((amdgpu_waves_per_eu(icd_waves_per_eu<YPL, NV>(PF || FW))))
 void k_icd_tick(const DevJob* __restrict__ jobs, const TickArgs args)
 {
   __shared__ IcdSmem<YPL> sm;
@@ -1067,8 +1068,7 @@ void k_icd_tick(const DevJob* __restrict__ jobs, const TickArgs args)
   if (i < 0 || i >= nl) return;
   const DevJob J = jobs[job];
   const LineRef ref = load_ref(J.sched + l0 + i);
-  icd_line<YPL, NV, false, PF>(J, ref, flags & 1, (flags >> 1) & 1, J.acc, sm,
-                               icd_col(args.slots));
+  icd_line<YPL, NV, FW, PF>(J, ref, flags & 1, (flags >> 1) & 1, J.acc, sm, icd_col(args.slots));
 }
 
 // jobs[slot] = J: a job descriptor written in stream order.
@@ -1544,10 +1544,27 @@ hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const Batc
 }
 
 hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& args, int nwg,
-                           int pf, hipStream_t s)
+                           int pf, hipStream_t s, int fw)
 {
   if (args.n <= 0 || args.n > kTickMax || nwg <= 0) return hipSuccess;
   const size_t dyn = icd_col_bytes(args.slots);
+  if (fw) {  // bright field: one detector row per slice
+    if (nv != 1 || ypl > 2) return hipErrorInvalidValue;
+#define MBIR_TICKF(Y)                                                                         \
+  if (pf)                                                                                     \
+    hipLaunchKernelGGL((k_icd_tick<Y, 1, true, true>), dim3(nwg), dim3(kIcdBlock), dyn, s,    \
+                       jobs, args);                                                           \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_icd_tick<Y, 1, false, true>), dim3(nwg), dim3(kIcdBlock), dyn, s,   \
+                       jobs, args);
+    if (ypl == 1) {
+      MBIR_TICKF(1)
+    } else {
+      MBIR_TICKF(2)
+    }
+#undef MBIR_TICKF
+    return hipGetLastError();
+  }
 #define MBIR_TICK(Y, N)                                                                        \
   if (pf)                                                                                      \
     hipLaunchKernelGGL((k_icd_tick<Y, N, true>), dim3(nwg), dim3(kIcdBlock), dyn, s, jobs,      \

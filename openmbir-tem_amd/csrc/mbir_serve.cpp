@@ -41,6 +41,7 @@ constexpr int kTickEvery = 4;   // ticks per run-ahead event
 constexpr int kTickEvents = 4;  // events in flight: the host runs at most 16 ticks ahead
 constexpr int kLanes = 2;       // 0: every job; 1: jobs with long sweeps
 constexpr int kLongStepsDefault = 2048;
+constexpr double kDefBragg = 5e10;  // DefBraggThreshold (BrightField/BFConstants.h)
 constexpr int kLongJobsDefault = 4;
 
 struct Req {
@@ -60,6 +61,8 @@ struct Req {
   bool finishing = false;  // final nuisance update in flight
   int ypl = 1, nv = 1;
   int lane = 0;
+  bool fw = false;          // bright field (per-entry weights)
+  double bragg_tmp = 0.0;   // bright field: the threshold of the sweeps to come
   SweepAccum* hacc = nullptr;  // pinned
   LineRef* hsched = nullptr;   // pinned copy of the current sweep's schedule
   hipEvent_t ev = nullptr;
@@ -75,8 +78,10 @@ int64_t estimate_bytes(const mbir_problem& p)
   const int64_t M = (int64_t)g.N_theta * g.N_r * g.N_t;
   const int64_t L = (int64_t)g.N_z * g.N_x;
   const int64_t NB = (int64_t)std::ceil(2.0 * g.delta_xz / g.delta_r - 1e-12) + 1;
-  // x; e, y; FP64 projection during setup; per-line arrays; column table bound
-  return V * 4 + M * 8 + M * 8 + L * 21 + L * g.N_theta * NB * 16 + (1 << 20);
+  // x; e, y; FP64 projection during setup; per-line arrays; column table bound;
+  // bright field: the per-entry weights and the counts kept for the noise step
+  return V * 4 + M * 8 + M * 8 + L * 21 + L * g.N_theta * NB * 16 + (1 << 20) +
+         (p.bf_mode ? M * 8 : 0);
 }
 
 // The job's descriptor for the server's launches: under a full device the plain
@@ -164,7 +169,10 @@ void mbir_server::setup_loop()
                         hipSuccess ||
                     hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess))
       rc = MBIR_ERR_HIP;
-    if (rc == 0 && (job->J.N_y > 128 || job->J.vlr_max > 2 || job->J.wfull))
+    // bright field: one detector row per slice and full sweeps (NHICD lists run
+    // as single jobs)
+    if (rc == 0 && (job->J.N_y > 128 || job->J.vlr_max > 2 ||
+                    (job->J.wfull && (job->prob.bf_nhicd || job->J.vlr_max > 1))))
       rc = MBIR_ERR_UNSUPPORTED;
 
     lk.lock();
@@ -213,10 +221,16 @@ void mbir_server::finish(Req* r, int state, int error)
 int mbir_server::start_sweep(Req* r)
 {
   mbir_job* j = r->job;
+  hipStream_t stream = stream_of(r);
+  if (r->fw) {
+    // the threshold 5e10 for the inner sweeps of an outer iteration with more than
+    // one (BrightField/BFReconstructionEngine.cpp:655-660); the descriptor carries it
+    if (r->num_iter > 1 && r->inner == 0) set_bragg(j, kDefBragg);
+    HIP_TRY(launch_set_devjob(djobs.p, r->slot, tick_desc(j), stream));
+  }
   build_schedule(j, true);
   if (!j->sched_h.empty())
     std::memcpy(r->hsched, j->sched_h.data(), j->sched_h.size() * sizeof(LineRef));
-  hipStream_t stream = stream_of(r);
   int rc = prepare_sweep_on(j, stream, r->hsched);
   if (rc < 0) return rc;
   r->step = 0;
@@ -248,6 +262,36 @@ int mbir_server::end_sweep(Req* r)
     std::lock_guard<std::mutex> lk(mu);
     finish(r, MBIR_JOB_FAILED, MBIR_ERR_CANCELLED);
     return 1;
+  }
+  if (r->fw) {
+    // bright field (BrightField/BFReconstructionEngine.cpp:641-870; mbir_job_run is
+    // the same loop): T from the last inner sweep of a multi-sweep loop on; the stop
+    // test ends the run before the nuisance step, which runs only for more than one
+    // outer iteration
+    const double des = j->prob.bragg_T;
+    if (status != 0 && r->inner == r->num_iter - 1 && r->inner != 0) {
+      r->bragg_tmp = des;
+      set_bragg(j, r->bragg_tmp);
+    }
+    r->inner++;
+    if (status != 0 && r->inner < r->num_iter) return start_sweep(r) < 0 ? -1 : 0;
+    const int outer = r->outer;
+    r->outer++;
+    r->outer_done = r->outer;
+    const bool stop = status == 0 && outer >= 1;
+    if (!stop && j->prob.num_outer > 1 && (j->prob.joint_estimation || j->prob.noise_model)) {
+      if (nuisance_update_on(j, stream) < 0) return -1;
+      if (launch_set_devjob(djobs.p, r->slot, tick_desc(j), stream) != hipSuccess) return -1;
+    }
+    if (stop || r->outer >= j->prob.num_outer) {
+      if (hipEventRecord(r->ev, stream) != hipSuccess) return -1;
+      r->finishing = true;
+      return 0;
+    }
+    r->num_iter = 1;
+    r->inner = 0;
+    set_bragg(j, r->bragg_tmp);
+    return start_sweep(r) < 0 ? -1 : 0;
   }
   r->inner++;
   if (status != 0 && r->inner < r->num_iter) return start_sweep(r) < 0 ? -1 : 0;
@@ -311,6 +355,12 @@ void mbir_server::run_loop()
         r->t_start = now();
         r->num_iter = r->job->prob.num_inner;
         icd_shape(r->job->J, r->job->J.vlr_max, &r->ypl, &r->nv);
+        r->fw = r->job->J.wfull != nullptr;
+        if (r->fw) {
+          r->nv = 1;  // one detector row per slice
+          r->bragg_tmp = r->num_iter > 1 ? kDefBragg : r->job->prob.bragg_T;
+          set_bragg(r->job, r->bragg_tmp);
+        }
         // the lane: jobs with long sweeps (conflict-free steps per sweep) go to lane 1
         const int64_t spp = (int64_t)r->job->g.N_z * r->job->spacing;
         r->lane = (long_steps > 0 && spp >= long_steps && long_resident < long_jobs) ? 1 : 0;
@@ -379,15 +429,16 @@ void mbir_server::run_loop()
       }
       hipStream_t stream = lane_stream[lane];
       bool launched = false;
-      for (int shape = 0; shape < 4 && !bad; ++shape) {
-        const int ypl = 1 + (shape >> 1), nv = 1 + (shape & 1);
+      for (int shape = 0; shape < 8 && !bad; ++shape) {
+        const int ypl = 1 + ((shape >> 1) & 1), nv = 1 + (shape & 1), fw = shape >> 2;
+        if (fw && nv != 1) continue;
         args.n = 0;
         args.slots = 4;
         int wg = 0;
         std::vector<Req*> last;  // jobs whose sweep ends with this launch
         auto flush = [&]() -> bool {
           if (args.n == 0) return true;
-          if (launch_icd_tick(ypl, nv, djobs.p, args, wg, tune.choose(ypl, nv, wg), stream) !=
+          if (launch_icd_tick(ypl, nv, djobs.p, args, wg, tune.choose(ypl, nv, wg), stream, fw) !=
               hipSuccess)
             return false;
           n_launch++;
@@ -408,8 +459,8 @@ void mbir_server::run_loop()
           return true;
         };
         for (Req* r : act) {
-          if (r->lane != lane || r->ypl != ypl || r->nv != nv || r->waiting || r->finishing ||
-              r->step >= r->nsteps)
+          if (r->lane != lane || r->ypl != ypl || r->nv != nv || (int)r->fw != fw || r->waiting ||
+              r->finishing || r->step >= r->nsteps)
             continue;
           mbir_job* j = r->job;
           TickEntry& E = args.e[args.n++];

@@ -150,3 +150,46 @@ def test_bf_nhicd_stop_rule_every_twenty_calls(gpu_ctx):
                        num_inner=45, stop_threshold=stop, bf_nhicd=1, **BF)
         rs, _ = job.run()
         assert rs.sweeps == calls, (stop, rs.sweeps)
+
+
+def test_bf_jobs_in_the_server_equal_alone(gpu_ctx):
+    """Bright-field jobs in the request server (their own launches, beside HAADF jobs)
+    follow the same loop as mbir_job_run: threshold schedule, stop test before the
+    nuisance step, offset and shared-variance steps.  Each ends bit for bit as
+    alone."""
+    from problems import small_problem
+
+    cases = []
+    for i, (frac, n_r, n_t, nin, nout, je, nm) in enumerate(
+            [(0.03, 32, 6, 6, 1, 0, 0), (0.05, 40, 8, 3, 3, 1, 1), (0.02, 24, 4, 1, 2, 1, 0)]):
+        g, ang, vol, counts = bf_problem(n_r=n_r, n_t=n_t, seed=30 + i, bragg_frac=frac)
+        geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
+        kw = dict(BF, init_offset=np.full(len(ang), -np.log(2000.0)), seed=11 + i,
+                  num_inner=nin, num_outer=nout, joint_estimation=je, noise_model=nm,
+                  stop_threshold=1e-3)
+        cases.append((geom, ang, counts.astype(np.float32), kw))
+    g, ang, vol, y = small_problem(n_r=32, n_t=8, n_views=31, seed=41)
+    cases.append((capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2), ang,
+                  y.astype(np.float32), dict(seed=5, stop_threshold=2e-3, num_inner=20)))
+    refs = []
+    for geom, ang, c, kw in cases:
+        job = capi.Job(gpu_ctx, geom, ang, c, **kw)
+        rs, _ = job.run(cap=0)
+        refs.append((job.volume(), job.error_sinogram(), job.nuisance(), rs.sweeps))
+        job.close()
+    srv = capi.Server(gpu_ctx, max_jobs=8)
+    tickets = [srv.submit(geom, ang, c, **kw) for geom, ang, c, kw in cases]
+    for t, (vol_r, err_r, (I0, mu, al), sweeps) in zip(tickets, refs):
+        rep = srv.wait(t, timeout=60.0)
+        assert rep is not None and rep.state == capi.JOB_DONE and rep.error == 0
+        assert rep.sweeps == sweeps
+        job = srv.job(t)
+        np.testing.assert_array_equal(job.volume(), vol_r)
+        np.testing.assert_array_equal(job.error_sinogram(), err_r)
+        I0s, mus, als = job.nuisance()
+        np.testing.assert_array_equal(mus, mu)
+        np.testing.assert_array_equal(als, al)
+        srv.release(t)
+    st = srv.stats()
+    assert st.jobs_done == len(cases) and st.jobs_failed == 0 and st.fatal_error == 0
+    srv.close()
