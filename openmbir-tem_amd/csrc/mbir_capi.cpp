@@ -814,27 +814,36 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
     mbir_job* j = jobs[i];
     if (j->ctx != ctx) return MBIR_ERR_INPUT;
     if (j->cancel.load()) return MBIR_ERR_CANCELLED;
-    if (j->J.N_y > 128 || j->J.vlr_max > 2 || j->J.wfull) return MBIR_ERR_UNSUPPORTED;
+    // bright field: one detector row per slice, full sweeps (NHICD lists run alone)
+    if (j->J.N_y > 128 || j->J.vlr_max > 2 ||
+        (j->J.wfull && (j->prob.bf_nhicd || j->J.vlr_max > 1)))
+      return MBIR_ERR_UNSUPPORTED;
     int rc = prepare_sweep(j);
     if (rc < 0) return rc;
     dj[i] = j->J;
     dj[i].e_atomic = j->prob.schedule == 1 ? 1 : 0;
     max_steps = std::max(max_steps, (int)j->step_ptr.size() - 1);
   }
-  // jobs ordered by kernel shape so that each step's entries form one run per shape
+  // jobs ordered by kernel shape (bright-field jobs apart, one detector row per
+  // slice) so that each step's entries form one run per shape
+  auto shape_of = [&](const mbir_job* j, int* y, int* v, int* f) {
+    icd_shape(j->J, j->J.vlr_max, y, v);
+    *f = j->J.wfull ? 1 : 0;
+    if (*f) *v = 1;
+  };
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    int ya, na, yb, nb;
-    icd_shape(jobs[a]->J, jobs[a]->J.vlr_max, &ya, &na);
-    icd_shape(jobs[b]->J, jobs[b]->J.vlr_max, &yb, &nb);
-    return ya != yb ? ya < yb : na < nb;
+    int ya, na, fa, yb, nb, fb;
+    shape_of(jobs[a], &ya, &na, &fa);
+    shape_of(jobs[b], &yb, &nb, &fb);
+    return fa != fb ? fa < fb : (ya != yb ? ya < yb : na < nb);
   });
   std::vector<BatchEntry> ent;
   std::vector<int> l_e0(max_steps), l_ne(max_steps), l_wg(max_steps);
   for (int k = 0; k < max_steps; ++k) {
     l_e0[k] = (int)ent.size();
-    int wg = 0, cur_ypl = -1, cur_nv = -1;
+    int wg = 0, cur_ypl = -1, cur_nv = -1, cur_fw = -1;
     for (int i : order) {
       mbir_job* j = jobs[i];
       if (k + 1 >= (int)j->step_ptr.size()) continue;
@@ -847,11 +856,12 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
       e.wg_off = wg;
       const int o = outer ? outer[i] : 0, in = inner ? inner[i] : 0;
       e.flags = ((o == 0 && in == 0) ? 1 : 0) | (j->prob.zero_skipping ? 2 : 0);
-      icd_shape(j->J, j->J.vlr_max, &e.ypl, &e.nv);
-      if (e.ypl != cur_ypl || e.nv != cur_nv) {  // a new launch: offsets restart
+      shape_of(j, &e.ypl, &e.nv, &e.fw);
+      if (e.ypl != cur_ypl || e.nv != cur_nv || e.fw != cur_fw) {  // a new launch: offsets restart
         wg = 0;
         cur_ypl = e.ypl;
         cur_nv = e.nv;
+        cur_fw = e.fw;
       }
       e.wg_off = wg;
       e.acc = j->acc.p;
@@ -888,7 +898,8 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
     while (e < eend) {
       int f = e, wg = 0;
       int slots = 4;
-      while (f < eend && ent[f].ypl == ent[e].ypl && ent[f].nv == ent[e].nv) {
+      while (f < eend && ent[f].ypl == ent[e].ypl && ent[f].nv == ent[e].nv &&
+             ent[f].fw == ent[e].fw) {
         slots = std::max(slots, jobs[ent[f].job]->J.col_slots);
         wg += ent[f++].nlines;
       }
@@ -902,7 +913,7 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
         HIP_TRY(hipEventRecord(a, s));
       }
       HIP_TRY(launch_icd_batch_step(ent[e].ypl, ent[e].nv, ctx->djobs.p, ctx->ents.p, e, f - e,
-                                    wg, slots, pf, s));
+                                    wg, slots, pf, s, ent[e].fw));
       if (pr) {
         HIP_TRY(hipEventRecord(b, s));
         pr->ev.push_back(a);

@@ -96,6 +96,7 @@ struct BatchEntry {
   int wg_off;        // first workgroup of this entry within the launch
   int flags;         // bit 0: first sweep, bit 1: zero skipping on
   int ypl, nv;       // kernel shape
+  int fw;            // bright field (per-entry weights): kernels of their own
   SweepAccum* acc;
 };
 
@@ -152,7 +153,7 @@ void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv);
 // one step of several jobs of one kernel shape: entries [e0, e0 + ne) of ent,
 // nwg workgroups in total
 hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const BatchEntry* ent,
-                                 int e0, int ne, int nwg, int slots, int pf, hipStream_t s);
+                                 int e0, int ne, int nwg, int slots, int pf, hipStream_t s, int fw = 0);
 hipError_t launch_build_columns(const DevJob& J, int* row, float* a, hipStream_t s);
 hipError_t launch_column_weights(const DevJob& J, float* ka, float* kw, int nnz, hipStream_t s);
 hipError_t launch_export_column(const DevJob& J, int z, int x, float* vals, int* rows,

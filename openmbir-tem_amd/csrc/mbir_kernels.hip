@@ -1018,8 +1018,8 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef*
 // (YPL, NV): entries [e0, e0 + ne) of the sweep's entry table; workgroup g serves
 // entry e with wg_off[e] <= g < wg_off[e] + nlines[e].  Jobs never share memory,
 // so one launch needs no ordering between them.
-template <int YPL, int NV, bool PF>
-__global__ __launch_bounds__(kIcdBlock) __attribute__((amdgpu_waves_per_eu(icd_waves_per_eu<YPL, NV>(PF))))
+template <int YPL, int NV, bool PF, bool FW = false>
+__global__ __launch_bounds__(kIcdBlock) __attribute__((amdgpu_waves_per_eu(icd_waves_per_eu<YPL, NV>(PF || FW))))
 void k_icd_batch_step(const DevJob* __restrict__ jobs,
                       const BatchEntry* __restrict__ ent, int e0, int ne, int slots)
 {
@@ -1037,8 +1037,7 @@ void k_icd_batch_step(const DevJob* __restrict__ jobs,
   if (i < 0 || i >= E.nlines) return;
   const DevJob J = jobs[__builtin_amdgcn_readfirstlane(E.job)];
   const LineRef ref = load_ref(E.lines + i);
-  icd_line<YPL, NV, false, PF>(J, ref, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm,
-                               icd_col(slots));
+  icd_line<YPL, NV, FW, PF>(J, ref, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm, icd_col(slots));
 }
 
 // One launch of the request server: entries travel by value in the kernel
@@ -1519,10 +1518,27 @@ hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, in
 // The caller picks the form of the line update (pf: prefetching; both compute the same
 // values) from its tuning table.
 hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const BatchEntry* ent,
-                                 int e0, int ne, int nwg, int slots, int pf, hipStream_t s)
+                                 int e0, int ne, int nwg, int slots, int pf, hipStream_t s, int fw)
 {
   if (ne <= 0 || nwg <= 0) return hipSuccess;
   const size_t dyn = icd_col_bytes(slots);
+  if (fw) {  // bright field: one detector row per slice
+    if (nv != 1 || ypl > 2) return hipErrorInvalidValue;
+#define MBIR_BATCHF(Y)                                                                        \
+  if (pf)                                                                                     \
+    hipLaunchKernelGGL((k_icd_batch_step<Y, 1, true, true>), dim3(nwg), dim3(kIcdBlock), dyn, \
+                       s, jobs, ent, e0, ne, slots);                                          \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_icd_batch_step<Y, 1, false, true>), dim3(nwg), dim3(kIcdBlock), dyn,\
+                       s, jobs, ent, e0, ne, slots);
+    if (ypl == 1) {
+      MBIR_BATCHF(1)
+    } else {
+      MBIR_BATCHF(2)
+    }
+#undef MBIR_BATCHF
+    return hipGetLastError();
+  }
 #define MBIR_BATCH(Y, N)                                                                      \
   if (pf)                                                                                     \
     hipLaunchKernelGGL((k_icd_batch_step<Y, N, true>), dim3(nwg), dim3(kIcdBlock), dyn, s,    \

@@ -193,3 +193,30 @@ def test_bf_jobs_in_the_server_equal_alone(gpu_ctx):
     st = srv.stats()
     assert st.jobs_done == len(cases) and st.jobs_failed == 0 and st.fatal_error == 0
     srv.close()
+
+
+def test_bf_jobs_in_a_batched_sweep_equal_alone(gpu_ctx):
+    """Bright-field jobs in mbir_batch_sweep (launches of their own, beside a HAADF
+    job): every sweep ends bit for bit as the job's own sweep."""
+    from problems import small_problem
+
+    specs = []
+    for i, (frac, n_r, n_t) in enumerate([(0.03, 32, 6), (0.05, 40, 8)]):
+        g, ang, vol, counts = bf_problem(n_r=n_r, n_t=n_t, seed=50 + i, bragg_frac=frac)
+        specs.append((capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2), ang,
+                      counts.astype(np.float32),
+                      dict(BF, init_offset=np.full(len(ang), -np.log(2000.0)), seed=21 + i)))
+    g, ang, vol, y = small_problem(n_r=32, n_t=8, n_views=31, seed=42)
+    specs.append((capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2), ang,
+                  y.astype(np.float32), dict(seed=6)))
+    alone = [capi.Job(gpu_ctx, *s[:3], **s[3]) for s in specs]
+    batched = [capi.Job(gpu_ctx, *s[:3], **s[3]) for s in specs]
+    for it in range(3):
+        for j in alone:
+            j.sweep(0, it)
+        capi.batch_sweep(gpu_ctx, batched, outer=[0] * 3, inner=[it] * 3)
+        for a, b in zip(alone, batched):
+            np.testing.assert_array_equal(b.volume(), a.volume())
+            np.testing.assert_array_equal(b.error_sinogram(), a.error_sinogram())
+    for j in alone + batched:
+        j.close()
