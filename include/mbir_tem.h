@@ -124,6 +124,14 @@ typedef struct mbir_problem {
    * volume sum every 20 calls; 0 = full random-order sweeps */
   int32_t bf_nhicd;
   int32_t pad_;
+  /* HAADF with a bright-field sinogram (the reference's BFTomoInputs / BFSinogram and
+   * BF_Flag, HAADF/HAADF_ReconstructionEngine_Extra.cpp:89-175): optional raw
+   * bright-field counts [N_theta][N_r][N_t]; every projection entry is scaled by
+   * counts + BF_OFFSET (32768, HAADF/HAADFConstants.h:49) of its detector element, the
+   * initial error is y - s Yest - mu (:718-727), and gains and offsets are not
+   * estimated (:429).  Single-job paths only (mbir_job_run, mbir_job_sweep); not with
+   * bf_mode.  NULL = none. */
+  const float* bf_sinogram;
 } mbir_problem;
 
 /* Per-sweep statistics (updateVoxels, HAADF_ReconstructionEngine_UpdateVoxels.cpp:717-786) */

@@ -362,6 +362,19 @@ int setup_job(mbir_job* j)
     HIP_TRY(hipMemcpyAsync(j->wfull.p, bf_w.data(), M * sizeof(float), hipMemcpyHostToDevice, s));
     j->bytes += (int64_t)M * 4;
   }
+  // HAADF with a bright-field sinogram (HAADF/HAADF_ReconstructionEngine_Extra.cpp:
+  // 143-168): the per-element scale counts + BF_OFFSET, a zero spare row behind it
+  std::vector<float> bfs_h;
+  if (P.bf_sinogram) {
+    if (P.bf_mode) return MBIR_ERR_INPUT;
+    constexpr double kBfOffset = 32768.0;  // BF_OFFSET, HAADF/HAADFConstants.h:49
+    bfs_h.resize(M);
+    for (size_t i = 0; i < M; ++i) bfs_h[i] = (float)((double)P.bf_sinogram[i] + kBfOffset);
+    HIP_TRY(j->bfs.alloc(M + g.N_t));
+    HIP_TRY(launch_fill_f32(j->bfs.p + M, 0.f, g.N_t, s));
+    HIP_TRY(hipMemcpyAsync(j->bfs.p, bfs_h.data(), M * sizeof(float), hipMemcpyHostToDevice, s));
+    j->bytes += (int64_t)M * 4;
+  }
   HIP_TRY(hipMemcpyAsync(j->y.p, P.bf_mode ? bf_y.data() : P.counts, M * sizeof(float),
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(j->cosv.p, cosv.data(), NT * sizeof(double), hipMemcpyHostToDevice, s));
@@ -404,6 +417,8 @@ int setup_job(mbir_job* j)
   // switch to the plain form for conflict-free schedules)
   J.e_atomic = 1;
   J.magmap = nullptr;
+  J.bfs = P.bf_sinogram ? j->bfs.p : nullptr;
+  if (J.bfs && J.vlr_max > 2) return MBIR_ERR_UNSUPPORTED;  // scaled line update: <= 2 rows
   if (P.bf_mode) {
     J.wfull = j->wfull.p;
     J.e_atomic = 0;  // the selector refresh is a read-modify-write of its own
@@ -571,7 +586,9 @@ int nuisance_update_on(mbir_job* j, hipStream_t s)
     }
     return MBIR_OK;
   }
-  if (j->prob.joint_estimation) {
+  // with a bright-field sinogram no gain and offset estimation
+  // (HAADF/HAADF_ReconstructionEngine_Extra.cpp:429)
+  if (j->prob.joint_estimation && !j->J.bfs) {
     HIP_TRY(launch_joint_sums(j->J, j->sums6.p, s));
     HIP_TRY(launch_joint_solve(NT, j->sums6.p, j->prob.target_gain, j->I0.p, j->mu.p, j->I0b.p,
                                j->mub.p, s));
@@ -814,8 +831,9 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
     mbir_job* j = jobs[i];
     if (j->ctx != ctx) return MBIR_ERR_INPUT;
     if (j->cancel.load()) return MBIR_ERR_CANCELLED;
-    // bright field: one detector row per slice, full sweeps (NHICD lists run alone)
-    if (j->J.N_y > 128 || j->J.vlr_max > 2 ||
+    // bright field: one detector row per slice, full sweeps (NHICD lists run alone);
+    // HAADF with a bright-field sinogram: single-job paths only
+    if (j->J.N_y > 128 || j->J.vlr_max > 2 || j->J.bfs ||
         (j->J.wfull && (j->prob.bf_nhicd || j->J.vlr_max > 1)))
       return MBIR_ERR_UNSUPPORTED;
     int rc = prepare_sweep(j);
@@ -1004,6 +1022,7 @@ extern "C" int32_t mbir_job_create(mbir_ctx* ctx, const mbir_problem* prob, mbir
   j->prob.counts = nullptr;
   j->prob.init_volume = nullptr;
   j->prob.init_gain = j->prob.init_offset = j->prob.init_variance = nullptr;
+  j->prob.bf_sinogram = nullptr;
   if (hipStreamCreate(&j->stream) != hipSuccess) {
     delete j;
     return MBIR_ERR_HIP;

@@ -84,6 +84,7 @@ struct mbir_job {
   DevBuf<LineRef> sched;
   DevBuf<float> col_a, col_ka, col_kw;
   DevBuf<float> wfull;       // bright field: per-entry weights, sign = selector
+  DevBuf<float> bfs;         // HAADF with a bright-field sinogram: per-element scale
   DevBuf<double> alpha_b;    // bright field: the next shared variance
   double bragg = 0;          // bright field: current Bragg threshold
   std::vector<int> nnz_h;  // per-line column entries (host copy)
@@ -119,7 +120,7 @@ struct mbir_job {
     I0b.release(); mub.release(); sums6.release(); yest.release(); cost.release(); prj.release();
     vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
     mask.release(); acc.release(); col_ptr.release(); col_row.release();
-    col_a.release(); col_ka.release(); col_kw.release(); wfull.release(); alpha_b.release();
+    col_a.release(); col_ka.release(); col_kw.release(); wfull.release(); bfs.release(); alpha_b.release();
     gsched.release();
     magmap.release();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);

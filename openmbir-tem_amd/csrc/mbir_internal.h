@@ -82,6 +82,8 @@ struct DevJob {
   uint32_t spare_b;      // M * 4: byte offset of the spare detector row (N_t zeros) that
                          // follows e and wfull, the target of padding column entries
   float* magmap;         // [N_z * N_x] per-line sum of |x_new - x_old| of the last update
+  const float* bfs;      // HAADF with a bright-field sinogram: per-element scale (counts +
+                         // BF_OFFSET), spare row 0; null without
                          // (NHICD), nullptr = not kept
   const LineRef* sched;  // [N_z * N_x] lines of the current sweep, step by step
   SweepAccum* acc;       // the current sweep's accumulators

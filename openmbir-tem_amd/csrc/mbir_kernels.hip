@@ -284,7 +284,9 @@ __global__ void k_init_error(DevJob J, const double* yest)
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < M;
        i += (size_t)gridDim.x * blockDim.x) {
     const int th = (int)(i / per);
-    J.e[i] = (float)((double)J.y[i] - yest[i] - J.mu[th]);
+    // with a bright-field sinogram: y - s Yest - mu (HAADF_ReconstructionEngine_Extra.cpp:724-727)
+    const double ye = J.bfs ? (double)J.bfs[i] * yest[i] : yest[i];
+    J.e[i] = (float)((double)J.y[i] - ye - J.mu[th]);
   }
 }
 
@@ -470,7 +472,8 @@ constexpr int icd_waves_per_eu(bool pf)
 
 // PF: issue the first theta batch's loads before the LDS staging (shorter line
 // latency, more registers: the form for launches that do not fill the device).
-template <int YPL, int NV, bool FW, bool PF>
+// SB: HAADF with a bright-field sinogram, every entry's element scaled by J.bfs
+template <int YPL, int NV, bool FW, bool PF, bool SB = false>
 __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_on,
                          SweepAccum* acc, IcdSmem<YPL>& sm, const IcdCol col)
 {
@@ -582,7 +585,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       col.rb[i] = in ? rw_[it] * 4u : J.spare_b;
       col.ka[i] = in ? ka_[it] : 0.f;
       col.kw[i] = in ? kw_[it] : 0.f;
-      if constexpr (!FW)
+      if constexpr (!FW && !SB)
         if (in) kk = __fma_rn((double)ka_[it], (double)kw_[it], kk);
     }
   }
@@ -611,6 +614,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       for (int k = 0; k < NV; ++k) {
         ev[jy][k] = ldo(J.e, rb + vtb[jy][k]);
         if constexpr (FW) wf[jy][k] = ldo(J.wfull, rb + vtb[jy][k]);
+        if constexpr (SB) wf[jy][k] = ldo(J.bfs, rb + vtb[jy][k]);  // the element's scale
       }
   };
   // the batch [base, base + U) of this lane's stream, rows from LDS
@@ -632,7 +636,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
 #pragma unroll
     for (int u = 0; u < U; ++u) load_entry(u < cnt ? r0[u] * 4u : J.spare_b, ev0[u], wf0[u]);
   }
-  if constexpr (!FW) {
+  if constexpr (!FW && !SB) {
     kk = wave_sum(kk);
     if (lane == 0) sm.kk[wave] = kk;
   }
@@ -652,13 +656,37 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   // accumulated in FP64 (no compensation needed); theta2 comes from the column
   // factor above.  Bright field: both per entry, Kahan-compensated fp32. ----
   Kahan t1[YPL], t2[YPL];
-  double a1[YPL][NV];
+  double a1[YPL][NV], a2[YPL][NV];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) a1[jy][k] = 0.0;
+    for (int k = 0; k < NV; ++k) a1[jy][k] = a2[jy][k] = 0.0;
   auto accumulate = [&](int base, const auto& ev, const auto& wf) {
-    if constexpr (!FW) {
+    if constexpr (SB) {
+      // scaled entries (HAADF_ReconstructionEngine_UpdateVoxels.cpp:354-362): theta1
+      // sums (e s) kw, theta2 sums s^2 (ka kw), per lane in FP64 (wf holds s)
+#pragma unroll
+      for (int q = 0; q < U / 4; ++q) {
+        if (base + 4 * q < cnt) {
+          const float4 a4 = ka4(base + 4 * q);
+          const float4 w4 = kw4(base + 4 * q);
+          const float kav[4] = {a4.x, a4.y, a4.z, a4.w};
+          const float kwv[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const double kd = (double)kwv[t], kk2 = (double)kav[t] * kd;
+#pragma unroll
+            for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+              for (int k = 0; k < NV; ++k) {
+                const double sd = (double)wf[4 * q + t][jy][k];
+                a1[jy][k] = __fma_rn((double)ev[4 * q + t][jy][k] * sd, kd, a1[jy][k]);
+                a2[jy][k] = __fma_rn(sd * sd, kk2, a2[jy][k]);
+              }
+          }
+        }
+      }
+    } else if constexpr (!FW) {
 #pragma unroll
       for (int q = 0; q < U / 4; ++q) {
         if (base + 4 * q < cnt) {
@@ -726,29 +754,32 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       d2[jy] = (double)t2[jy].s - (double)t2[jy].c;
     } else {
       d1[jy] = 0.0;
-#pragma unroll
-      for (int k = 0; k < NV; ++k) d1[jy] = __fma_rn((double)vv[jy][k], a1[jy][k], d1[jy]);
       d2[jy] = 0.0;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        d1[jy] = __fma_rn((double)vv[jy][k], a1[jy][k], d1[jy]);
+        if constexpr (SB) d2[jy] = __fma_rn((double)vv[jy][k] * vv[jy][k], a2[jy][k], d2[jy]);
+      }
     }
   }
   for (int m = TY; m < 64; m <<= 1)
 #pragma unroll
     for (int jy = 0; jy < YPL; ++jy) {
       d1[jy] += __shfl_xor(d1[jy], m);
-      if constexpr (FW) d2[jy] += __shfl_xor(d2[jy], m);
+      if constexpr (FW || SB) d2[jy] += __shfl_xor(d2[jy], m);
     }
   if (sub == 0)
 #pragma unroll
     for (int jy = 0; jy < YPL; ++jy) {
       sm.pd1[wave][yl + TY * jy] = d1[jy];
-      if constexpr (FW) sm.pd2[wave][yl + TY * jy] = d2[jy];
+      if constexpr (FW || SB) sm.pd2[wave][yl + TY * jy] = d2[jy];
     }
   __syncthreads();  // D
   if (tid < Ny) {
     double T1 = 0.0, T2 = 0.0;
 #pragma unroll
     for (int w = 0; w < kIcdWaves; ++w) T1 += sm.pd1[w][tid];
-    if constexpr (FW) {
+    if constexpr (FW || SB) {
 #pragma unroll
       for (int w = 0; w < kIcdWaves; ++w) T2 += sm.pd2[w][tid];
     } else {
@@ -908,6 +939,10 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   auto enew = [&](float e, float ka, int jy, int k) {
     return __fadd_rn(e, -__fmul_rn(ka, vd[jy][k]));
   };
+  // scaled entries (..._UpdateVoxels.cpp:438-446): e -= s (I0 a vlr dx)
+  auto enew_s = [&](float e, float ka, int jy, int k, float sc) {
+    return __fadd_rn(e, -__fmul_rn(__fmul_rn(ka, vd[jy][k]), sc));
+  };
   // padding entries (zero gains, spare row) are updated like the others: they leave
   // the spare row at zero
   for (int base = 0; base < cnt; base += U) {
@@ -955,16 +990,18 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
             for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
               for (int k = 0; k < NV; ++k)
-                if (d[jy] != 0.f && vok[jy][k])
-                  (void)__hip_atomic_fetch_add(pto(J.e, rr[t] + vtb[jy][k]),
-                                               -__fmul_rn(aa[t], vd[jy][k]), __ATOMIC_RELAXED,
+                if (d[jy] != 0.f && vok[jy][k]) {
+                  float p = __fmul_rn(aa[t], vd[jy][k]);
+                  if constexpr (SB) p = __fmul_rn(p, ldo(J.bfs, rr[t] + vtb[jy][k]));
+                  (void)__hip_atomic_fetch_add(pto(J.e, rr[t] + vtb[jy][k]), -p, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT);
+                }
         }
       }
     } else {
       // full batches: read-modify-write, every load of a batch in flight before its
       // stores (lanes whose slices did not change load and store nothing)
-      float ev[U][YPL][NV];
+      float ev[U][YPL][NV], sv[SB ? U : 1][YPL][NV];
 #pragma unroll
       for (int q = 0; q < U / 4; ++q) {
         if (base + 4 * q < cnt) {
@@ -976,8 +1013,10 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
             for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
               for (int k = 0; k < NV; ++k)
-                if (d[jy] != 0.f && vok[jy][k])
+                if (d[jy] != 0.f && vok[jy][k]) {
                   ev[4 * q + t][jy][k] = ldo(J.e, rr[t] + vtb[jy][k]);
+                  if constexpr (SB) sv[4 * q + t][jy][k] = ldo(J.bfs, rr[t] + vtb[jy][k]);
+                }
         }
       }
 #pragma unroll
@@ -993,8 +1032,13 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
             for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
               for (int k = 0; k < NV; ++k)
-                if (d[jy] != 0.f && vok[jy][k])
-                  *pto(J.e, rr[t] + vtb[jy][k]) = enew(ev[4 * q + t][jy][k], aa[t], jy, k);
+                if (d[jy] != 0.f && vok[jy][k]) {
+                  if constexpr (SB)
+                    *pto(J.e, rr[t] + vtb[jy][k]) =
+                        enew_s(ev[4 * q + t][jy][k], aa[t], jy, k, sv[4 * q + t][jy][k]);
+                  else
+                    *pto(J.e, rr[t] + vtb[jy][k]) = enew(ev[4 * q + t][jy][k], aa[t], jy, k);
+                }
         }
       }
     }
@@ -1003,7 +1047,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   if constexpr (!PF) line_stats();
 }
 
-template <int YPL, int NV, bool FW = false>
+template <int YPL, int NV, bool FW = false, bool SB = false>
 __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef* lines, int nlines,
                                                         int first, int zs_on, SweepAccum* acc)
 {
@@ -1011,7 +1055,7 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef*
   if ((int)blockIdx.x >= nlines) return;
   const LineRef ref = lines[blockIdx.x];
   if (__builtin_amdgcn_readfirstlane(ref.line) < 0) return;  // padding of a graph's step slot
-  icd_line<YPL, NV, FW, true>(J, ref, first, zs_on, acc, sm, icd_col(J.col_slots));
+  icd_line<YPL, NV, FW, true, SB>(J, ref, first, zs_on, acc, sm, icd_col(J.col_slots));
 }
 
 // One conflict-free step of every job of a batch that has kernel shape
@@ -1488,6 +1532,12 @@ static void launch_step_nv(const DevJob& J, const LineRef* lines, int nlines, in
   if (J.wfull)  // bright field: per-entry weights and selector (one row per slice)
     hipLaunchKernelGGL((k_icd_step<YPL, 1, true>), dim3(nlines), dim3(kIcdBlock), dyn, s, J, lines,
                        nlines, first, zs, acc);
+  else if (J.bfs && J.vlr_max <= 1)  // HAADF with a bright-field sinogram
+    hipLaunchKernelGGL((k_icd_step<YPL, 1, false, true>), dim3(nlines), dim3(kIcdBlock), dyn, s, J,
+                       lines, nlines, first, zs, acc);
+  else if (J.bfs)
+    hipLaunchKernelGGL((k_icd_step<YPL, 2, false, true>), dim3(nlines), dim3(kIcdBlock), dyn, s, J,
+                       lines, nlines, first, zs, acc);
   else if (J.vlr_max <= 1)
     hipLaunchKernelGGL((k_icd_step<YPL, 1>), dim3(nlines), dim3(kIcdBlock), dyn, s, J, lines, nlines,
                        first, zs, acc);

@@ -433,6 +433,9 @@ struct orc_engine {
   double* mu;
   double* alpha;
   double* init_variance;
+  /* bright-field sinogram of the HAADF engine (BF_Flag): bright-field counts +
+   * BF_OFFSET per detector element, scaling every projection entry; NULL without */
+  double* bfs;
   double stop_threshold;
   int32_t sweep_counter;
   uint8_t* sel;    /* bright field: Bragg selector [N_theta][N_r][N_t] */
@@ -451,7 +454,7 @@ void orc_engine_destroy(orc_engine* e)
   free(e->H); free(e->Ht); free(e->vlr_count); free(e->vlr_t); free(e->vlr_val);
   free(e->col_ptr); free(e->col_idx); free(e->col_val); free(e->mask); free(e->x);
   free(e->counts); free(e->e); free(e->w); free(e->yest); free(e->I0); free(e->mu);
-  free(e->alpha); free(e->init_variance); free(e->sel);
+  free(e->alpha); free(e->init_variance); free(e->sel); free(e->bfs);
   free(e);
 }
 
@@ -668,6 +671,7 @@ static int update_voxel(orc_engine* e, int32_t j_new, int32_t k_new, int32_t i, 
     for (int k = 0; k < e->vlr_count[i]; ++k) {
       size_t eidx = ((size_t)it * g->N_r + ir) * g->N_t + e->vlr_t[i * VLR_CAP + k];
       double PE = kConst0 * e->vlr_val[i * VLR_CAP + k];
+      if (e->bfs) PE *= e->bfs[eidx]; /* BF_Flag, ..._UpdateVoxels.cpp:354-362 */
       if (e->p.bf_mode && e->sel[eidx] == 0) {
         /* rejected entry: BFForwardModel::computeTheta, :874-882 */
         double QC = (e->p.bragg_delta * e->bragg) / (fabs(e->e[eidx]) * sqrt(e->w[eidx]));
@@ -705,6 +709,7 @@ static int update_voxel(orc_engine* e, int32_t j_new, int32_t k_new, int32_t i, 
     for (int k = 0; k < e->vlr_count[i]; ++k) {
       size_t eidx = ((size_t)it * g->N_r + ir) * g->N_t + e->vlr_t[i * VLR_CAP + k];
       double kConst2 = (e->I0[it] * (e->col_val[q] * e->vlr_val[i * VLR_CAP + k] * dx));
+      if (e->bfs) kConst2 *= e->bfs[eidx]; /* BF_Flag, ..._UpdateVoxels.cpp:438-446 */
       e->e[eidx] -= kConst2;
       if (e->p.bf_mode) /* selector refresh, BFForwardModel::updateErrorSinogram :924-929 */
         e->sel[eidx] = fabs(e->e[eidx] * sqrt(e->w[eidx])) < e->bragg ? 1 : 0;
@@ -884,6 +889,7 @@ int32_t orc_engine_joint_estimation(orc_engine* e)
     bf_joint_estimation(e);
     return 0;
   }
+  if (e->bfs) return 0; /* with bright-field data no gain/offset estimation, ..._Extra.cpp:429 */
   const orc_geom* g = &e->g;
   int NT = g->N_theta;
   size_t per = (size_t)g->N_r * g->N_t;
@@ -1136,6 +1142,20 @@ int32_t orc_engine_run(orc_engine* e, double* costs, int32_t cap, int32_t* n_cos
 
 void orc_engine_get_x(const orc_engine* e, double* x) { memcpy(x, e->x, sizeof(double) * vol_size(&e->g)); }
 void orc_engine_set_x(orc_engine* e, const double* x) { memcpy(e->x, x, sizeof(double) * vol_size(&e->g)); }
+
+/* The HAADF engine's bright-field sinogram (HAADF/HAADF_ReconstructionEngine_Extra.cpp:
+ * 89-175): s = counts + BF_OFFSET (HAADF/HAADFConstants.h:49) per element, then the
+ * error sinogram e = y - s Yest - mu (:718-727) */
+void orc_engine_set_bf_sinogram(orc_engine* e, const double* bf_counts)
+{
+  const orc_geom* g = &e->g;
+  size_t M = sino_size(g);
+  if (!e->bfs) e->bfs = (double*)malloc(sizeof(double) * M);
+  for (size_t i = 0; i < M; ++i) e->bfs[i] = bf_counts[i] + ORC_BF_OFFSET;
+  size_t per = (size_t)g->N_r * g->N_t;
+  for (size_t i = 0; i < M; ++i)
+    e->e[i] = e->counts[i] - e->bfs[i] * e->yest[i] - e->mu[i / per];
+}
 void orc_engine_get_e(const orc_engine* e, double* v) { memcpy(v, e->e, sizeof(double) * sino_size(&e->g)); }
 void orc_engine_get_w(const orc_engine* e, double* v) { memcpy(v, e->w, sizeof(double) * sino_size(&e->g)); }
 void orc_engine_get_yest(const orc_engine* e, double* v) { memcpy(v, e->yest, sizeof(double) * sino_size(&e->g)); }

@@ -67,6 +67,7 @@ typedef struct orc_params {
 
 #define ORC_BF_MAX 1.0            /* BrightField/BFConstants.h */
 #define ORC_DEF_BRAGG 5e10        /* DefBraggThreshold, BrightField/BFConstants.h */
+#define ORC_BF_OFFSET 32768.0     /* BF_OFFSET, HAADF/HAADFConstants.h:49 */
 
 /* ---- setup pieces (pure functions) ---- */
 int orc_geometry_init(int32_t N_r, int32_t N_t, int32_t N_theta, double delta_r, double delta_t,
@@ -121,6 +122,9 @@ int32_t orc_engine_run(orc_engine* e, double* costs, int32_t cap, int32_t* n_cos
 /* getters (copies) */
 void orc_engine_get_x(const orc_engine* e, double* x);
 void orc_engine_set_x(orc_engine* e, const double* x);
+/* HAADF with a bright-field sinogram (BF_Flag): projection entries scaled by
+ * bf_counts + ORC_BF_OFFSET per element, e re-derived, no gain/offset estimation */
+void orc_engine_set_bf_sinogram(orc_engine* e, const double* bf_counts);
 void orc_engine_get_e(const orc_engine* e, double* err);
 void orc_engine_get_w(const orc_engine* e, double* w);
 void orc_engine_get_yest(const orc_engine* e, double* yest);

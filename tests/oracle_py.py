@@ -88,6 +88,7 @@ def load():
         "orc_engine_run": (C.c_int32, [C.c_void_p, _D, C.c_int32, _I, _I]),
         "orc_engine_get_x": (None, [C.c_void_p, _D]),
         "orc_engine_set_x": (None, [C.c_void_p, _D]),
+        "orc_engine_set_bf_sinogram": (None, [C.c_void_p, _D]),
         "orc_engine_get_e": (None, [C.c_void_p, _D]),
         "orc_engine_get_w": (None, [C.c_void_p, _D]),
         "orc_engine_get_yest": (None, [C.c_void_p, _D]),
@@ -215,6 +216,12 @@ class Engine:
     def set_x(self, x):
         x = np.ascontiguousarray(x, np.float64)
         lib().orc_engine_set_x(self.h, dp(x))
+
+    def set_bf_sinogram(self, bf_counts):
+        """HAADF with a bright-field sinogram: entries scaled by counts + 32768."""
+        b = np.ascontiguousarray(bf_counts, np.float64).reshape(-1)
+        assert b.size == int(np.prod(self.sshape))
+        lib().orc_engine_set_bf_sinogram(self.h, dp(b))
 
     def e(self):
         return self._get(lib().orc_engine_get_e, self.sshape)

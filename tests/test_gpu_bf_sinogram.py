@@ -1,0 +1,83 @@
+"""GPU: HAADF with a bright-field sinogram (every projection entry scaled by its
+detector element's bright-field counts + 32768, the reference's BF_Flag) against the
+FP64 oracle, whose scaled path the CPU tests pin by identities."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from mbirtem import capi
+from problems import small_problem
+
+pytestmark = pytest.mark.gpu
+
+BF_OFFSET = 32768.0
+
+
+def _case(seed=12):
+    g, ang, vol, y0 = small_problem(seed=seed)
+    nt = len(ang)
+    rng = np.random.default_rng(seed)
+    # scales on a 1/256 grid: the raw counts s - 32768 are then exact in float32 (the
+    # problem carries them as float32; near 32768 its spacing is 1/256)
+    s = (205 + rng.integers(0, 103, (nt, g.N_r, g.N_t))) / 256.0
+    y = (s * y0.reshape(nt, g.N_r, g.N_t)).reshape(y0.shape)  # measurements of the scaled model
+    return g, ang, y, s - BF_OFFSET, s
+
+
+def _job(ctx, g, ang, y, **kw):
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, thickness=g.LengthZ / 2)
+    return capi.Job(ctx, geom, ang, np.asarray(y, np.float32), **kw)
+
+
+def test_scaled_setup_sweeps_and_convergence_match_oracle(gpu_ctx):
+    g, ang, y, bf, s = _case()
+    eng = O.Engine(g, ang, y, O.params(), default_recon=0.2)
+    eng.set_bf_sinogram(bf)
+    job = _job(gpu_ctx, g, ang, y, bf_sinogram=bf, default_recon=0.2)
+    er, eg = eng.e(), job.error_sinogram()
+    assert np.max(np.abs(eg - er)) <= 1e-5 * np.abs(er).max()
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+    costs = [job.cost()]
+    for it in range(60):
+        eng.sweep(0, it)
+        job.sweep(0, it)
+        if it < 8:
+            costs.append(job.cost())
+    for a, b in zip(costs, costs[1:]):
+        assert b <= a * (1 + 1e-6), costs
+    # e stays y - s A x - mu for the current volume (fp32 accuracy)
+    xg = job.volume()
+    e_ref = y.reshape(s.shape) - s * job.forward_project(xg).reshape(s.shape)
+    assert np.max(np.abs(job.error_sinogram().reshape(s.shape) - e_ref)) <= 2e-4 * np.abs(y).max()
+    xr = eng.x()
+    rmse = np.sqrt(np.mean((xr - xg) ** 2))
+    scale = np.sqrt(np.mean(xr ** 2))
+    assert rmse <= 1e-2 * scale, (rmse, scale)
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-4)
+
+
+def test_scaled_job_estimates_no_gains_or_offsets(gpu_ctx):
+    g, ang, y, bf, s = _case(seed=13)
+    nt = len(ang)
+    gains, offs = np.full(nt, 0.98), np.full(nt, 0.01)
+    job = _job(gpu_ctx, g, ang, y, bf_sinogram=bf, joint_estimation=1, num_outer=3, num_inner=4,
+               init_gain=gains, init_offset=offs, target_gain=10.0)
+    job.run()
+    I0, mu, al = job.nuisance()
+    np.testing.assert_array_equal(I0, gains)
+    np.testing.assert_array_equal(mu, offs)
+
+
+def test_scaled_jobs_are_single_job_only(gpu_ctx):
+    g, ang, y, bf, s = _case(seed=14)
+    job = _job(gpu_ctx, g, ang, y, bf_sinogram=bf)
+    with pytest.raises(capi.MbirError) as ei:
+        capi.batch_sweep(gpu_ctx, [job])
+    assert ei.value.code == -3
+    srv = capi.Server(gpu_ctx, max_jobs=2)
+    with pytest.raises(capi.MbirError) as ei:
+        srv.submit(capi.geometry_init(g.N_r, g.N_t, ang, thickness=g.LengthZ / 2), ang,
+                   np.asarray(y, np.float32), bf_sinogram=bf)
+    assert ei.value.code == -3
+    srv.close()
+    job.close()
