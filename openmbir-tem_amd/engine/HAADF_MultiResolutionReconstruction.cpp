@@ -142,6 +142,26 @@ void HAADF_MultiResolutionReconstruction::execute()
     eng->setGeometry(geom);
     eng->setAdvParams(AdvParams);
     eng->setForwardModel(fm);
+    if (!BrightFieldFile.empty()) {
+      // the bright-field tilt series with the level's selection
+      // (HAADF/HAADF_MultiResolutionReconstruction.cpp:189-203, 366-390, 421-422)
+      auto bf = std::make_shared<TomoInputs>();
+      HAADF_ReconstructionEngine::InitializeTomoInputs(bf);
+      bf->sinoFile = BrightFieldFile;
+      if (Subvolume.size() == 6) {
+        bf->useSubvolume = true;
+        bf->xStart = Subvolume[0];
+        bf->xEnd = Subvolume[3];
+        bf->yStart = Subvolume[1];
+        bf->yEnd = Subvolume[4];
+        bf->zStart = Subvolume[2];
+        bf->zEnd = Subvolume[5];
+      }
+      bf->interpolateFactor = T.interpolateFactor;
+      bf->excludedViews = ViewMasks;
+      eng->setBFTomoInputs(bf);
+      eng->setBFSinogram(std::make_shared<Sinogram>());
+    }
     if (dev && i > 0) {
       // what the level files would carry: the nuisance files exist only for the
       // estimated parameters (gains and offsets with joint estimation, variances

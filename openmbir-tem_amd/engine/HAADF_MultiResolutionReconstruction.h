@@ -26,6 +26,7 @@ class HAADF_MultiResolutionReconstruction {
   std::string TempDir;
   std::string OutputFile;                // MRC volume of the last level
   std::string InitialReconstructionFile; // optional .bin volume for the first level
+  std::string BrightFieldFile;           // optional bright-field tilt series (MRC)
   bool DeleteTempFiles = false;
   int NumberResolutions = 1;
   float SampleThickness = 0;

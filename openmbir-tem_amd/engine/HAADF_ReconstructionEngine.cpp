@@ -68,6 +68,62 @@ int HAADF_ReconstructionEngine::readInputData()
   return 0;
 }
 
+// The bright-field tilt series (HAADF/HAADF_ReconstructionEngine_Extra.cpp:85-175):
+// read with its own selection (the HAADF series' tilts), once more over the full
+// ranges if that fails; it must have the HAADF sinogram's sizes.  The counts stay raw:
+// the C-ABI adds BF_OFFSET.
+int HAADF_ReconstructionEngine::readBFData()
+{
+  m_BFFlag = false;
+  if (!m_BFTomoInputs || !m_BFSinogram || m_BFTomoInputs->sinoFile.empty()) return 0;
+  TomoInputs& B = *m_BFTomoInputs;
+  Sinogram& BS = *m_BFSinogram;
+  const Sinogram& S = *m_Sinogram;
+  const std::string& f = B.sinoFile;
+  const size_t dot = f.find_last_of('.');
+  const std::string ext = dot == std::string::npos ? std::string() : f.substr(dot + 1);
+  if (ext != "mrc" && ext != "ali") {
+    fail(-1, "A supported file reader for the Bright Field file was not found.");
+    return -1;
+  }
+  B.tilts = m_TomoInputs->tilts;
+  auto read = [&]() -> int {
+    io::TiltSeriesSelect sel;
+    sel.useSubvolume = B.useSubvolume;
+    sel.xStart = B.xStart; sel.xEnd = B.xEnd;
+    sel.yStart = B.yStart; sel.yEnd = B.yEnd;
+    sel.zStart = B.zStart; sel.zEnd = B.zEnd;
+    sel.interpolateFactor = B.interpolateFactor;
+    sel.excludedViews.assign(B.excludedViews.begin(), B.excludedViews.end());
+    sel.tilts = B.tilts;
+    io::TiltSeries ts;
+    const int rc = io::readMRCTiltSeries(f, sel, &ts);
+    if (rc < 0) return rc;
+    BS.N_theta = (uint16_t)ts.N_theta;
+    BS.N_r = (uint16_t)ts.N_r;
+    BS.N_t = (uint16_t)ts.N_t;
+    BS.counts = std::move(ts.counts);
+    BS.angles = std::move(ts.angles);
+    return 0;
+  };
+  if (read() < 0) {
+    // the sizes may not match: the full bright-field sinogram (:113-141)
+    B.xStart = 0; B.xEnd = (uint16_t)(S.N_r - 1);
+    B.yStart = 0; B.yEnd = (uint16_t)(S.N_t - 1);
+    B.zStart = 0; B.zEnd = (uint16_t)(S.N_theta - 1);
+    if (read() < 0) {
+      fail(-1, "Error reading Input Sinogram Data file");
+      return -1;
+    }
+  }
+  if (BS.N_r != S.N_r || BS.N_t != S.N_t || BS.N_theta != S.N_theta) {
+    fail(-1, "The two file sizes are not matched");
+    return -1;
+  }
+  m_BFFlag = true;
+  return 0;
+}
+
 // Initial gains, offsets and variances from the files named in TomoInputs, one double
 // per used view (IOFilters/GainsOffsetsReader.cpp:64-230): a file that cannot be
 // opened leaves the defaults, a short one is an error.
@@ -167,6 +223,7 @@ void HAADF_ReconstructionEngine::execute()
   const size_t M = (size_t)S.N_theta * S.N_r * S.N_t;
   if (S.N_theta == 0 || S.counts.size() != M || S.angles.size() != S.N_theta)
     return fail(-1, "Error reading Input Sinogram Data");
+  if (readBFData() < 0) return;  // the error is set
 
   mbir_geometry g{};
   int rc = mbir_geometry_init(S.N_r, S.N_t, S.N_theta, S.delta_r, S.delta_t, T.LengthZ,
@@ -226,6 +283,11 @@ void HAADF_ReconstructionEngine::execute()
   p.angles_deg = S.angles.data();
   p.counts = counts.data();
   p.init_volume = init.empty() ? nullptr : init.data();
+  std::vector<float> bfc;
+  if (m_BFFlag) {
+    bfc.assign(m_BFSinogram->counts.begin(), m_BFSinogram->counts.end());
+    p.bf_sinogram = bfc.data();
+  }
   p.default_recon = T.defaultInitialRecon;
   p.init_gain = opt(F.InitialGain);
   p.init_offset = T.useDefaultOffset ? nullptr : opt(F.InitialOffset);

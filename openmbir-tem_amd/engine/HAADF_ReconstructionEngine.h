@@ -117,6 +117,13 @@ class HAADF_ReconstructionEngine {
   void setAdvParams(AdvancedParametersPtr v) { m_AdvParams = v; }
   AdvancedParametersPtr getAdvParams() const { return m_AdvParams; }
   void setForwardModel(HAADF_ForwardModel::Pointer v) { m_ForwardModel = v; }
+  // Bright-field tilt series (HAADF/HAADF_ReconstructionEngine.h:75-84): when
+  // BFTomoInputs::sinoFile names one, every projection entry is scaled by its
+  // element's bright-field counts + BF_OFFSET and gains and offsets are not estimated
+  void setBFTomoInputs(TomoInputsPtr v) { m_BFTomoInputs = v; }
+  TomoInputsPtr getBFTomoInputs() const { return m_BFTomoInputs; }
+  void setBFSinogram(SinogramPtr v) { m_BFSinogram = v; }
+  SinogramPtr getBFSinogram() const { return m_BFSinogram; }
   HAADF_ForwardModel::Pointer getForwardModel() const { return m_ForwardModel; }
 
   // HIP device the engine runs on (default 0)
@@ -154,6 +161,7 @@ class HAADF_ReconstructionEngine {
   void fail(int code, const std::string& msg);
   int readInputData();
   int readNuisanceInputs();
+  int readBFData();
   void writeOutputs(mbir_job* job, const std::vector<float>& vol);
 
   TomoInputsPtr m_TomoInputs;
@@ -161,6 +169,9 @@ class HAADF_ReconstructionEngine {
   GeometryPtr m_Geometry;
   AdvancedParametersPtr m_AdvParams;
   HAADF_ForwardModel::Pointer m_ForwardModel;
+  TomoInputsPtr m_BFTomoInputs;
+  SinogramPtr m_BFSinogram;
+  bool m_BFFlag = false;
   int m_Device = 0;
   int m_ErrorCondition = 0;
   std::string m_ErrorMessage;

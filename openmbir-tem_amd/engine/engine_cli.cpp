@@ -8,10 +8,13 @@
 // Mu, Alpha) and <out_prefix>.txt (dimensions, sweeps, error code, costs).
 //
 //   engine_cli --mrc <tilts.mrc> thickness sigma_x p outer inner joint noise seed <out_dir>
+//              [bright_field.mrc]
 // reads the tilt series (tilt angles and pixel size from its FEI extended header) and
 // lets the engine write the reference's outputs into out_dir: recon.mrc, gains.bin,
 // offsets.bin, variances.bin, ReconstructedSinogram.bin, ReconstructedObject.bin, and
-// run.txt as above.
+// run.txt as above.  With a bright-field tilt series of the same size every projection
+// entry is scaled by its element's bright-field counts + 32768 (no gain or offset
+// estimation then).
 //
 //   engine_cli --multires <tilts.mrc> levels thickness sigma_x diffuseness outer inner
 //              stop seed <temp_dir> <out.mrc> [device_handover 0|1]
@@ -42,9 +45,9 @@ static std::vector<double> read_f64(const char* path, size_t n)
 
 static int run_mrc(int argc, char** argv)
 {
-  if (argc != 12) {
+  if (argc != 12 && argc != 13) {
     std::fprintf(stderr, "usage: %s --mrc tilts.mrc thickness sigma_x p outer inner joint noise "
-                         "seed out_dir\n", argv[0]);
+                         "seed out_dir [bright_field.mrc]\n", argv[0]);
     return 2;
   }
   const std::string dir = argv[11];
@@ -75,6 +78,15 @@ static int run_mrc(int argc, char** argv)
   eng->setGeometry(geom);
   eng->setAdvParams(adv);
   eng->setForwardModel(fm);
+  if (argc == 13) {  // bright-field tilt series with the same selection
+    auto bf = std::make_shared<TomoInputs>();
+    HAADF_ReconstructionEngine::InitializeTomoInputs(bf);
+    bf->sinoFile = argv[12];
+    bf->interpolateFactor = ti->interpolateFactor;
+    bf->excludedViews = ti->excludedViews;
+    eng->setBFTomoInputs(bf);
+    eng->setBFSinogram(std::make_shared<Sinogram>());
+  }
   eng->execute();
   std::ofstream txt(dir + "/run.txt");
   txt << std::setprecision(17);

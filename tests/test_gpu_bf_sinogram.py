@@ -81,3 +81,57 @@ def test_scaled_jobs_are_single_job_only(gpu_ctx):
     assert ei.value.code == -3
     srv.close()
     job.close()
+
+
+def test_adapter_reads_bright_field_tilt_series(gpu_ctx, tmp_path):
+    """The C++ adapter given a bright-field MRC beside the HAADF one (the reference's
+    BFTomoInputs / BFSinogram) must write exactly the C-ABI's volume for the same
+    problem with the counts as bf_sinogram; a bright-field file of another size or
+    type is an error."""
+    import os
+    import subprocess
+    from test_io import make_tilt_series, parse_mrc
+
+    cli = os.path.join(os.path.dirname(capi.LIB_PATH), "engine_cli")
+    g, ang, y, bf, s = _case(seed=15)
+    nv = len(ang)
+    ang32 = ang.astype(np.float32)
+    y32 = y.astype(np.float32).reshape(nv, g.N_r, g.N_t)
+    bf32 = bf.astype(np.float32)
+    assert np.array_equal(bf32.astype(np.float64), bf)  # the counts are exact in float32
+    mrc, bmrc = str(tmp_path / "tilts.mrc"), str(tmp_path / "bf.mrc")
+    make_tilt_series(mrc, y32.transpose(0, 2, 1), mode=2, tilts=ang32, pixel=1e-9)
+    make_tilt_series(bmrc, bf32.transpose(0, 2, 1), mode=2, tilts=ang32, pixel=1e-9)
+    d = float(np.float32(1e-9)) * 1e9
+    th = g.LengthZ / 2
+    args = [str(th), "0.1", "1.2", "4", "3", "1", "1", "7"]
+    out = str(tmp_path / "out")
+    os.makedirs(out)
+    r = subprocess.run([cli, "--mrc", mrc] + args + [out, bmrc], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    geom = capi.geometry_init(g.N_r, g.N_t, ang32.astype(np.float64), th, delta_r=d, delta_t=d)
+    job = capi.Job(gpu_ctx, geom, ang32.astype(np.float64), y32.astype(np.float64), seed=7,
+                   num_outer=4, num_inner=3, joint_estimation=1, noise_model=1,
+                   stop_threshold=1e-4, bf_sinogram=bf32)
+    job.run()
+    x = job.volume()
+    nz, nx, ny = x.shape
+    raw = np.fromfile(os.path.join(out, "ReconstructedObject.bin"), "<f8").reshape(ny, nx, nz)
+    np.testing.assert_array_equal(raw, x.transpose(2, 1, 0).astype(np.float64))
+    h, data = parse_mrc(os.path.join(out, "recon.mrc"))
+    assert h["nz"] == nz
+    I0, mu, _ = job.nuisance()
+    np.testing.assert_array_equal(np.fromfile(os.path.join(out, "gains.bin"), "<f8"), I0)
+    np.testing.assert_array_equal(np.fromfile(os.path.join(out, "offsets.bin"), "<f8"), mu)
+    job.close()
+    # a bright-field series with one view fewer, and one that is no MRC file
+    short = str(tmp_path / "short.mrc")
+    make_tilt_series(short, bf32[1:].transpose(0, 2, 1), mode=2, tilts=ang32[1:], pixel=1e-9)
+    for bad in (short, str(tmp_path / "bf.bin")):
+        o = str(tmp_path / ("bad_" + os.path.basename(bad)))
+        os.makedirs(o)
+        r = subprocess.run([cli, "--mrc", mrc] + args + [o, bad], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode != 0, bad
+        assert "error -1" in open(os.path.join(o, "run.txt")).read()
