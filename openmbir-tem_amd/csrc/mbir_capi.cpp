@@ -641,7 +641,7 @@ int run_sweep_steps(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
   for (int k = 0; k < nsteps; ++k) {
     const int b = j->step_ptr[k], n = j->step_ptr[k + 1] - b;
     HIP_TRY(launch_icd_step(j->J, j->sched.p + b, n, outer, inner, j->prob.zero_skipping,
-                            j->acc.p, s));
+                            j->acc.p, s, j->sched_h.data() + b));
   }
   if (j->timing) HIP_TRY(hipEventRecord(e1, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -724,7 +724,7 @@ int run_list_steps(mbir_job* j, const std::vector<int>& lines, int outer, int in
   for (int k = 0; k < nsteps; ++k) {
     const int b = j->step_ptr[k], n = j->step_ptr[k + 1] - b;
     HIP_TRY(launch_icd_step(j->J, j->sched.p + b, n, outer, inner, j->prob.zero_skipping,
-                            j->acc.p, s));
+                            j->acc.p, s, j->sched_h.data() + b));
   }
   return finish_sweep(j, 0.0, nullptr);
 }

@@ -28,6 +28,14 @@ struct LineRef {
   int pad;
 };
 
+// The lines of one step carried in the kernel arguments (single-job launches of at
+// most kInlineLines lines): they arrive with the job descriptor in the launch's first
+// memory round trip instead of one dependent load later.
+constexpr int kInlineLines = 128;
+struct StepLines {
+  LineRef l[kInlineLines];
+};
+
 struct SweepAccum {  // device accumulators of one sweep
   double sum_abs_update;
   double sum_abs_value;
@@ -149,8 +157,12 @@ hipError_t launch_init_error(const DevJob& J, const double* yest, hipStream_t s)
 // lines with line_nnz 0 stay 0
 hipError_t launch_upsample2(const float* coarse, float* fine, const int* line_nnz, int Nz, int Nx,
                             int Ny, hipStream_t s);
+// host_lines: the same nlines entries in host memory; steps of at most kInlineLines
+// lines then travel in the kernel arguments (not for graph capture, whose step slots
+// are rewritten in device memory between replays).
 hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, int outer, int inner,
-                           int zero_skipping, SweepAccum* acc, hipStream_t s);
+                           int zero_skipping, SweepAccum* acc, hipStream_t s,
+                           const LineRef* host_lines = nullptr);
 void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv);
 // one step of several jobs of one kernel shape: entries [e0, e0 + ne) of ent,
 // nwg workgroups in total

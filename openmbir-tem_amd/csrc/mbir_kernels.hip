@@ -13,6 +13,7 @@
 // into LDS; the per-voxel VLR weights are applied from registers.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <algorithm>
 
 #include "mbir_internal.h"
@@ -1047,14 +1048,23 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   if constexpr (!PF) line_stats();
 }
 
-template <int YPL, int NV, bool FW = false, bool SB = false>
+// INL: the step's lines come in the kernel arguments (`inl`, exactly one per
+// workgroup, no padding), so no branch precedes the argument loads and the line's
+// entry arrives with the job descriptor.
+template <int YPL, int NV, bool FW = false, bool SB = false, bool INL = false>
 __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef* lines, int nlines,
-                                                        int first, int zs_on, SweepAccum* acc)
+                                                        int first, int zs_on, SweepAccum* acc,
+                                                        StepLines inl)
 {
   __shared__ IcdSmem<YPL> sm;
-  if ((int)blockIdx.x >= nlines) return;
-  const LineRef ref = lines[blockIdx.x];
-  if (__builtin_amdgcn_readfirstlane(ref.line) < 0) return;  // padding of a graph's step slot
+  LineRef ref;
+  if constexpr (INL) {
+    ref = inl.l[blockIdx.x];
+  } else {
+    if ((int)blockIdx.x >= nlines) return;
+    ref = lines[blockIdx.x];
+    if (__builtin_amdgcn_readfirstlane(ref.line) < 0) return;  // padding of a graph's step slot
+  }
   icd_line<YPL, NV, FW, true, SB>(J, ref, first, zs_on, acc, sm, icd_col(J.col_slots));
 }
 
@@ -1524,44 +1534,58 @@ void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv)
   *nv = max_vlr <= 1 ? 1 : (max_vlr == 2 ? 2 : 4);
 }
 
-template <int YPL>
+template <int YPL, bool INL>
 static void launch_step_nv(const DevJob& J, const LineRef* lines, int nlines, int first, int zs,
-                           SweepAccum* acc, hipStream_t s)
+                           SweepAccum* acc, hipStream_t s, const StepLines& inl)
 {
   const size_t dyn = icd_col_bytes(J.col_slots);
   if (J.wfull)  // bright field: per-entry weights and selector (one row per slice)
-    hipLaunchKernelGGL((k_icd_step<YPL, 1, true>), dim3(nlines), dim3(kIcdBlock), dyn, s, J, lines,
-                       nlines, first, zs, acc);
+    hipLaunchKernelGGL((k_icd_step<YPL, 1, true, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
+                       s, J, lines, nlines, first, zs, acc, inl);
   else if (J.bfs && J.vlr_max <= 1)  // HAADF with a bright-field sinogram
-    hipLaunchKernelGGL((k_icd_step<YPL, 1, false, true>), dim3(nlines), dim3(kIcdBlock), dyn, s, J,
-                       lines, nlines, first, zs, acc);
+    hipLaunchKernelGGL((k_icd_step<YPL, 1, false, true, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
+                       s, J, lines, nlines, first, zs, acc, inl);
   else if (J.bfs)
-    hipLaunchKernelGGL((k_icd_step<YPL, 2, false, true>), dim3(nlines), dim3(kIcdBlock), dyn, s, J,
-                       lines, nlines, first, zs, acc);
+    hipLaunchKernelGGL((k_icd_step<YPL, 2, false, true, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
+                       s, J, lines, nlines, first, zs, acc, inl);
   else if (J.vlr_max <= 1)
-    hipLaunchKernelGGL((k_icd_step<YPL, 1>), dim3(nlines), dim3(kIcdBlock), dyn, s, J, lines, nlines,
-                       first, zs, acc);
+    hipLaunchKernelGGL((k_icd_step<YPL, 1, false, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
+                       s, J, lines, nlines, first, zs, acc, inl);
   else if (J.vlr_max == 2)
-    hipLaunchKernelGGL((k_icd_step<YPL, 2>), dim3(nlines), dim3(kIcdBlock), dyn, s, J, lines, nlines,
-                       first, zs, acc);
+    hipLaunchKernelGGL((k_icd_step<YPL, 2, false, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
+                       s, J, lines, nlines, first, zs, acc, inl);
   else
-    hipLaunchKernelGGL((k_icd_step<YPL, 4>), dim3(nlines), dim3(kIcdBlock), dyn, s, J, lines, nlines,
-                       first, zs, acc);
+    hipLaunchKernelGGL((k_icd_step<YPL, 4, false, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
+                       s, J, lines, nlines, first, zs, acc, inl);
 }
 
-hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, int outer, int inner,
-                           int zero_skipping, SweepAccum* acc, hipStream_t s)
+template <bool INL>
+static void launch_step_ypl(const DevJob& J, const LineRef* lines, int nlines, int first, int zs,
+                            SweepAccum* acc, hipStream_t s, const StepLines& inl)
 {
-  if (nlines <= 0) return hipSuccess;
-  const int first = (outer == 0 && inner == 0) ? 1 : 0;
   int ypl, nv;
   icd_shape(J, J.vlr_max, &ypl, &nv);
   if (ypl == 1)
-    launch_step_nv<1>(J, lines, nlines, first, zero_skipping, acc, s);
+    launch_step_nv<1, INL>(J, lines, nlines, first, zs, acc, s, inl);
   else if (ypl == 2)
-    launch_step_nv<2>(J, lines, nlines, first, zero_skipping, acc, s);
+    launch_step_nv<2, INL>(J, lines, nlines, first, zs, acc, s, inl);
   else
-    launch_step_nv<4>(J, lines, nlines, first, zero_skipping, acc, s);
+    launch_step_nv<4, INL>(J, lines, nlines, first, zs, acc, s, inl);
+}
+
+hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, int outer, int inner,
+                           int zero_skipping, SweepAccum* acc, hipStream_t s,
+                           const LineRef* host_lines)
+{
+  if (nlines <= 0) return hipSuccess;
+  const int first = (outer == 0 && inner == 0) ? 1 : 0;
+  StepLines inl;
+  if (host_lines && nlines <= kInlineLines) {
+    std::memcpy(inl.l, host_lines, (size_t)nlines * sizeof(LineRef));
+    launch_step_ypl<true>(J, lines, nlines, first, zero_skipping, acc, s, inl);
+  } else {
+    launch_step_ypl<false>(J, lines, nlines, first, zero_skipping, acc, s, inl);
+  }
   return hipGetLastError();
 }
 
