@@ -1060,6 +1060,12 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef*
   LineRef ref;
   if constexpr (INL) {
     ref = inl.l[blockIdx.x];
+    // the arguments the line's first loads depend on, fetched in the same round trip
+    // as its entry (an empty asm that takes them as operands: no instruction)
+    asm volatile(";" ::"s"(ref.line), "s"(ref.p0), "s"(ref.nc), "s"(J.N_x), "s"(J.N_y), "s"(J.NB),
+                 "s"(J.N_z), "s"(J.mask), "s"(J.vlr_t0), "s"(J.vlr_n), "s"(J.col_row),
+                 "s"(J.col_ka), "s"(J.col_kw), "s"(J.x), "s"(J.e), "s"(J.col_slots),
+                 "s"(J.spare_b));
   } else {
     if ((int)blockIdx.x >= nlines) return;
     ref = lines[blockIdx.x];
