@@ -36,6 +36,9 @@ SOURCES = [
     # the engine's files (MRC tilt series in, MRC volume and binary outputs out)
     (os.path.join(ENGINE, "MBIRIO.cpp"), ["-x", "c++"]),
     (os.path.join(ENGINE, "HAADF_MultiResolutionReconstruction.cpp"), ["-x", "c++"]),
+    # bright-field engine and driver on the same adapter
+    (os.path.join(ENGINE, "BFReconstructionEngine.cpp"), ["-x", "c++"]),
+    (os.path.join(ENGINE, "BFMultiResolutionReconstruction.cpp"), ["-x", "c++"]),
     (os.path.join(ENGINE, "mbir_io_capi.cpp"), ["-x", "c++"]),
     # initial-parameter estimators (per-view reductions on the host)
     (os.path.join(ENGINE, "mbir_estimate.cpp"), ["-x", "c++"]),

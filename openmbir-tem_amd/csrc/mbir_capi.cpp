@@ -342,7 +342,7 @@ int setup_job(mbir_job* j)
   std::vector<float> bf_y, bf_w;
   if (P.bf_mode) {
     for (int c : vn)
-      if (c > 1) return MBIR_ERR_UNSUPPORTED;  // one detector row per slice
+      if (c > 2) return MBIR_ERR_UNSUPPORTED;  // at most two detector rows per slice
     if (P.schedule != 0) return MBIR_ERR_UNSUPPORTED;
     bf_y.resize(M);
     bf_w.resize(M);

@@ -1545,8 +1545,11 @@ static void launch_step_nv(const DevJob& J, const LineRef* lines, int nlines, in
                            SweepAccum* acc, hipStream_t s, const StepLines& inl)
 {
   const size_t dyn = icd_col_bytes(J.col_slots);
-  if (J.wfull)  // bright field: per-entry weights and selector (one row per slice)
+  if (J.wfull && J.vlr_max <= 1)  // bright field: per-entry weights and selector
     hipLaunchKernelGGL((k_icd_step<YPL, 1, true, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
+                       s, J, lines, nlines, first, zs, acc, inl);
+  else if (J.wfull)  // bright field, two detector rows per slice (coarse voxels)
+    hipLaunchKernelGGL((k_icd_step<YPL, 2, true, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
                        s, J, lines, nlines, first, zs, acc, inl);
   else if (J.bfs && J.vlr_max <= 1)  // HAADF with a bright-field sinogram
     hipLaunchKernelGGL((k_icd_step<YPL, 1, false, true, INL>), dim3(nlines), dim3(kIcdBlock), dyn,

@@ -18,6 +18,18 @@ void HAADF_MultiResolutionReconstruction::setCancel(bool v)
   if (e) e->setCancel(v);
 }
 
+HAADF_ReconstructionEngine::Pointer HAADF_MultiResolutionReconstruction::makeEngine(
+    int level, TomoInputs& T, HAADF_ForwardModel::Pointer* fm)
+{
+  (void)level;
+  (void)T;
+  *fm = HAADF_ForwardModel::New();
+  (*fm)->TargetGain = TargetGain;
+  auto eng = HAADF_ReconstructionEngine::New();
+  eng->setForwardModel(*fm);
+  return eng;
+}
+
 void HAADF_MultiResolutionReconstruction::execute()
 {
   m_ErrorCondition = 0;
@@ -96,7 +108,7 @@ void HAADF_MultiResolutionReconstruction::execute()
       T.varianceOutputFile = T.tempDir + "/FinalVariances.bin";
       for (const char* f : {"/FinalGainParameters.bin", "/FinalOffsetParameters.bin",
                             "/FinalVariances.bin", "/ReconstructedSinogram.bin",
-                            "/ReconstructedObject.bin"})
+                            "/ReconstructedObject.bin", "/BraggSelector.mrc"})
         temps.push_back(T.tempDir + f);
     }
     // iterations, prior and stop rule per level (:325-349)
@@ -134,14 +146,12 @@ void HAADF_MultiResolutionReconstruction::execute()
     sino->delta_r = DefaultPixelSize;  // replaced by the FEI pixel size when present
     sino->delta_t = DefaultPixelSize;
     auto geom = std::make_shared<Geometry>();
-    auto fm = HAADF_ForwardModel::New();
-    fm->TargetGain = TargetGain;
-    auto eng = HAADF_ReconstructionEngine::New();
+    HAADF_ForwardModel::Pointer fm;
+    auto eng = makeEngine(i, T, &fm);
     eng->setTomoInputs(in);
     eng->setSinogram(sino);
     eng->setGeometry(geom);
     eng->setAdvParams(AdvParams);
-    eng->setForwardModel(fm);
     if (!BrightFieldFile.empty()) {
       // the bright-field tilt series with the level's selection
       // (HAADF/HAADF_MultiResolutionReconstruction.cpp:189-203, 366-390, 421-422)

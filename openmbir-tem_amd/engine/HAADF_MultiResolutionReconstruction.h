@@ -20,6 +20,7 @@ class HAADF_MultiResolutionReconstruction {
  public:
   using Pointer = std::shared_ptr<HAADF_MultiResolutionReconstruction>;
   static Pointer New() { return Pointer(new HAADF_MultiResolutionReconstruction()); }
+  virtual ~HAADF_MultiResolutionReconstruction() = default;
 
   // properties of the reference driver (HAADF_MultiResolutionReconstruction.h:66-110)
   std::string InputFile;                 // MRC tilt series
@@ -56,7 +57,7 @@ class HAADF_MultiResolutionReconstruction {
   bool DeviceHandover = false;
 
   // Runs every level; 0 on success, else the failing level's error code.
-  void execute();
+  virtual void execute();
   int getErrorCondition() const { return m_ErrorCondition; }
   const std::string& getErrorMessage() const { return m_ErrorMessage; }
   void setCancel(bool v);
@@ -69,10 +70,14 @@ class HAADF_MultiResolutionReconstruction {
 
  protected:
   HAADF_MultiResolutionReconstruction() = default;
-
- private:
+  // The engine of level `level` with its forward model (*fm), given the level's
+  // inputs T (the bright-field driver makes a bright-field engine here).
+  virtual HAADF_ReconstructionEngine::Pointer makeEngine(int level, TomoInputs& T,
+                                                         HAADF_ForwardModel::Pointer* fm);
   int m_ErrorCondition = 0;
   std::string m_ErrorMessage;
+
+ private:
   std::atomic<bool> m_Cancel{false};
   HAADF_ReconstructionEngine::Pointer m_Current;
   std::vector<TomoInputs> m_LevelInputs;

@@ -173,8 +173,10 @@ void HAADF_ReconstructionEngine::writeOutputs(mbir_job* job, const std::vector<f
     const size_t M = (size_t)S.N_theta * S.N_r * S.N_t;
     std::vector<float> e(M);
     if (mbir_job_get_error_sinogram(job, e.data()) < 0) return fail(-100, "error sinogram readback failed");
+    std::vector<double> y;
+    measurement(y);
     note(io::writeReconstructedSinogram(T.tempDir + "/ReconstructedSinogram.bin", S.N_theta, S.N_r,
-                                        S.N_t, S.counts.data(), e.data(), F.I_0.data(),
+                                        S.N_t, y.data(), e.data(), F.I_0.data(),
                                         F.Mu.data()),
          "ReconstructedSinogram.bin");
     note(io::writeRawVolume(T.tempDir + "/ReconstructedObject.bin", vol.data(), G.N_z, G.N_x, G.N_y),
@@ -259,11 +261,13 @@ void HAADF_ReconstructionEngine::execute()
       return fail(-1, "Error reading Initial Reconstruction Data from File");
   }
   if (readNuisanceInputs() < 0) return fail(-1, "Error Reading Gains from File");
-  if (!T.useDefaultOffset && m_ForwardModel->InitialOffset.size() != S.N_theta) {
+  if (!useDefaultOffset() && m_ForwardModel->InitialOffset.size() != S.N_theta) {
     // offsets neither given nor defaulted: least-squares fit of the view means
     // (HAADF/HAADF_ForwardModel.cpp:327-345, GenericFilters/ComputeInitialOffsets.cpp)
+    std::vector<double> y;
+    measurement(y);
     m_ForwardModel->InitialOffset.assign(S.N_theta, 0.0);
-    if (mbir_initial_offsets(S.counts.data(), S.N_theta, S.N_r, S.N_t, S.angles.data(),
+    if (mbir_initial_offsets(y.data(), S.N_theta, S.N_r, S.N_t, S.angles.data(),
                              m_ForwardModel->InitialOffset.data(), nullptr) < 0)
       return fail(-1, "Error initializing Input Offsets Data file");
   }
@@ -290,7 +294,7 @@ void HAADF_ReconstructionEngine::execute()
   }
   p.default_recon = T.defaultInitialRecon;
   p.init_gain = opt(F.InitialGain);
-  p.init_offset = T.useDefaultOffset ? nullptr : opt(F.InitialOffset);
+  p.init_offset = useDefaultOffset() ? nullptr : opt(F.InitialOffset);
   p.init_variance = opt(F.InitialVariance);
   p.target_gain = T.targetGain;
   p.default_offset = T.defaultOffset;
@@ -308,6 +312,7 @@ void HAADF_ReconstructionEngine::execute()
   p.schedule = 0;
   p.rows_per_step = 1;
   p.kernel_mode = 0;
+  if (configureProblem(p) < 0) return;  // the error is set
 
   mbir_opts o{};
   o.device = m_Device;
@@ -348,6 +353,7 @@ void HAADF_ReconstructionEngine::execute()
     } else {
       G.Object.assign(vol.begin(), vol.end());
       writeOutputs(job, vol);
+      if (m_ErrorCondition == 0) writeExtraOutputs(job);
     }
   }
   {

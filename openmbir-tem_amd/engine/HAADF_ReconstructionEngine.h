@@ -77,6 +77,7 @@ struct TomoInputs {
   std::string tempDir;                  // ReconstructedSinogram.bin, ReconstructedObject.bin
   std::string gainsOutputFile, offsetsOutputFile, varianceOutputFile;
   std::string mrcOutputFile;
+  std::string braggSelectorFile;        // bright field: the final Bragg selector (MRC)
   std::vector<uint8_t> excludedViews;   // file sections left out
   std::vector<int> goodViews;           // file sections used (set when read from a file)
 };
@@ -106,7 +107,7 @@ class HAADF_ReconstructionEngine {
  public:
   using Pointer = std::shared_ptr<HAADF_ReconstructionEngine>;
   static Pointer New() { return Pointer(new HAADF_ReconstructionEngine()); }
-  ~HAADF_ReconstructionEngine();
+  virtual ~HAADF_ReconstructionEngine();
 
   void setTomoInputs(TomoInputsPtr v) { m_TomoInputs = v; }
   TomoInputsPtr getTomoInputs() const { return m_TomoInputs; }
@@ -156,9 +157,21 @@ class HAADF_ReconstructionEngine {
 
  protected:
   HAADF_ReconstructionEngine() = default;
+  void fail(int code, const std::string& msg);
+
+  // Hooks of the bright-field engine (BFReconstructionEngine), which runs the same
+  // setup, loop and outputs with another data term:
+  // the filled problem before the job is created (< 0: failed, the error is set)
+  virtual int configureProblem(mbir_problem& p) { (void)p; return 0; }
+  // the measurement the model fits, [N_theta][N_r][N_t] (the counts here): the
+  // initial-offset fit and ReconstructedSinogram.bin use it
+  virtual void measurement(std::vector<double>& y) const { y = m_Sinogram->counts; }
+  // offsets neither from a file nor estimated: the default offset (else fitted)
+  virtual bool useDefaultOffset() const { return m_TomoInputs->useDefaultOffset; }
+  // files beyond the engine's common outputs (after they are written)
+  virtual void writeExtraOutputs(mbir_job* job) { (void)job; }
 
  private:
-  void fail(int code, const std::string& msg);
   int readInputData();
   int readNuisanceInputs();
   int readBFData();

@@ -22,6 +22,12 @@
 // hand over through files, or on the device with device_handover 1) and writes
 // <temp_dir>/levels.txt: one line per level with its voxel size multiple, sigma_x, q,
 // stop threshold, inner iterations, interpolation flag, geometry and sweeps.
+//
+//   engine_cli --bf-multires <tilts.mrc> levels thickness sigma_x diffuseness outer inner
+//              stop seed <temp_dir> <out.mrc> bragg_T bragg_delta bf_offset default_offset
+//              use_default_offset
+// the same with BFMultiResolutionReconstruction (transmitted counts, bright-field
+// engines; each level also writes BraggSelector.mrc).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -29,6 +35,7 @@
 #include <string>
 #include <vector>
 
+#include "BFMultiResolutionReconstruction.h"
 #include "HAADF_MultiResolutionReconstruction.h"
 #include "HAADF_ReconstructionEngine.h"
 
@@ -99,14 +106,28 @@ static int run_mrc(int argc, char** argv)
   return eng->getErrorCondition() == 0 ? 0 : 1;
 }
 
-static int run_multires(int argc, char** argv)
+static int run_multires(int argc, char** argv, bool bf)
 {
-  if (argc != 13 && argc != 14) {
+  if (bf ? argc != 18 : (argc != 13 && argc != 14)) {
     std::fprintf(stderr, "usage: %s --multires tilts.mrc levels thickness sigma_x diffuseness "
-                         "outer inner stop seed temp_dir out.mrc [device_handover 0|1]\n", argv[0]);
+                         "outer inner stop seed temp_dir out.mrc [device_handover 0|1]\n"
+                         "       %s --bf-multires tilts.mrc levels thickness sigma_x diffuseness "
+                         "outer inner stop seed temp_dir out.mrc bragg_T bragg_delta bf_offset "
+                         "default_offset use_default_offset\n", argv[0], argv[0]);
     return 2;
   }
-  auto mr = HAADF_MultiResolutionReconstruction::New();
+  HAADF_MultiResolutionReconstruction::Pointer mr;
+  if (bf) {
+    auto b = BFMultiResolutionReconstruction::New();
+    b->BraggThreshold = (float)std::atof(argv[13]);
+    b->BraggDelta = (float)std::atof(argv[14]);
+    b->BfOffset = (float)std::atof(argv[15]);
+    b->DefaultOffsetValue = (float)std::atof(argv[16]);
+    b->UseDefaultOffset = std::atoi(argv[17]) != 0;
+    mr = b;
+  } else {
+    mr = HAADF_MultiResolutionReconstruction::New();
+  }
   mr->InputFile = argv[2];
   mr->NumberResolutions = std::atoi(argv[3]);
   mr->SampleThickness = (float)std::atof(argv[4]);
@@ -118,7 +139,7 @@ static int run_multires(int argc, char** argv)
   mr->Seed = (uint32_t)std::atoi(argv[10]);
   mr->TempDir = argv[11];
   mr->OutputFile = argv[12];
-  mr->DeviceHandover = argc == 14 && std::atoi(argv[13]) != 0;
+  mr->DeviceHandover = !bf && argc == 14 && std::atoi(argv[13]) != 0;
   mr->execute();
   std::ofstream txt(std::string(argv[11]) + "/levels.txt");
   txt << std::setprecision(17);
@@ -139,7 +160,8 @@ static int run_multires(int argc, char** argv)
 int main(int argc, char** argv)
 {
   if (argc > 1 && std::string(argv[1]) == "--mrc") return run_mrc(argc, argv);
-  if (argc > 1 && std::string(argv[1]) == "--multires") return run_multires(argc, argv);
+  if (argc > 1 && std::string(argv[1]) == "--multires") return run_multires(argc, argv, false);
+  if (argc > 1 && std::string(argv[1]) == "--bf-multires") return run_multires(argc, argv, true);
   if (argc != 15) {
     std::fprintf(stderr, "usage: %s sino.f64 angles.f64 N_r N_t N_theta thickness sigma_x p "
                          "outer inner joint noise seed out_prefix\n", argv[0]);

@@ -126,11 +126,12 @@ def dp(a):
     return a.ctypes.data_as(_D)
 
 
-def geometry(N_r, N_t, angles_deg, thickness, delta=1.0, extend_object=0):
+def geometry(N_r, N_t, angles_deg, thickness, delta=1.0, extend_object=0, delta_xz=1.0,
+             interpolate_factor=1.0):
     g = Geom()
     ang = np.ascontiguousarray(angles_deg, dtype=np.float64)
-    rc = lib().orc_geometry_init(N_r, N_t, len(ang), delta, delta, thickness, 1.0, 1.0,
-                                 extend_object, dp(ang), C.byref(g))
+    rc = lib().orc_geometry_init(N_r, N_t, len(ang), delta, delta, thickness, delta_xz,
+                                 interpolate_factor, extend_object, dp(ang), C.byref(g))
     assert rc == 0
     return g
 

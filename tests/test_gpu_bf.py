@@ -220,3 +220,54 @@ def test_bf_jobs_in_a_batched_sweep_equal_alone(gpu_ctx):
             np.testing.assert_array_equal(b.error_sinogram(), a.error_sinogram())
     for j in alone + batched:
         j.close()
+
+
+def test_bf_two_rows_per_slice_matches_oracle(gpu_ctx):
+    """Voxels twice the detector pitch (a coarse level of the multi-resolution
+    driver): every slice spans two detector rows.  Setup against the oracle, then
+    sweeps: the cost never rises, e stays y - A x - mu for the GPU's own volume, and
+    after 60 sweeps the volume and cost lie within 3 of the oracle's own scatters over
+    update orders (seeds) of their means: the Bragg rejector makes the data term
+    non-convex, so orders settle slightly apart."""
+    _, ang, _, counts = bf_problem(n_t=8, bragg_frac=0.05)
+    n_r, n_t = counts.shape[1], counts.shape[2]
+    g = O.geometry(n_r, n_t, ang, thickness=8, delta_xz=2.0, interpolate_factor=2.0)
+    offs = np.full(len(ang), -np.log(2000.0))
+    prm = dict(BF, zero_skipping=0)
+    eng = O.Engine(g, ang, counts, O.params(**prm), offsets=offs)
+    geom = capi.geometry_init(n_r, n_t, ang, g.LengthZ / 2, delta_xz_in=2.0,
+                              interpolate_factor=2.0)
+    assert (geom.N_x, geom.N_y, geom.N_z) == (g.N_x, g.N_y, g.N_z)
+    assert geom.N_y * 2 == n_t
+    job = capi.Job(gpu_ctx, geom, ang, counts.astype(np.float32), init_offset=offs, **prm)
+    ref_e = eng.e()
+    assert np.max(np.abs(job.error_sinogram() - ref_e)) <= 1e-5 * np.max(np.abs(ref_e))
+    np.testing.assert_allclose(job.weights(), eng.w(), rtol=1e-6)
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+    costs = [job.cost()]
+    for it in range(60):
+        eng.sweep(0, it)
+        job.sweep(0, it)
+        if it < 8:
+            costs.append(job.cost())
+    assert all(b <= a * (1 + 1e-6) + 1e-9 for a, b in zip(costs, costs[1:])), costs
+    y = -np.log(counts)
+    e_ref = y - eng.project(job.volume().astype(np.float64)) - offs[:, None, None]
+    assert np.max(np.abs(job.error_sinogram() - e_ref)) <= 2e-5 * np.max(np.abs(e_ref))
+    runs = [(eng.x(), eng.cost())]
+    for seed in (1, 2, 3):
+        e2 = O.Engine(g, ang, counts, O.params(**dict(prm, seed=seed)), offsets=offs)
+        for it in range(60):
+            e2.sweep(0, it)
+        runs.append((e2.x(), e2.cost()))
+
+    def rms(a):
+        return np.sqrt(np.mean(a ** 2))
+
+    xbar = np.mean([r[0] for r in runs], axis=0)
+    spread = max(rms(r[0] - xbar) for r in runs)
+    xg = job.volume().astype(np.float64)
+    assert rms(xg - xbar) <= max(3.0 * spread, 1e-2 * rms(xbar)), (rms(xg - xbar), spread)
+    cbar = np.mean([r[1] for r in runs])
+    cspread = max(abs(r[1] - cbar) for r in runs)
+    assert abs(job.cost() - cbar) <= max(3.0 * cspread, 1e-5 * abs(cbar)), (job.cost(), cbar, cspread)
