@@ -1588,12 +1588,13 @@ hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, in
 {
   if (nlines <= 0) return hipSuccess;
   const int first = (outer == 0 && inner == 0) ? 1 : 0;
-  StepLines inl;
   if (host_lines && nlines <= kInlineLines) {
+    StepLines inl;  // entries past nlines are never read
     std::memcpy(inl.l, host_lines, (size_t)nlines * sizeof(LineRef));
     launch_step_ypl<true>(J, lines, nlines, first, zero_skipping, acc, s, inl);
   } else {
-    launch_step_ypl<false>(J, lines, nlines, first, zero_skipping, acc, s, inl);
+    static const StepLines none{};
+    launch_step_ypl<false>(J, lines, nlines, first, zero_skipping, acc, s, none);
   }
   return hipGetLastError();
 }
