@@ -18,16 +18,16 @@ HAADF_ReconstructionEngine::Pointer BFMultiResolutionReconstruction::makeEngine(
     int level, TomoInputs& T, HAADF_ForwardModel::Pointer* fm)
 {
   auto f = BFForwardModel::New();
-  f->TargetGain = TargetGain;  // :413
-  f->BraggDelta = BraggDelta;  // :414-416
+  f->TargetGain = TargetGain;  // :355
+  f->BraggDelta = BraggDelta;  // :357-359
   f->BraggThreshold = BraggThreshold;
   f->BfOffset = BfOffset;
-  if (level == 0) {  // :367-370, 403-407
+  if (level == 0) {  // :342-353
     f->DefaultVariance = DefaultVariance;
     f->DefaultOffset = DefaultOffsetValue;
     f->UseDefaultOffset = UseDefaultOffset;
   }
-  if (!T.tempDir.empty()) T.braggSelectorFile = T.tempDir + "/BraggSelector.mrc";  // :276-279
+  if (!T.tempDir.empty()) T.braggSelectorFile = T.tempDir + "/BraggSelector.mrc";  // :282-285
   auto eng = BFReconstructionEngine::New();
   eng->setForwardModel(f);
   *fm = f;
