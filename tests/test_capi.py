@@ -35,3 +35,13 @@ def test_host_only_entry_points():
     with pytest.raises(capi.MbirError) as ei:
         capi.geometry_init(0, 4, np.zeros(3), 8)
     assert ei.value.code == -1
+
+
+def test_engine_cli_rejects_wrong_argument_counts():
+    """The CLI checks its arguments before any device call (runs without a GPU)."""
+    import subprocess
+    cli = os.path.join(os.path.dirname(capi.LIB_PATH), "engine_cli")
+    for mode in ("--mrc", "--multires", "--bf-multires"):
+        r = subprocess.run([cli, mode, "x"], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2, (mode, r.stderr)
+        assert mode in r.stderr
