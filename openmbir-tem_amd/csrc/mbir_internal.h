@@ -31,7 +31,7 @@ struct LineRef {
 // The lines of one step carried in the kernel arguments (single-job launches of at
 // most kInlineLines lines): they arrive with the job descriptor in the launch's first
 // memory round trip instead of one dependent load later.
-constexpr int kInlineLines = 128;
+constexpr int kInlineLines = 192;
 struct StepLines {
   LineRef l[kInlineLines];
 };
