@@ -128,10 +128,17 @@ typedef struct mbir_problem {
    * BF_Flag, HAADF/HAADF_ReconstructionEngine_Extra.cpp:89-175): optional raw
    * bright-field counts [N_theta][N_r][N_t]; every projection entry is scaled by
    * counts + BF_OFFSET (32768, HAADF/HAADFConstants.h:49) of its detector element, the
-   * initial error is y - s Yest - mu (:718-727), and gains and offsets are not
-   * estimated (:429).  Single-job paths only (mbir_job_run, mbir_job_sweep); not with
+   * initial error is y - s Yest - mu (:718-727), and joint_estimation estimates the
+   * offsets only: per view the weighted mean of e moves from e into mu, the gains stay
+   * (:636-669).  Single-job paths only (mbir_job_run, mbir_job_sweep); not with
    * bf_mode.  NULL = none. */
   const float* bf_sinogram;
+  /* AdvancedParameters::ESTIMATE_PRIOR (HAADF/HAADF_ReconstructionEngine.cpp:257, 807-813):
+   * 1 = after every outer iteration that does not stop the run, re-estimate sigma_x from
+   * the volume (mbir_job_estimate_prior).  HAADF single-job runs only (the server refuses
+   * such jobs with -3). */
+  int32_t estimate_prior;
+  int32_t pad2_;
 } mbir_problem;
 
 /* Per-sweep statistics (updateVoxels, HAADF_ReconstructionEngine_UpdateVoxels.cpp:717-786) */
@@ -226,6 +233,17 @@ int32_t mbir_job_nuisance_update(mbir_job* job);
 /* computeCost (HAADF/HAADF_ReconstructionEngine.cpp:1147-1364) in FP64 on the device.
  * terms3 (may be NULL) receives {data, prior, noise}. */
 int32_t mbir_job_cost(mbir_job* job, double* total, double* terms3);
+
+/* estimateSigmaX (HAADF/HAADF_ReconstructionEngine.cpp:1827-1945) then updatePriorModel
+ * (HAADF/HAADF_QGGMRFPriorModel.cpp:24-31), FP64 on the device: estimate = the QGGMRF
+ * value of the 13 forward neighbour pairs of every voxel in the planes
+ * [(int) 0.25 N_z, (int) 0.75 N_z), differences scaled by the current sigma_x, divided by
+ * N_x N_y (zEnd - zStart + 1); the new sigma_x = 5 (estimate q)^(1/q) replaces the prior's
+ * constants for the following sweeps.  estimate and sigma_x (may be NULL) receive both.
+ * -3 for bright-field jobs, -1 when the volume gives no scale (all zero). */
+int32_t mbir_job_estimate_prior(mbir_job* job, double* estimate, double* sigma_x);
+/* The prior scale the next sweep uses (TomoInputs::SigmaX). */
+int32_t mbir_job_get_sigma_x(mbir_job* job, double* sigma_x);
 
 /* execute()'s outer/inner loop (HAADF/HAADF_ReconstructionEngine.cpp:700-815).
  * costs (may be NULL) receives up to cap FP64 cost values, one after the setup,

@@ -436,12 +436,7 @@ int setup_job(mbir_job* j)
       j->bytes += (int64_t)L * 4;
     }
   }
-  // QGGMRF constants (HAADF/HAADF_QGGMRFPriorModel.cpp:10-19), MRF_P = 2
-  const double sp = std::pow(P.sigma_x, 2.0), spq = std::pow(P.sigma_x, 2.0 - P.mrf_q);
-  J.mrf_q = (float)P.mrf_q; J.mrf_c = (float)P.mrf_c; J.sigma_p = (float)sp;
-  J.sigma_pq = (float)spq; J.alpha = 1.5f; J.b0 = (float)(2.0 / (sp * P.mrf_c));
-  J.pq2 = (float)(2.0 - P.mrf_q); J.inv_sigma_p = (float)(1.0 / sp); J.inv_sigma_pq = (float)(1.0 / spq);
-  J.mrf_qd = P.mrf_q; J.mrf_cd = P.mrf_c; J.sigma_pd = sp; J.sigma_pqd = spq;
+  set_prior(j, P.sigma_x);
 
   // columns: per-line nnz, zero lines with empty columns (:615-632), then the
   // compact column table of all lines (built once, FP32, with gains and weights folded in)
@@ -562,6 +557,20 @@ int sweep_status(mbir_job* j, const SweepAccum& a, double ms, mbir_sweep_stats* 
   return status;
 }
 
+// QGGMRF constants of sigma_x (HAADF/HAADF_QGGMRFPriorModel.cpp:10-19 at setup,
+// updatePriorModel :24-31 after a re-estimation), MRF_P = 2, MRF_ALPHA = 1.5.
+void set_prior(mbir_job* j, double sigma_x)
+{
+  DevJob& J = j->J;
+  const mbir_problem& P = j->prob;
+  j->sigma_x = sigma_x;
+  const double sp = std::pow(sigma_x, 2.0), spq = std::pow(sigma_x, 2.0 - P.mrf_q);
+  J.mrf_q = (float)P.mrf_q; J.mrf_c = (float)P.mrf_c; J.sigma_p = (float)sp;
+  J.sigma_pq = (float)spq; J.alpha = 1.5f; J.b0 = (float)(2.0 / (sp * P.mrf_c));
+  J.pq2 = (float)(2.0 - P.mrf_q); J.inv_sigma_p = (float)(1.0 / sp); J.inv_sigma_pq = (float)(1.0 / spq);
+  J.mrf_qd = P.mrf_q; J.mrf_cd = P.mrf_c; J.sigma_pd = sp; J.sigma_pqd = spq;
+}
+
 void set_bragg(mbir_job* j, double T)
 {
   j->bragg = T;
@@ -586,8 +595,10 @@ int nuisance_update_on(mbir_job* j, hipStream_t s)
     }
     return MBIR_OK;
   }
-  // with a bright-field sinogram no gain and offset estimation
-  // (HAADF/HAADF_ReconstructionEngine_Extra.cpp:429)
+  // with a bright-field sinogram only the offsets are estimated: per view the weighted
+  // mean of e moves from e into mu (HAADF/HAADF_ReconstructionEngine_Extra.cpp:636-669)
+  if (j->prob.joint_estimation && j->J.bfs)
+    HIP_TRY(launch_offset_only(j->J, j->sums6.p, j->mu.p, s));
   if (j->prob.joint_estimation && !j->J.bfs) {
     HIP_TRY(launch_joint_sums(j->J, j->sums6.p, s));
     HIP_TRY(launch_joint_solve(NT, j->sums6.p, j->prob.target_gain, j->I0.p, j->mu.p, j->I0b.p,
@@ -1001,6 +1012,33 @@ int job_cost(mbir_job* j, double* total, double* terms3)
   return MBIR_OK;
 }
 
+// estimateSigmaX (HAADF/HAADF_ReconstructionEngine.cpp:1827-1945): the forward-pair prior
+// sum of the planes [(int16) 0.25 N_z, (int16) 0.75 N_z), differences scaled by the current
+// sigma_x, over N_x N_y (zEnd - zStart + 1) entries; then updatePriorModel
+// (HAADF/HAADF_QGGMRFPriorModel.cpp:24-31): sigma_x = gamma (estimate q)^(1/q) with
+// gamma = k_QGGMRF_Gamma = 5 (Reconstruction/ReconstructionConstants.h:78), and the
+// prior constants of the new sigma_x.
+int estimate_prior(mbir_job* j, double* estimate, double* sigma_x)
+{
+  hipStream_t s = j->stream;
+  const int z0 = (int)(int16_t)(0.25 * j->g.N_z), z1 = (int)(int16_t)(0.75 * j->g.N_z);
+  double sum = 0.0;
+  if (z1 > z0) {
+    HIP_TRY(launch_sigma_sum(j->J, z0, z1, j->sigma_x, j->cost.p + 4, j->cost.p + 3, s));
+    HIP_TRY(hipMemcpyAsync(&sum, j->cost.p + 3, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  const double entries = (double)j->g.N_x * j->g.N_y * (double)(z1 - z0 + 1);
+  const double est = sum / entries;
+  const double q = j->prob.mrf_q;
+  const double sx = 5.0 * std::pow(est * q, 1.0 / q);
+  if (estimate) *estimate = est;
+  if (!(sx > 0) || !std::isfinite(sx)) return MBIR_ERR_INPUT;  // a zero volume has no scale
+  set_prior(j, sx);
+  if (sigma_x) *sigma_x = sx;
+  return MBIR_OK;
+}
+
 }  // namespace
 
 extern "C" int32_t mbir_job_create(mbir_ctx* ctx, const mbir_problem* prob, mbir_job** out)
@@ -1167,6 +1205,21 @@ extern "C" int32_t mbir_job_cost(mbir_job* job, double* total, double* terms3)
   return job_cost(job, total, terms3);
 }
 
+extern "C" int32_t mbir_job_estimate_prior(mbir_job* job, double* estimate, double* sigma_x)
+{
+  if (!job) return MBIR_ERR_INPUT;
+  if (job->J.wfull) return MBIR_ERR_UNSUPPORTED;  // the HAADF engine's step only
+  if (job->cancel.load()) return MBIR_ERR_CANCELLED;
+  return estimate_prior(job, estimate, sigma_x);
+}
+
+extern "C" int32_t mbir_job_get_sigma_x(mbir_job* job, double* sigma_x)
+{
+  if (!job || !sigma_x) return MBIR_ERR_INPUT;
+  *sigma_x = job->sigma_x;
+  return MBIR_OK;
+}
+
 extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, int32_t cap,
                                 int32_t* n_costs)
 {
@@ -1322,6 +1375,8 @@ extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, 
     }
     outer_done = outer + 1;
     if (status == 0 && outer >= 1) break;  // :789-802
+    if (j->prob.estimate_prior && (rc = estimate_prior(j, nullptr, nullptr)) < 0)
+      return rc;  // ESTIMATE_PRIOR (:807-813)
   }
   if (rs) {
     rs->sweeps = ns;

@@ -112,6 +112,7 @@ struct mbir_job {
   int timing = 0;
   int sweep_counter = 0;
   double stop_threshold = 0;
+  double sigma_x = 0;  // current prior scale (TomoInputs::SigmaX), re-estimated with estimate_prior
   int64_t bytes = 0;
   ~mbir_job()
   {
@@ -142,5 +143,7 @@ int sweep_status(mbir_job* j, const SweepAccum& a, double ms, mbir_sweep_stats* 
 int nuisance_update_on(mbir_job* j, hipStream_t s);
 // Bright field: the Bragg threshold used by the next launches (DevJob fields).
 void set_bragg(mbir_job* j, double T);
+// QGGMRF constants of the prior scale sigma_x (DevJob fields).
+void set_prior(mbir_job* j, double sigma_x);
 
 }  // namespace mbir

@@ -180,6 +180,13 @@ hipError_t launch_joint_solve(int N_theta, const double* sums6, double target_ga
 hipError_t launch_joint_apply(const DevJob& J, const double* I0_old, const double* mu_old,
                               const double* I0_new, const double* mu_new, hipStream_t s);
 hipError_t launch_noise_update(const DevJob& J, double* alpha, float* wview, hipStream_t s);
+// estimateSigmaX's prior sum (HAADF/HAADF_ReconstructionEngine.cpp:1827-1945) over the
+// planes [z0, z1), differences scaled by sx, into *out; part holds 1024 doubles
+hipError_t launch_sigma_sum(const DevJob& J, int z0, int z1, double sx, double* part, double* out,
+                            hipStream_t s);
+// HAADF given bright-field data: the offset-only branch of jointEstimation
+// (HAADF/HAADF_ReconstructionEngine_Extra.cpp:636-669); nd holds 2 N_theta doubles
+hipError_t launch_offset_only(const DevJob& J, double* nd, double* mu, hipStream_t s);
 hipError_t launch_cost(const DevJob& J, const double* alpha, int noise_model, double* out3,
                        hipStream_t s);
 // one launch of the request server: kernel shape (ypl, nv), nwg workgroups

@@ -1207,6 +1207,42 @@ __global__ void k_joint_apply(DevJob J, const double* I0o, const double* muo, co
   }
 }
 
+// Offset-only estimation of the HAADF engine given bright-field data
+// (HAADF/HAADF_ReconstructionEngine_Extra.cpp:636-669): per view the weighted mean
+// a = Σ e w / Σ w of the error sinogram, FP64, one block per view.  Under the
+// identity noise model the weight of every entry of a view is its w[th].
+__global__ __launch_bounds__(kBlock) void k_offset_sums(DevJob J, double* nd)
+{
+  __shared__ double red[kBlock];
+  const int th = blockIdx.x;
+  const size_t per = (size_t)J.N_r * J.N_t, base = (size_t)th * per;
+  const double w = (double)J.wview[th];
+  double num = 0, den = 0;
+  for (size_t k = threadIdx.x; k < per; k += blockDim.x) {
+    num += (double)J.e[base + k] * w;
+    den += w;
+  }
+  num = block_sum_d(num, red);
+  den = block_sum_d(den, red);
+  if (threadIdx.x == 0) {
+    nd[2 * th] = num;
+    nd[2 * th + 1] = den;
+  }
+}
+
+// e -= a_th, mu_th += a_th (:656-665)
+__global__ void k_offset_apply(DevJob J, const double* nd, double* mu)
+{
+  const size_t per = (size_t)J.N_r * J.N_t, M = per * J.N_theta;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < M;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int th = (int)(i / per);
+    const double a = nd[2 * th] / nd[2 * th + 1];
+    J.e[i] = (float)((double)J.e[i] - a);
+    if (i == (size_t)th * per) mu[th] += a;
+  }
+}
+
 // updateWeights, identity noise model (…_Extra.cpp:795-866): alpha = mean e², w = 1/alpha.
 __global__ __launch_bounds__(kBlock) void k_noise_update(DevJob J, double* alpha, float* wview)
 {
@@ -1234,6 +1270,47 @@ __device__ inline double qggmrf_value_d(double d, const DevJob& J)
   return ((a * a) / J.sigma_pd) / (J.mrf_cd + pow(a, 2.0 - J.mrf_qd) / J.sigma_pqd);
 }
 
+// The QGGMRF terms of voxel v's 13 forward neighbour pairs, each difference scaled by
+// s: the prior sum of computeCost (HAADF/HAADF_ReconstructionEngine.cpp:1147-1364,
+// s = 1) and of estimateSigmaX (:1827-1945, s = sigma_x), with FILTER weights 0.0523
+// (faces), 0.0370 (edges), 0.0302 (corners).
+__device__ inline double forward_pairs(const DevJob& J, size_t v, double s)
+{
+  const int Ny = J.N_y, Nx = J.N_x, Nz = J.N_z;
+  const int k = (int)(v % Ny);
+  const int j = (int)((v / Ny) % Nx);
+  const int i = (int)(v / ((size_t)Ny * Nx));
+  const double xv = J.x[v];
+  double prior = 0.0;
+#define XV(a, b, c) (double)J.x[((size_t)(a) * Nx + (b)) * Ny + (c)]
+#define ACC(a, b, c, f) prior += (f) * qggmrf_value_d((xv - XV(a, b, c)) * s, J)
+  const double fF = 0.0523, fE = 0.0370, fC = 0.0302;
+  if (k + 1 < Ny) ACC(i, j, k + 1, fF);
+  if (j + 1 < Nx) {
+    if (k - 1 >= 0) ACC(i, j + 1, k - 1, fE);
+    ACC(i, j + 1, k, fF);
+    if (k + 1 < Ny) ACC(i, j + 1, k + 1, fE);
+  }
+  if (i + 1 < Nz) {
+    if (j - 1 >= 0) ACC(i + 1, j - 1, k, fE);
+    ACC(i + 1, j, k, fF);
+    if (j + 1 < Nx) ACC(i + 1, j + 1, k, fE);
+    if (j - 1 >= 0) {
+      if (k - 1 >= 0) ACC(i + 1, j - 1, k - 1, fC);
+      if (k + 1 < Ny) ACC(i + 1, j - 1, k + 1, fC);
+    }
+    if (k - 1 >= 0) ACC(i + 1, j, k - 1, fE);
+    if (j + 1 < Nx) {
+      if (k - 1 >= 0) ACC(i + 1, j + 1, k - 1, fC);
+      if (k + 1 < Ny) ACC(i + 1, j + 1, k + 1, fC);
+    }
+    if (k + 1 < Ny) ACC(i + 1, j, k + 1, fE);
+  }
+#undef ACC
+#undef XV
+  return prior;
+}
+
 __global__ __launch_bounds__(kBlock) void k_cost_partials(DevJob J, double* part)
 {
   __shared__ double red[kBlock];
@@ -1256,39 +1333,7 @@ __global__ __launch_bounds__(kBlock) void k_cost_partials(DevJob J, double* part
       data += ev * ev * (double)J.wview[i / per];
     }
   }
-  const int Ny = J.N_y, Nx = J.N_x, Nz = J.N_z;
-  for (size_t v = g0; v < V; v += gs) {
-    const int k = (int)(v % Ny);
-    const int j = (int)((v / Ny) % Nx);
-    const int i = (int)(v / ((size_t)Ny * Nx));
-    const double xv = J.x[v];
-#define XV(a, b, c) (double)J.x[((size_t)(a) * Nx + (b)) * Ny + (c)]
-#define ACC(a, b, c, f) prior += (f) * qggmrf_value_d(xv - XV(a, b, c), J)
-    const double fF = 0.0523, fE = 0.0370, fC = 0.0302;
-    if (k + 1 < Ny) ACC(i, j, k + 1, fF);
-    if (j + 1 < Nx) {
-      if (k - 1 >= 0) ACC(i, j + 1, k - 1, fE);
-      ACC(i, j + 1, k, fF);
-      if (k + 1 < Ny) ACC(i, j + 1, k + 1, fE);
-    }
-    if (i + 1 < Nz) {
-      if (j - 1 >= 0) ACC(i + 1, j - 1, k, fE);
-      ACC(i + 1, j, k, fF);
-      if (j + 1 < Nx) ACC(i + 1, j + 1, k, fE);
-      if (j - 1 >= 0) {
-        if (k - 1 >= 0) ACC(i + 1, j - 1, k - 1, fC);
-        if (k + 1 < Ny) ACC(i + 1, j - 1, k + 1, fC);
-      }
-      if (k - 1 >= 0) ACC(i + 1, j, k - 1, fE);
-      if (j + 1 < Nx) {
-        if (k - 1 >= 0) ACC(i + 1, j + 1, k - 1, fC);
-        if (k + 1 < Ny) ACC(i + 1, j + 1, k + 1, fC);
-      }
-      if (k + 1 < Ny) ACC(i + 1, j, k + 1, fE);
-    }
-#undef ACC
-#undef XV
-  }
+  for (size_t v = g0; v < V; v += gs) prior += forward_pairs(J, v, 1.0);
   data = block_sum_d(data, red);
   prior = block_sum_d(prior, red);
   if (threadIdx.x == 0) {
@@ -1326,6 +1371,32 @@ __global__ __launch_bounds__(kBlock) void k_cost_final(DevJob J, const double* p
       out3[2] = (double)(M / 2) * log(alpha[0]);
     }
   }
+}
+
+// estimateSigmaX (HAADF/HAADF_ReconstructionEngine.cpp:1827-1945): the forward-pair
+// prior sum, differences scaled by sigma_x, over the planes z0 <= z < z1; FP64 block
+// partials, then one block adds them in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_sigma_partials(DevJob J, int z0, int z1, double sx,
+                                                           double* part)
+{
+  __shared__ double red[kBlock];
+  const size_t plane = (size_t)J.N_x * J.N_y;
+  const size_t v0 = (size_t)z0 * plane, v1 = (size_t)z1 * plane;
+  double acc = 0.0;
+  for (size_t v = v0 + blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < v1;
+       v += (size_t)gridDim.x * blockDim.x)
+    acc += forward_pairs(J, v, sx);
+  acc = block_sum_d(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_final(const double* part, int nparts, double* out)
+{
+  __shared__ double red[kBlock];
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) a += part[b];
+  a = block_sum_d(a, red);
+  if (threadIdx.x == 0) *out = a;
 }
 
 // Bright field, offset (log dose) per view: the minimiser of the selector's
@@ -1741,6 +1812,26 @@ hipError_t launch_cost(const DevJob& J, const double* alpha, int noise_model, do
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_cost_final, dim3(1), dim3(kBlock), 0, s, J, part, nparts, alpha,
                      noise_model, out3);
+  return hipGetLastError();
+}
+
+hipError_t launch_sigma_sum(const DevJob& J, int z0, int z1, double sx, double* part, double* out,
+                            hipStream_t s)
+{
+  const int nparts = 1024;
+  hipLaunchKernelGGL(k_sigma_partials, dim3(nparts), dim3(kBlock), 0, s, J, z0, z1, sx, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, s, (const double*)part, nparts, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_offset_only(const DevJob& J, double* nd, double* mu, hipStream_t s)
+{
+  const size_t M = (size_t)J.N_theta * J.N_r * J.N_t;
+  hipLaunchKernelGGL(k_offset_sums, dim3(J.N_theta), dim3(kBlock), 0, s, J, nd);
+  hipLaunchKernelGGL(k_offset_apply, dim3(grid_for(M)), dim3(kBlock), 0, s, J,
+                     (const double*)nd, mu);
   return hipGetLastError();
 }
 

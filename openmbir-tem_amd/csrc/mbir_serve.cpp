@@ -564,7 +564,7 @@ extern "C" int32_t mbir_server_create(mbir_ctx* ctx, const mbir_server_opts* o, 
 extern "C" int32_t mbir_server_submit(mbir_server* s, const mbir_problem* p, uint64_t* ticket)
 {
   if (!s || !p || !ticket || !p->angles_deg || !p->counts) return MBIR_ERR_INPUT;
-  if (p->bf_sinogram) return MBIR_ERR_UNSUPPORTED;  // single-job paths only
+  if (p->bf_sinogram || p->estimate_prior) return MBIR_ERR_UNSUPPORTED;  // single-job paths only
   const mbir_geometry& g = p->geom;
   if (g.N_theta <= 0 || g.N_r <= 0 || g.N_t <= 0 || g.N_x <= 0 || g.N_y <= 0 || g.N_z <= 0)
     return MBIR_ERR_INPUT;

@@ -308,6 +308,7 @@ void HAADF_ReconstructionEngine::execute()
   p.joint_estimation = (int)m_AdvParams->JOINT_ESTIMATION;
   p.noise_model = (int)m_AdvParams->NOISE_MODEL;
   p.zero_skipping = (int)m_AdvParams->ZERO_SKIPPING;
+  p.estimate_prior = (int)m_AdvParams->ESTIMATE_PRIOR;
   p.seed = T.seed;
   p.schedule = 0;
   p.rows_per_step = 1;
@@ -352,6 +353,8 @@ void HAADF_ReconstructionEngine::execute()
       rc = -100;
     } else {
       G.Object.assign(vol.begin(), vol.end());
+      // the re-estimated prior scale, as the reference writes it back (:811)
+      if (m_AdvParams->ESTIMATE_PRIOR) (void)mbir_job_get_sigma_x(job, &T.SigmaX);
       writeOutputs(job, vol);
       if (m_ErrorCondition == 0) writeExtraOutputs(job);
     }

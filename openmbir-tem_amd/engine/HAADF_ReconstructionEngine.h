@@ -87,6 +87,8 @@ struct AdvancedParameters {
   unsigned int ZERO_SKIPPING = 1;
   unsigned int NOISE_MODEL = 1;
   unsigned int NOISE_ESTIMATION = 1;  // write the variances (HAADF/HAADF_ForwardModel.cpp:62)
+  unsigned int ESTIMATE_PRIOR = 0;    // re-estimate SigmaX per outer iteration
+                                      // (HAADF/HAADF_ReconstructionEngine.cpp:257, 807-813)
 };
 
 class HAADF_ForwardModel {

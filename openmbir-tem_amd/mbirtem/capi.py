@@ -56,7 +56,7 @@ class Problem(C.Structure):
         ("schedule", C.c_int32), ("rows_per_step", C.c_int32), ("kernel_mode", C.c_int32),
         ("bf_mode", C.c_int32), ("bragg_T", C.c_double), ("bragg_delta", C.c_double),
         ("bf_offset", C.c_double), ("bf_nhicd", C.c_int32), ("pad_", C.c_int32),
-        ("bf_sinogram", _F),
+        ("bf_sinogram", _F), ("estimate_prior", C.c_int32), ("pad2_", C.c_int32),
     ]
 
 
@@ -188,6 +188,8 @@ _SIG = {
     "mbir_job_get_selector": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "mbir_job_get_weights": (C.c_int32, [C.c_void_p, _F]),
     "mbir_job_set_bragg_threshold": (C.c_int32, [C.c_void_p, C.c_double]),
+    "mbir_job_estimate_prior": (C.c_int32, [C.c_void_p, _D, _D]),
+    "mbir_job_get_sigma_x": (C.c_int32, [C.c_void_p, _D]),
     "mbir_server_create": (C.c_int32, [C.c_void_p, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     "mbir_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(Problem), C.POINTER(C.c_uint64)]),
     "mbir_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(JobReport)]),
@@ -300,7 +302,7 @@ def make_problem(geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3
                  seed=12345, schedule=0, rows_per_step=1, kernel_mode=0, init_volume=None,
                  default_recon=0.0, init_gain=None, init_offset=None, init_variance=None,
                  bf_mode=0, bragg_T=3.0, bragg_delta=0.5, bf_offset=0.0, bf_nhicd=0,
-                 bf_sinogram=None):
+                 bf_sinogram=None, estimate_prior=0):
     """mbir_problem for the arrays given; returns (Problem, arrays to keep alive)."""
     ang = np.ascontiguousarray(angles_deg, np.float64)
     cnt = np.ascontiguousarray(counts, np.float32)
@@ -332,6 +334,7 @@ def make_problem(geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3
     p.schedule, p.rows_per_step, p.kernel_mode = schedule, rows_per_step, kernel_mode
     p.bf_mode, p.bragg_T, p.bragg_delta, p.bf_offset = bf_mode, bragg_T, bragg_delta, bf_offset
     p.bf_nhicd = bf_nhicd
+    p.estimate_prior = estimate_prior
     if bf_sinogram is not None:
         b = np.ascontiguousarray(bf_sinogram, np.float32)
         assert b.size == cnt.size
@@ -390,6 +393,18 @@ class Job:
 
     def nuisance_update(self):
         _chk("mbir_job_nuisance_update", lib().mbir_job_nuisance_update(self.h))
+
+    def estimate_prior(self):
+        """sigma_x re-estimation: (the estimate, the new sigma_x)."""
+        est, sx = C.c_double(), C.c_double()
+        _chk("mbir_job_estimate_prior",
+             lib().mbir_job_estimate_prior(self.h, C.byref(est), C.byref(sx)))
+        return est.value, sx.value
+
+    def sigma_x(self):
+        sx = C.c_double()
+        _chk("mbir_job_get_sigma_x", lib().mbir_job_get_sigma_x(self.h, C.byref(sx)))
+        return sx.value
 
     def cost(self, terms=False):
         tot = C.c_double()

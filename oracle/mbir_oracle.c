@@ -882,14 +882,37 @@ double orc_bf_beta(double ev, double w, double T, double delta)
   return 2 * T * delta * fabs(ev) * sqrt(w) + T * T * (1 - 2 * delta);
 }
 
-/* jointEstimation, gain + offset branch, HAADF/HAADF_ReconstructionEngine_Extra.cpp:429-637 */
+/* jointEstimation given bright-field data: offsets only,
+ * HAADF/HAADF_ReconstructionEngine_Extra.cpp:636-669.  Per view alpha = sum e w / sum w
+ * (:641-652), then e -= alpha over the view (:654-660) and mu += alpha (:662). */
+static void offset_only_estimation(orc_engine* e)
+{
+  const orc_geom* g = &e->g;
+  size_t per = (size_t)g->N_r * g->N_t;
+  for (int it = 0; it < g->N_theta; ++it) {
+    double num_sum = 0, den_sum = 0;
+    for (size_t k = 0; k < per; ++k) {
+      num_sum += (e->e[it * per + k] * e->w[it * per + k]);
+      den_sum += e->w[it * per + k];
+    }
+    double alpha = num_sum / den_sum;
+    for (size_t k = 0; k < per; ++k) e->e[it * per + k] -= alpha;
+    e->mu[it] += alpha;
+  }
+}
+
+/* jointEstimation, HAADF/HAADF_ReconstructionEngine_Extra.cpp:419-687: the gain + offset
+ * branch (:429-635) without bright-field data, the offset-only branch (:636-669) with it */
 int32_t orc_engine_joint_estimation(orc_engine* e)
 {
   if (e->p.bf_mode) {
     bf_joint_estimation(e);
     return 0;
   }
-  if (e->bfs) return 0; /* with bright-field data no gain/offset estimation, ..._Extra.cpp:429 */
+  if (e->bfs) {
+    offset_only_estimation(e);
+    return 0;
+  }
   const orc_geom* g = &e->g;
   int NT = g->N_theta;
   size_t per = (size_t)g->N_r * g->N_t;
@@ -964,6 +987,67 @@ void orc_engine_update_weights(orc_engine* e)
     for (size_t k = 0; k < per; ++k) e->w[it * per + k] = 1.0 / e->alpha[it];
   }
 }
+
+/* estimateSigmaX, HAADF/HAADF_ReconstructionEngine.cpp:1827-1945: the 13 forward
+ * neighbour pairs of every voxel in the planes zStart = (int16) 0.25 N_z <= i < zEnd =
+ * (int16) 0.75 N_z, each difference multiplied by SigmaX before QGGMRF::Value, weighted by
+ * FILTER, summed and divided by N_x N_y (zEnd - zStart + 1) (:1938-1939).  Then
+ * updatePriorModel, HAADF/HAADF_QGGMRFPriorModel.cpp:24-31: SigmaX = gamma (est q)^(1/q),
+ * gamma = k_QGGMRF_Gamma = 5 (Reconstruction/ReconstructionConstants.h:78), and the
+ * SIGMA_X_* constants of the new SigmaX. */
+double orc_engine_estimate_sigma_x(orc_engine* e, double* sigma_x)
+{
+  const orc_geom* g = &e->g;
+  const double fF = 0.0523, fE = 0.0370, fC = 0.0302;
+  const double sx = e->p.sigma_x;
+  int16_t zStart = (int16_t)(0.25 * g->N_z);
+  int16_t zEnd = (int16_t)(0.75 * g->N_z);
+  double temp = 0, delta;
+#define XV(a, b, c) e->x[((size_t)(a) * g->N_x + (b)) * g->N_y + (c)]
+#define ACC(a, b, c, f)                                  \
+  do {                                                   \
+    delta = XV(i, j, k) - XV(a, b, c);                   \
+    delta *= sx;                                         \
+    temp += (f) * orc_qggmrf_value(delta, &e->q);        \
+  } while (0)
+  for (int i = zStart; i < zEnd; ++i)
+    for (int j = 0; j < g->N_x; ++j)
+      for (int k = 0; k < g->N_y; ++k) {
+        if (k + 1 < g->N_y) ACC(i, j, k + 1, fF);
+        if (j + 1 < g->N_x) {
+          if (k - 1 >= 0) ACC(i, j + 1, k - 1, fE);
+          ACC(i, j + 1, k, fF);
+          if (k + 1 < g->N_y) ACC(i, j + 1, k + 1, fE);
+        }
+        if (i + 1 < g->N_z) {
+          if (j - 1 >= 0) ACC(i + 1, j - 1, k, fE);
+          ACC(i + 1, j, k, fF);
+          if (j + 1 < g->N_x) ACC(i + 1, j + 1, k, fE);
+          if (j - 1 >= 0) {
+            if (k - 1 >= 0) ACC(i + 1, j - 1, k - 1, fC);
+            if (k + 1 < g->N_y) ACC(i + 1, j - 1, k + 1, fC);
+          }
+          if (k - 1 >= 0) ACC(i + 1, j, k - 1, fE);
+          if (j + 1 < g->N_x) {
+            if (k - 1 >= 0) ACC(i + 1, j + 1, k - 1, fC);
+            if (k + 1 < g->N_y) ACC(i + 1, j + 1, k + 1, fC);
+          }
+          if (k + 1 < g->N_y) ACC(i + 1, j, k + 1, fE);
+        }
+      }
+#undef ACC
+#undef XV
+  double NumEntries = (double)g->N_x * g->N_y * (zEnd - zStart + 1);
+  double est = temp / NumEntries;
+  double SigmaX = 5.0 * pow(est * e->q.MRF_Q, 1.0 / e->q.MRF_Q);
+  e->q.SIGMA_X_P = pow(SigmaX, e->q.MRF_P);
+  e->q.SIGMA_X_P_Q = pow(SigmaX, (e->q.MRF_P - e->q.MRF_Q));
+  e->q.SIGMA_X_Q = pow(SigmaX, e->q.MRF_Q);
+  e->p.sigma_x = SigmaX;
+  if (sigma_x) *sigma_x = SigmaX;
+  return est;
+}
+double orc_engine_get_sigma_x(const orc_engine* e) { return e->p.sigma_x; }
 
 /* computeCost, HAADF/HAADF_ReconstructionEngine.cpp:1147-1364 (QGGMRF branch). */
 double orc_engine_cost_terms(orc_engine* e, double* terms3)
@@ -1133,6 +1217,7 @@ int32_t orc_engine_run(orc_engine* e, double* costs, int32_t cap, int32_t* n_cos
       PUSH_COST();
     }
     if (status == 0 && outer >= 1) break;
+    if (e->p.estimate_prior) orc_engine_estimate_sigma_x(e, NULL); /* :807-813 */
   }
 #undef PUSH_COST
   if (n_costs) *n_costs = nc;

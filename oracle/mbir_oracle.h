@@ -63,6 +63,8 @@ typedef struct orc_params {
   double bragg_T;      /* Bragg threshold T (BFReconstructionArgsParser.cpp:176, default 3) */
   double bragg_delta;  /* delta of the Bragg rejector (:179, default 0.5) */
   double bf_offset;    /* added to the counts before the log (:182, default 0) */
+  int32_t estimate_prior; /* AdvancedParameters::ESTIMATE_PRIOR (HAADF_ReconstructionEngine.cpp:807-813) */
+  int32_t bf_nhicd;       /* bright field: the NHICD schedule (BrightField/BFConstants.h:51) */
 } orc_params;
 
 #define ORC_BF_MAX 1.0            /* BrightField/BFConstants.h */
@@ -113,6 +115,9 @@ int32_t orc_engine_sweep_threads(orc_engine* e, int32_t outer, int32_t inner, in
                                  double* stats);
 int32_t orc_engine_joint_estimation(orc_engine* e);
 void orc_engine_update_weights(orc_engine* e);
+/* estimateSigmaX then updatePriorModel; returns the estimate, the new SigmaX in *sigma_x */
+double orc_engine_estimate_sigma_x(orc_engine* e, double* sigma_x);
+double orc_engine_get_sigma_x(const orc_engine* e);
 double orc_engine_cost(orc_engine* e);
 double orc_engine_cost_terms(orc_engine* e, double* terms3);
 /* full outer/inner loop of execute(); costs[] receives one value per cost evaluation */

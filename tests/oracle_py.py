@@ -42,7 +42,7 @@ class Params(C.Structure):
         ("joint_estimation", C.c_int32), ("noise_model", C.c_int32),
         ("zero_skipping", C.c_int32), ("compute_cost", C.c_int32), ("seed", C.c_uint32),
         ("bf_mode", C.c_int32), ("bragg_T", C.c_double), ("bragg_delta", C.c_double),
-        ("bf_offset", C.c_double),
+        ("bf_offset", C.c_double), ("estimate_prior", C.c_int32), ("bf_nhicd", C.c_int32),
     ]
 
 
@@ -83,6 +83,8 @@ def load():
         "orc_engine_sweep_threads": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _D]),
         "orc_engine_joint_estimation": (C.c_int32, [C.c_void_p]),
         "orc_engine_update_weights": (None, [C.c_void_p]),
+        "orc_engine_estimate_sigma_x": (C.c_double, [C.c_void_p, _D]),
+        "orc_engine_get_sigma_x": (C.c_double, [C.c_void_p]),
         "orc_engine_cost": (C.c_double, [C.c_void_p]),
         "orc_engine_cost_terms": (C.c_double, [C.c_void_p, _D]),
         "orc_engine_run": (C.c_int32, [C.c_void_p, _D, C.c_int32, _I, _I]),
@@ -138,10 +140,11 @@ def geometry(N_r, N_t, angles_deg, thickness, delta=1.0, extend_object=0, delta_
 
 def params(sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3, target_gain=1.0, stop_threshold=1e-3,
            num_outer=1, num_inner=20, joint_estimation=0, noise_model=0, zero_skipping=1,
-           compute_cost=1, seed=12345, bf_mode=0, bragg_T=3.0, bragg_delta=0.5, bf_offset=0.0):
+           compute_cost=1, seed=12345, bf_mode=0, bragg_T=3.0, bragg_delta=0.5, bf_offset=0.0,
+           estimate_prior=0, bf_nhicd=0):
     return Params(sigma_x, mrf_q, mrf_c, target_gain, stop_threshold, num_outer, num_inner,
                   joint_estimation, noise_model, zero_skipping, compute_cost, seed, bf_mode,
-                  bragg_T, bragg_delta, bf_offset)
+                  bragg_T, bragg_delta, bf_offset, estimate_prior, bf_nhicd)
 
 
 class Engine:
@@ -193,6 +196,15 @@ class Engine:
 
     def update_weights(self):
         lib().orc_engine_update_weights(self.h)
+
+    def estimate_sigma_x(self):
+        """estimateSigmaX + updatePriorModel: (estimate, new SigmaX)."""
+        sx = C.c_double(0)
+        est = lib().orc_engine_estimate_sigma_x(self.h, C.byref(sx))
+        return est, sx.value
+
+    def sigma_x(self):
+        return lib().orc_engine_get_sigma_x(self.h)
 
     def cost(self, terms=False):
         t = np.zeros(3)

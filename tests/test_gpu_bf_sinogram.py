@@ -56,16 +56,45 @@ def test_scaled_setup_sweeps_and_convergence_match_oracle(gpu_ctx):
     assert job.cost() == pytest.approx(eng.cost(), rel=1e-4)
 
 
-def test_scaled_job_estimates_no_gains_or_offsets(gpu_ctx):
+def test_scaled_job_offset_step_matches_oracle(gpu_ctx):
+    """Joint estimation with bright-field data estimates the offsets only
+    (HAADF/HAADF_ReconstructionEngine_Extra.cpp:636-669): from the same state, one
+    offset step and one variance step on the GPU equal the oracle's to 1e-6, the gains
+    stay, and a whole run keeps the gains while the offsets move."""
     g, ang, y, bf, s = _case(seed=13)
     nt = len(ang)
-    gains, offs = np.full(nt, 0.98), np.full(nt, 0.01)
-    job = _job(gpu_ctx, g, ang, y, bf_sinogram=bf, joint_estimation=1, num_outer=3, num_inner=4,
-               init_gain=gains, init_offset=offs, target_gain=10.0)
-    job.run()
-    I0, mu, al = job.nuisance()
-    np.testing.assert_array_equal(I0, gains)
-    np.testing.assert_array_equal(mu, offs)
+    rng = np.random.default_rng(13)
+    gains = np.full(nt, 1.0)
+    offs = rng.uniform(-0.02, 0.02, nt)
+    var = rng.uniform(0.5, 2.0, nt)
+    kw = dict(joint_estimation=1, noise_model=1)
+    eng = O.Engine(g, ang, y, O.params(**kw), gains=gains, offsets=offs, variances=var,
+                   default_recon=0.2)
+    eng.set_bf_sinogram(bf)
+    job = _job(gpu_ctx, g, ang, y, bf_sinogram=bf, init_gain=gains, init_offset=offs,
+               init_variance=var, default_recon=0.2, **kw)
+    scale = np.abs(y).max()
+    assert np.max(np.abs(job.error_sinogram() - eng.e())) <= 1e-5 * scale
+    for step in range(2):
+        eng.joint_estimation()
+        eng.update_weights()
+        job.nuisance_update()
+        I0r, mur, alr = eng.nuisance()
+        I0g, mug, alg = job.nuisance()
+        np.testing.assert_array_equal(I0g, gains)
+        np.testing.assert_allclose(mug, mur, rtol=0, atol=1e-6 * scale)
+        np.testing.assert_allclose(alg, alr, rtol=1e-6)
+        assert np.max(np.abs(job.error_sinogram() - eng.e())) <= 1e-5 * scale
+    assert np.max(np.abs(mug - offs)) > 1e-3  # the offsets moved
+    # after the step every view's error has zero weighted mean
+    assert np.abs(job.error_sinogram().reshape(nt, -1).mean(axis=1)).max() <= 1e-6 * scale
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+    run = _job(gpu_ctx, g, ang, y, bf_sinogram=bf, num_outer=3, num_inner=4,
+               init_gain=np.full(nt, 0.98), init_offset=offs, target_gain=10.0, **kw)
+    run.run()
+    I0, mu, al = run.nuisance()
+    np.testing.assert_array_equal(I0, np.full(nt, 0.98))
+    assert np.max(np.abs(mu - offs)) > 1e-4
 
 
 def test_scaled_jobs_are_single_job_only(gpu_ctx):

@@ -155,6 +155,74 @@ def test_joint_estimation_and_noise_model(gpu_ctx):
     assert job.cost() == pytest.approx(chk.cost(), rel=1e-6)
 
 
+def test_nuisance_step_from_identical_state_matches_oracle(gpu_ctx):
+    """One joint gain/offset step and one noise-variance step
+    (HAADF/HAADF_ReconstructionEngine_Extra.cpp:419-637, 795-881) from the same state on
+    both sides: volume, gains, offsets and variances away from their optimum.  The GPU's
+    FP64 per-view sums, constrained 2x2 solves and variance update work on the float32
+    error sinogram, so I0, mu and alpha must equal the oracle's to 1e-6 relative."""
+    rng = np.random.default_rng(17)
+    nv = 31
+    gains = rng.uniform(0.8, 1.2, nv)
+    offs = rng.uniform(-0.05, 0.05, nv)
+    g, ang, vol, y = small_problem(n_views=nv, gains=gains, offsets=offs)
+    x0 = np.maximum(vol + rng.normal(0, 0.05, vol.shape), 0).astype(np.float32)
+    g0 = rng.uniform(0.9, 1.1, nv)
+    g0 /= g0.mean()  # the engine keeps gains whose mean is the target
+    m0 = rng.uniform(-0.02, 0.02, nv)
+    v0 = rng.uniform(0.5, 2.0, nv)
+    kw = dict(joint_estimation=1, noise_model=1)
+    eng = O.Engine(g, ang, y, O.params(**kw), gains=g0, offsets=m0, variances=v0,
+                   x0=x0.astype(np.float64))
+    job = _job(gpu_ctx, g, ang, y, init_volume=x0, init_gain=g0, init_offset=m0,
+               init_variance=v0, **kw)
+    for step in range(2):
+        eng.joint_estimation()
+        eng.update_weights()
+        job.nuisance_update()
+        I0r, mur, alr = eng.nuisance()
+        I0g, mug, alg = job.nuisance()
+        np.testing.assert_allclose(I0g, I0r, rtol=1e-6)
+        np.testing.assert_allclose(mug, mur, rtol=0, atol=1e-6 * np.abs(y).max())
+        np.testing.assert_allclose(alg, alr, rtol=1e-6)
+        er = eng.e()
+        assert np.max(np.abs(job.error_sinogram() - er)) <= 1e-5 * np.abs(y).max()
+    assert np.mean(I0g) == pytest.approx(1.0, abs=1e-12)
+    assert np.max(np.abs(I0g - g0)) > 1e-3 and np.max(np.abs(mug - m0)) > 1e-3  # moved
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+
+
+def test_prior_reestimation_matches_oracle(gpu_ctx):
+    """ESTIMATE_PRIOR: estimateSigmaX + updatePriorModel on the device (FP64 reduction over
+    the middle half of the z planes) from the same volume as the oracle: the estimate and
+    the new sigma_x to 1e-9 relative, and the prior cost term with the new sigma_x to 1e-6.
+    A run with re-estimation after every outer iteration keeps falling in cost between
+    re-estimations and ends within 2 % of the oracle's sigma_x (the update orders differ)."""
+    g, ang, vol, y = small_problem(seed=23)
+    rng = np.random.default_rng(5)
+    x0 = (vol + 0.05 * rng.random(vol.shape)).astype(np.float32)
+    eng = O.Engine(g, ang, y, O.params(), x0=x0.astype(np.float64))
+    job = _job(gpu_ctx, g, ang, y, init_volume=x0)
+    est_r, sx_r = eng.estimate_sigma_x()
+    est_g, sx_g = job.estimate_prior()
+    assert est_g == pytest.approx(est_r, rel=1e-9)
+    assert sx_g == pytest.approx(sx_r, rel=1e-9)
+    assert job.sigma_x() == sx_g and abs(sx_g - 0.1) > 1e-3
+    _, tr = eng.cost(terms=True)
+    _, tg = job.cost(terms=True)
+    assert tg[1] == pytest.approx(tr[1], rel=1e-6)
+    kw = dict(num_outer=4, num_inner=5, stop_threshold=0.0, estimate_prior=1)
+    eng = O.Engine(g, ang, y, O.params(**{k: v for k, v in kw.items()}))
+    job = _job(gpu_ctx, g, ang, y, **kw)
+    costs_r, _ = eng.run()
+    rs, costs_g = job.run()
+    assert rs.sweeps == 5 + 3
+    assert job.sigma_x() == pytest.approx(eng.sigma_x(), rel=2e-2)
+    # within the first outer iteration (fixed sigma_x) the cost never rises
+    c = np.asarray(costs_g)
+    assert np.all(np.diff(c[:6]) <= 1e-6 * np.abs(c[1:6]))
+
+
 def test_cancel_returns_minus_999(gpu_ctx):
     g, ang, vol, y = small_problem()
     job = _job(gpu_ctx, g, ang, y)

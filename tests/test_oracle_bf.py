@@ -137,6 +137,56 @@ def test_shared_variance_is_the_maximum_likelihood_value_without_rejections():
     assert alpha[0] == pytest.approx(r.x, rel=1e-5)
 
 
+def test_shared_variance_with_rejections_is_a_majorize_minimize_step():
+    """With rejected entries the shared variance has no closed form.  The data and noise
+    terms of the engine's cost, as a function of the variance a with e, the unscaled
+    weights w_u and the selector fixed, are
+        f(a) = 1/2 sum_sel e^2 w_u / a + delta T sum_rej |e| sqrt(w_u / a) + (M/2) log a.
+    Bounding a^(-1/2) <= (sqrt(a0) / a + 1 / sqrt(a0)) / 2 (AM-GM, equal at a0) gives a
+    majoriser whose minimiser is the reference's update
+    (BrightField/BFForwardModel.cpp:447-511).  So each update must not raise f, and
+    repeated updates must reach f's minimiser (scipy); a wrong power of a, a missing
+    delta T or a dropped sum moves the fixed point."""
+    g, ang, vol, counts = bf_problem(bragg_frac=0.05)
+    prm = bf_params(joint_estimation=1, noise_model=1)
+    eng = O.Engine(g, ang, counts, prm, offsets=np.full(len(ang), -np.log(2000.0) + 0.05))
+    for it in range(3):
+        eng.sweep(0, it)
+    eng.update_selector()
+    sel = eng.sel()
+    assert 1e-3 < 1.0 - sel.mean() < 0.5  # rejections take part
+    e = eng.e()
+    a0 = eng.nuisance()[2][0]
+    wu = eng.w() * a0
+    T, d = eng.bragg(), 0.5
+    M = e.size
+    assert M % 2 == 0  # the engine's noise term halves M as an integer
+    rej = sel == 0
+    S1 = np.sum(np.where(rej, 0.0, e * e * wu))
+    S2 = np.sum(np.where(rej, np.abs(e) * np.sqrt(wu), 0.0))
+
+    def f(a):
+        return 0.5 * S1 / a + d * T * S2 / np.sqrt(a) + (M // 2) * np.log(a)
+
+    # f is the engine's own data + noise cost up to a constant
+    const = 0.5 * np.sum(rej) * T * T * (1 - 2 * d)
+    _, t = eng.cost(terms=True)
+    assert t[0] + t[2] == pytest.approx(f(a0) + const, rel=1e-12)
+    r = optimize.minimize_scalar(f, bounds=(a0 * 1e-3, a0 * 1e3), method="bounded",
+                                 options=dict(xatol=1e-12 * a0))
+    fa = f(a0)
+    for k in range(80):
+        eng.update_weights()
+        a = eng.nuisance()[2]
+        assert np.all(a == a[0])
+        assert f(a[0]) <= fa + 1e-12 * abs(fa), k  # majorize-minimize: f never rises
+        fa = f(a[0])
+    assert abs(a[0] - a0) > 1e-3 * a0  # the fixed point is away from the start
+    assert a[0] == pytest.approx(r.x, rel=1e-6)
+    _, t = eng.cost(terms=True)
+    assert t[0] + t[2] == pytest.approx(f(a[0]) + const, rel=1e-12)
+
+
 def test_threshold_schedule():
     g, ang, vol, counts = bf_problem(bragg_frac=0.05)
     # NumIter > 1: the first outer iteration runs without rejection, then T applies

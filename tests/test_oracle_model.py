@@ -140,6 +140,49 @@ def test_voxel_line_response_is_identity_when_voxels_match_rows():
         assert n == 1 and t[0] == y and v[0] == g.delta_t
 
 
+def _overlap(d, dxy, dt):
+    """Length of the intersection of a voxel slab [-dxy/2, dxy/2] with a detector row
+    of width dt centred at distance d."""
+    return np.maximum(0.0, np.minimum(dxy / 2, d + dt / 2) - np.maximum(-dxy / 2, d - dt / 2))
+
+
+@pytest.mark.parametrize("ratio", [2, 4])
+def test_voxel_line_response_of_coarse_voxels(ratio):
+    """Voxels ratio rows thick (the multi-resolution driver's coarse levels,
+    delta_xy = delta_xz = ratio delta_t): slice y covers rows ratio y .. ratio y + ratio - 1.
+    Each value is the slab/row overlap (an interval intersection, closed form) sampled
+    every OffsetT and interpolated linearly (numpy), as the response table H_t is; the
+    neighbouring rows' overlap is 0 and they are left out."""
+    ang = P.tilt_series(7, 60.0)
+    n_t = 4 * ratio
+    g = O.geometry(16 * ratio, n_t, ang, thickness=4 * ratio, delta_xz=float(ratio))
+    g.delta_xy = float(ratio)  # the driver sets delta_xy with delta_xz per level
+    g.N_y = int(np.floor(g.LengthY / g.delta_xy))
+    g.y0 = -g.LengthY / 2
+    assert g.N_y * ratio == n_t
+    offT = O.lib().orc_offset_t(C.byref(g))
+    assert offT == pytest.approx((ratio / 2 + 0.5) / 64)
+    Ht = np.zeros((7, 64))
+    O.lib().orc_init_ht(C.byref(g), offT, O.dp(Ht))
+    samples = np.arange(64) * offT
+    np.testing.assert_allclose(Ht[0], _overlap(samples, g.delta_xy, g.delta_t), atol=1e-15)
+    for y in range(g.N_y):
+        t = np.zeros(8, np.int32)
+        v = np.zeros(8)
+        n = O.lib().orc_vlr(C.byref(g), O.dp(Ht), offT, y,
+                            t.ctypes.data_as(C.POINTER(C.c_int32)), O.dp(v), 8)
+        assert n == ratio
+        np.testing.assert_array_equal(t[:n], ratio * y + np.arange(ratio))
+        tc = g.y0 + (y + 0.5) * g.delta_xy
+        rows = g.T0 + (t[:n] + 0.5) * g.delta_t
+        expect = np.interp(np.abs(rows - tc), samples, _overlap(samples, g.delta_xy, g.delta_t))
+        np.testing.assert_allclose(v[:n], expect, rtol=0, atol=1e-14)
+        # every row lies inside the slab (overlap delta_t, up to the interpolation's dip
+        # at the trapezoid's kink), so the slice's response sums to its thickness
+        assert np.all(v[:n] <= g.delta_t) and np.all(v[:n] > 0.98 * g.delta_t)
+        assert v[:n].sum() == pytest.approx(g.delta_xy, rel=1e-2)
+
+
 def test_columns_preserve_area_and_centroid():
     ang = P.tilt_series(31, 70.0)
     g = O.geometry(48, 2, ang, thickness=12)
