@@ -245,6 +245,26 @@ int32_t mbir_job_estimate_prior(mbir_job* job, double* estimate, double* sigma_x
 /* The prior scale the next sweep uses (TomoInputs::SigmaX). */
 int32_t mbir_job_get_sigma_x(mbir_job* job, double* sigma_x);
 
+/* The update order of the job's last sweep or NHICD call (no reference counterpart: the
+ * reference draws its order inside UpdateYSlice, …_UpdateVoxels.cpp:213-231).  Returns the
+ * number of conflict-free steps; step k holds units[step_ptr[k] .. step_ptr[k + 1]), each
+ * unit three int32 (line = z N_x + x, first slice y0, slices ny), all of one step updated
+ * at once; a step equals any sequential order of its units, and a unit is its even
+ * slices, then its odd slices.  step_ptr needs cap_steps + 1 entries, units 3 cap_units;
+ * either may be NULL. */
+int32_t mbir_job_get_schedule(mbir_job* job, int32_t* step_ptr, int32_t cap_steps,
+                              int32_t* units, int64_t cap_units);
+
+/* Inspection of the bright-field NHICD schedule (no reference counterpart; the
+ * reference prints its lists only in debug builds, BFReconstructionEngine.cpp:700-706):
+ * with the log on, mbir_job_run records the lines of every NHICD call.
+ * mbir_job_get_nhicd_log returns the number of calls of the last run and fills
+ * call_ptr[0 .. calls] (call c's lines are lines[call_ptr[c] .. call_ptr[c + 1])) and
+ * lines, as far as cap_calls and cap_lines allow; either may be NULL. */
+int32_t mbir_job_set_nhicd_log(mbir_job* job, int32_t on);
+int32_t mbir_job_get_nhicd_log(mbir_job* job, int64_t* call_ptr, int32_t cap_calls,
+                               int32_t* lines, int64_t cap_lines);
+
 /* execute()'s outer/inner loop (HAADF/HAADF_ReconstructionEngine.cpp:700-815).
  * costs (may be NULL) receives up to cap FP64 cost values, one after the setup,
  * one after every sweep and one after every nuisance step (COST_CALCULATE order). */

@@ -228,6 +228,25 @@ extern "C" int32_t mbir_geometry_init(int32_t N_r, int32_t N_t, int32_t N_theta,
 
 namespace {
 
+// The reference's random list (GenRandList, BrightField/BFReconstructionEngine_Extra.cpp:
+// 1072-1105) of 0 .. n - 1 with a seeded MT19937 in place of its clock-seeded generator:
+// entry t is Counter[Index], Index = next() % remaining, then Counter[Index] takes the last
+// remaining counter.
+std::vector<int> rand_list(int n, uint32_t seed)
+{
+  std::mt19937 rng(seed);
+  std::vector<int> counter(n), out(n);
+  for (int i = 0; i < n; ++i) counter[i] = i;
+  int size = n;
+  for (int t = 0; t < n; ++t) {
+    const uint32_t idx = (uint32_t)rng() % (uint32_t)size;
+    out[t] = counter[idx];
+    counter[idx] = counter[size - 1];
+    size--;
+  }
+  return out;
+}
+
 int setup_job(mbir_job* j)
 {
   const mbir_problem& P = j->prob;
@@ -429,10 +448,7 @@ int setup_job(mbir_job* j)
       HIP_TRY(j->magmap.alloc(L));
       HIP_TRY(launch_fill_f32(j->magmap.p, 0.f, L, s));
       J.magmap = j->magmap.p;
-      j->nh_all.resize(L);
-      for (size_t l = 0; l < L; ++l) j->nh_all[l] = (int)l;
-      std::mt19937 rng(P.seed * 2654435761u + 17u);
-      std::shuffle(j->nh_all.begin(), j->nh_all.end(), rng);
+      j->nh_all = rand_list((int)L, P.seed * 2654435761u + 17u);
       j->bytes += (int64_t)L * 4;
     }
   }
@@ -1213,6 +1229,47 @@ extern "C" int32_t mbir_job_estimate_prior(mbir_job* job, double* estimate, doub
   return estimate_prior(job, estimate, sigma_x);
 }
 
+extern "C" int32_t mbir_job_get_schedule(mbir_job* job, int32_t* step_ptr, int32_t cap_steps,
+                                         int32_t* units, int64_t cap_units)
+{
+  if (!job) return MBIR_ERR_INPUT;
+  const int32_t steps = job->step_ptr.empty() ? 0 : (int32_t)job->step_ptr.size() - 1;
+  if (step_ptr)
+    for (int32_t k = 0; k <= steps && k <= cap_steps; ++k) step_ptr[k] = job->step_ptr[k];
+  if (units) {
+    const int64_t n = std::min<int64_t>((int64_t)job->sched_h.size(), cap_units);
+    for (int64_t i = 0; i < n; ++i) {
+      const LineRef& r = job->sched_h[i];
+      const int ny = r.pad >> 16;
+      units[3 * i] = r.line;
+      units[3 * i + 1] = ny ? (r.pad & 0xffff) : 0;
+      units[3 * i + 2] = ny ? ny : job->g.N_y;
+    }
+  }
+  return steps;
+}
+
+extern "C" int32_t mbir_job_set_nhicd_log(mbir_job* job, int32_t on)
+{
+  if (!job) return MBIR_ERR_INPUT;
+  job->nh_log_on = on != 0;
+  return MBIR_OK;
+}
+
+extern "C" int32_t mbir_job_get_nhicd_log(mbir_job* job, int64_t* call_ptr, int32_t cap_calls,
+                                          int32_t* lines, int64_t cap_lines)
+{
+  if (!job) return MBIR_ERR_INPUT;
+  const int32_t calls = job->nh_log_ptr.empty() ? 0 : (int32_t)job->nh_log_ptr.size() - 1;
+  if (call_ptr)
+    for (int32_t c = 0; c <= calls && c <= cap_calls; ++c) call_ptr[c] = job->nh_log_ptr[c];
+  if (lines) {
+    const int64_t n = std::min<int64_t>((int64_t)job->nh_log.size(), cap_lines);
+    for (int64_t i = 0; i < n; ++i) lines[i] = job->nh_log[i];
+  }
+  return calls;
+}
+
 extern "C" int32_t mbir_job_get_sigma_x(mbir_job* job, double* sigma_x)
 {
   if (!job || !sigma_x) return MBIR_ERR_INPUT;
@@ -1250,6 +1307,8 @@ extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, 
     // NHICD state (BFReconstructionEngine.cpp:606-635): the call counter, the next
     // homogeneous part and the ROI volume sum of the last check
     const bool nhicd = j->prob.bf_nhicd != 0;
+    j->nh_log.clear();
+    j->nh_log_ptr.assign(1, 0);
     uint32_t eff = 0;
     int part = 0;
     double prev_mag = 0;
@@ -1284,8 +1343,10 @@ extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, 
             const double thr = nhicd_threshold(filt);
             for (size_t l = 0; l < n; ++l)
               if (filt[l] > thr) list.push_back((int)l);
-            std::mt19937 rng(j->prob.seed * 31u + eff);
-            std::shuffle(list.begin(), list.end(), rng);
+          }
+          if (j->nh_log_on) {
+            j->nh_log.insert(j->nh_log.end(), list.begin(), list.end());
+            j->nh_log_ptr.push_back((int64_t)j->nh_log.size());
           }
           rc = run_list_steps(j, list, outer, inner);
           if (rc < 0) return rc;

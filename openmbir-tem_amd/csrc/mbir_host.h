@@ -100,6 +100,9 @@ struct mbir_job {
   std::vector<float> magmap_h;
   std::vector<int> nh_all;
   std::vector<uint8_t> mask_h;
+  int nh_log_on = 0;                 // record the lines of every NHICD call
+  std::vector<int> nh_log;           // lines of all calls of the last run
+  std::vector<int64_t> nh_log_ptr;   // call c: nh_log[nh_log_ptr[c] .. nh_log_ptr[c + 1])
   // kernel_mode 3: the sweep's step launches as a HIP graph over a padded schedule
   // (step k's lines at gsched[k * graph_lines], unused entries line = -1)
   DevBuf<LineRef> gsched;

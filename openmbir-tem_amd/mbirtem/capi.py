@@ -190,6 +190,11 @@ _SIG = {
     "mbir_job_set_bragg_threshold": (C.c_int32, [C.c_void_p, C.c_double]),
     "mbir_job_estimate_prior": (C.c_int32, [C.c_void_p, _D, _D]),
     "mbir_job_get_sigma_x": (C.c_int32, [C.c_void_p, _D]),
+    "mbir_job_get_schedule": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32,
+                                          C.POINTER(C.c_int32), C.c_int64]),
+    "mbir_job_set_nhicd_log": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "mbir_job_get_nhicd_log": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32,
+                                           C.POINTER(C.c_int32), C.c_int64]),
     "mbir_server_create": (C.c_int32, [C.c_void_p, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     "mbir_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(Problem), C.POINTER(C.c_uint64)]),
     "mbir_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(JobReport)]),
@@ -400,6 +405,31 @@ class Job:
         _chk("mbir_job_estimate_prior",
              lib().mbir_job_estimate_prior(self.h, C.byref(est), C.byref(sx)))
         return est.value, sx.value
+
+    def schedule(self):
+        """The last sweep's steps: a list of (n, 3) arrays of units (line, y0, ny)."""
+        n = _chk("mbir_job_get_schedule", lib().mbir_job_get_schedule(self.h, None, 0, None, 0))
+        ptr = (C.c_int32 * (n + 1))()
+        lib().mbir_job_get_schedule(self.h, ptr, n, None, 0)
+        total = ptr[n] if n else 0
+        u = np.zeros((max(total, 1), 3), np.int32)
+        lib().mbir_job_get_schedule(self.h, ptr, n, u.ctypes.data_as(C.POINTER(C.c_int32)), total)
+        return [u[ptr[k]:ptr[k + 1]].copy() for k in range(n)]
+
+    def set_nhicd_log(self, on=True):
+        _chk("mbir_job_set_nhicd_log", lib().mbir_job_set_nhicd_log(self.h, int(on)))
+
+    def nhicd_lists(self):
+        """The lines of every NHICD call of the last run, one array per call."""
+        n = _chk("mbir_job_get_nhicd_log",
+                 lib().mbir_job_get_nhicd_log(self.h, None, 0, None, 0))
+        ptr = (C.c_int64 * (n + 1))()
+        lib().mbir_job_get_nhicd_log(self.h, ptr, n, None, 0)
+        total = ptr[n] if n else 0
+        lines = np.zeros(max(total, 1), np.int32)
+        lib().mbir_job_get_nhicd_log(self.h, ptr, n,
+                                     lines.ctypes.data_as(C.POINTER(C.c_int32)), total)
+        return [lines[ptr[c]:ptr[c + 1]].copy() for c in range(n)]
 
     def sigma_x(self):
         sx = C.c_double()

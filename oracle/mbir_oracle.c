@@ -440,6 +440,11 @@ struct orc_engine {
   int32_t sweep_counter;
   uint8_t* sel;    /* bright field: Bragg selector [N_theta][N_r][N_t] */
   double bragg;    /* bright field: current Bragg threshold */
+  /* bright-field NHICD: the lines of every call (call c: nh_log[nh_ptr[c] .. nh_ptr[c+1])) */
+  int32_t* nh_log;
+  int64_t* nh_ptr;
+  int32_t nh_calls, nh_cap_calls;
+  int64_t nh_cap_lines;
 };
 
 #define VLR_CAP 8
@@ -455,6 +460,7 @@ void orc_engine_destroy(orc_engine* e)
   free(e->col_ptr); free(e->col_idx); free(e->col_val); free(e->mask); free(e->x);
   free(e->counts); free(e->e); free(e->w); free(e->yest); free(e->I0); free(e->mu);
   free(e->alpha); free(e->init_variance); free(e->sel); free(e->bfs);
+  free(e->nh_log); free(e->nh_ptr);
   free(e);
 }
 
@@ -783,6 +789,35 @@ int32_t orc_engine_sweep(orc_engine* e, int32_t outer, int32_t inner, double* st
   a.seed = e->p.seed + 7919u * (uint32_t)e->sweep_counter;
   run_slab(&a);
   return finish_sweep(e, a.avg_update, a.avg_mag, stats, a.zeros, a.visits);
+}
+
+/* A given update order: units u = (line, y0, ny) in sequence, each the slices y0 .. y0 + ny - 1
+ * of line (z, x) = (line / N_x, line % N_x), even local slices first, then odd ones (the
+ * prior couples slices y and y +- 1 only, so this is one sequential ICD order of
+ * UpdateYSlice's voxel updates, ..._UpdateVoxels.cpp:260-455).  Units whose line has an
+ * empty column are skipped (:245-249).  stats as orc_engine_sweep's (no stop test). */
+void orc_engine_update_units(orc_engine* e, const int32_t* units, int32_t n, int32_t outer,
+                             int32_t inner, double* stats)
+{
+  const orc_geom* g = &e->g;
+  double au = 0, am = 0, zeros = 0, visits = 0;
+  for (int32_t u = 0; u < n; ++u) {
+    int32_t line = units[3 * u], y0 = units[3 * u + 1], ny = units[3 * u + 2];
+    if (e->col_ptr[line + 1] - e->col_ptr[line] <= 0) continue;
+    for (int phase = 0; phase < 2; ++phase)
+      for (int32_t yl = phase; yl < ny; yl += 2) {
+        if (update_voxel(e, line / g->N_x, line % g->N_x, y0 + yl, outer, inner, &au, &am))
+          visits += 1;
+        else
+          zeros += 1;
+      }
+  }
+  if (stats) {
+    stats[0] = au;
+    stats[1] = am;
+    stats[2] = zeros;
+    stats[3] = visits;
+  }
 }
 
 /* y split round-robin into per-thread counts, one slab per thread, :670-723 */
@@ -1121,6 +1156,142 @@ double orc_engine_cost_terms(orc_engine* e, double* terms3)
 
 double orc_engine_cost(orc_engine* e) { return orc_engine_cost_terms(e, NULL); }
 
+/* ---- bright-field NHICD (non-homogeneous ICD) ---- */
+
+/* GenRandList, BrightField/BFReconstructionEngine_Extra.cpp:1072-1105: output t is the
+ * input entry Counter[Index], Index = next() % remaining, then Counter[Index] takes the
+ * last remaining counter (the same draw as UpdateYSlice's voxel order). */
+static void nh_rand_list(const int32_t* in, int32_t n, orc_mt* rng, int32_t* out)
+{
+  int32_t* counter = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  for (int32_t j = 0; j < n; ++j) counter[j] = j;
+  int32_t size = n;
+  for (int32_t t = 0; t < n; ++t) {
+    uint32_t idx = orc_mt_next(rng) % (uint32_t)size;
+    out[t] = in[counter[idx]];
+    counter[idx] = counter[size - 1];
+    size--;
+  }
+  free(counter);
+}
+
+/* m_HammingWindow, BrightField/BFReconstructionEngine.cpp:157-181 */
+static const double kNhWindow[5][5] = {{0.0013, 0.0086, 0.0159, 0.0086, 0.0013},
+                                       {0.0086, 0.0581, 0.1076, 0.0581, 0.0086},
+                                       {0.0159, 0.1076, 0.1993, 0.1076, 0.0159},
+                                       {0.0086, 0.0581, 0.1076, 0.0581, 0.0086},
+                                       {0.0013, 0.0086, 0.0159, 0.0086, 0.0013}};
+#define ORC_NH_PARTS 20 /* k_NumHomogeniousIter = k_NumNonHomogeniousIter */
+
+/* ComputeVSC, BrightField/BFReconstructionEngine_Extra.cpp:888-925 */
+void orc_nhicd_filter(const double* mag, int32_t Nz, int32_t Nx, double* filt)
+{
+  for (int i = 0; i < Nz; ++i)
+    for (int j = 0; j < Nx; ++j) {
+      double f = 0;
+      for (int p = -2; p <= 2; ++p)
+        for (int q = -2; q <= 2; ++q)
+          if (i + p >= 0 && i + p < Nz && j + q >= 0 && j + q < Nx)
+            f += kNhWindow[p + 2][q + 2] * mag[(size_t)(i + p) * Nx + (j + q)];
+      filt[(size_t)i * Nx + j] = f;
+    }
+}
+
+static int cmp_double(const void* a, const void* b)
+{
+  double x = *(const double*)a, y = *(const double*)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* SetNonHomThreshold, :934-1010: RandomizedSelect(A, 0, n - 1, n - n/20) is the order
+ * statistic at 0-based position n - n/20 (:1124-1147) */
+double orc_nhicd_threshold(const double* v, int32_t n)
+{
+  double* t = (double*)malloc(sizeof(double) * n);
+  memcpy(t, v, sizeof(double) * n);
+  qsort(t, n, sizeof(double), cmp_double);
+  int32_t k = n - n / ORC_NH_PARTS;
+  double r = t[k < n ? k : n - 1];
+  free(t);
+  return r;
+}
+
+static void nh_log_call(orc_engine* e, const int32_t* lines, int32_t n)
+{
+  if (e->nh_calls + 2 > e->nh_cap_calls) {
+    e->nh_cap_calls = 2 * e->nh_cap_calls + 64;
+    e->nh_ptr = (int64_t*)realloc(e->nh_ptr, sizeof(int64_t) * e->nh_cap_calls);
+    if (e->nh_calls == 0) e->nh_ptr[0] = 0;
+  }
+  int64_t b = e->nh_ptr[e->nh_calls];
+  if (b + n > e->nh_cap_lines) {
+    e->nh_cap_lines = 2 * (b + n) + 1024;
+    e->nh_log = (int32_t*)realloc(e->nh_log, sizeof(int32_t) * e->nh_cap_lines);
+  }
+  memcpy(e->nh_log + b, lines, sizeof(int32_t) * n);
+  e->nh_ptr[++e->nh_calls] = b + n;
+}
+
+/* The lines of every NHICD call of the last run: returns the number of calls; call_ptr
+ * (cap_calls + 1 entries) and lines (cap_lines) receive what fits. */
+int32_t orc_engine_nhicd_log(const orc_engine* e, int64_t* call_ptr, int32_t cap_calls,
+                             int32_t* lines, int64_t cap_lines)
+{
+  for (int32_t c = 0; c <= e->nh_calls && c <= cap_calls; ++c)
+    call_ptr[c] = e->nh_calls ? e->nh_ptr[c] : 0;
+  int64_t n = e->nh_calls ? e->nh_ptr[e->nh_calls] : 0;
+  if (lines && n > 0) memcpy(lines, e->nh_log, sizeof(int32_t) * (n < cap_lines ? n : cap_lines));
+  return e->nh_calls;
+}
+
+/* One NHICD call, BFReconstructionEngine_Extra.cpp:476-880 with BFUpdateYSlice.cpp:147-335
+ * on one y-slab (the serial path): the list's magnitudes are reset (:640-646), the list is
+ * put in a random order (GenRandList, :677), and every slice of each line with a non-empty
+ * column is updated in that order, the line's magnitude accumulating |x_new - x_old|
+ * (BFUpdateYSlice.cpp:303-304).  The stop test (stopCriteria, :1236-1270) on calls that
+ * are multiples of 20: ROI magnitude sum / the ROI volume sum of 20 calls before. */
+static int32_t nh_call(orc_engine* e, const int32_t* list, int32_t n, int32_t outer,
+                       int32_t inner, uint32_t eff, double prev_mag, double* mag)
+{
+  const orc_geom* g = &e->g;
+  int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  orc_mt rng;
+  orc_mt_seed(&rng, e->p.seed + 7919u * eff + 1u);
+  nh_rand_list(list, n, &rng, order);
+  for (int32_t t = 0; t < n; ++t) mag[list[t]] = 0.0;
+  for (int32_t t = 0; t < n; ++t) {
+    int32_t line = order[t];
+    int32_t j_new = line / g->N_x, k_new = line % g->N_x;
+    if (e->col_ptr[line + 1] - e->col_ptr[line] <= 0) continue;
+    for (int32_t i = 0; i < g->N_y; ++i) {
+      double au = 0, am = 0;
+      double old = e->x[(size_t)line * g->N_y + i];
+      update_voxel(e, j_new, k_new, i, outer, inner, &au, &am);
+      mag[line] += fabs(e->x[(size_t)line * g->N_y + i] - old);
+    }
+  }
+  free(order);
+  int32_t status = 1;
+  if (eff % ORC_NH_PARTS == 0 && eff > 0 && prev_mag > 0) {
+    double sum = 0;
+    for (size_t l = 0; l < (size_t)g->N_z * g->N_x; ++l)
+      if (e->mask[l] == 1) sum += mag[l];
+    if (fabs(sum) / fabs(prev_mag) < e->stop_threshold) status = 0;
+  }
+  return status;
+}
+
+/* roiVolumeSum, BFReconstructionEngine_Extra.cpp:1216-1232 */
+static double roi_volume_sum(const orc_engine* e)
+{
+  const orc_geom* g = &e->g;
+  double sum = 0;
+  for (size_t l = 0; l < (size_t)g->N_z * g->N_x; ++l)
+    if (e->mask[l] == 1)
+      for (int i = 0; i < g->N_y; ++i) sum += e->x[l * g->N_y + i];
+  return sum;
+}
+
 /* BFReconstructionEngine::execute() loop, BrightField/BFReconstructionEngine.cpp:575-872,
  * with regular (full, random-order) sweeps in place of the NHICD voxel lists:
  * the Bragg threshold is DefBraggThreshold for the first outer iteration's inner
@@ -1143,6 +1314,29 @@ static int32_t bf_run(orc_engine* e, double* costs, int32_t cap, int32_t* n_cost
   const double des = e->p.bragg_T;
   double tmp = num_iter > 1 ? ORC_DEF_BRAGG : des;
   e->bragg = tmp;
+  /* NHICD state (BFReconstructionEngine.cpp:595-635): the random list of all lines, the
+   * update-magnitude map, the next homogeneous part, the call counter and the ROI volume
+   * sum of the last check */
+  const int32_t L = e->g.N_z * e->g.N_x;
+  int32_t* all = NULL;
+  int32_t* list = NULL;
+  double* mag = NULL;
+  double* filt = NULL;
+  uint32_t eff = 0, part = 0;
+  double prev_mag = 0;
+  e->nh_calls = 0;
+  if (e->p.bf_nhicd) {
+    int32_t* reg = (int32_t*)malloc(sizeof(int32_t) * L);
+    for (int32_t l = 0; l < L; ++l) reg[l] = l; /* GenRegularList: z-major (x, z) pairs */
+    all = (int32_t*)malloc(sizeof(int32_t) * L);
+    orc_mt rng;
+    orc_mt_seed(&rng, e->p.seed * 2654435761u + 17u);
+    nh_rand_list(reg, L, &rng, all);
+    free(reg);
+    list = (int32_t*)malloc(sizeof(int32_t) * L);
+    mag = (double*)calloc(L, sizeof(double));
+    filt = (double*)malloc(sizeof(double) * L);
+  }
   for (int outer = 0; outer < e->p.num_outer; ++outer) {
     if (outer > 0) {
       num_iter = 1;
@@ -1151,8 +1345,32 @@ static int32_t bf_run(orc_engine* e, double* costs, int32_t cap, int32_t* n_cost
     int32_t status = 1;
     for (int inner = 0; inner < num_iter; ++inner) {
       if (num_iter > 1 && inner == 0) e->bragg = ORC_DEF_BRAGG;
-      double st[4];
-      status = orc_engine_sweep(e, outer, inner, st);
+      if (e->p.bf_nhicd) {
+        int32_t n = 0;
+        if (eff % 2 == 0) {
+          /* homogeneous: the next twentieth of the list, the last one longer (:681-712) */
+          part %= ORC_NH_PARTS;
+          int32_t ne = L / ORC_NH_PARTS;
+          if (part == ORC_NH_PARTS - 1) ne += L - ne * ORC_NH_PARTS;
+          int32_t start = (L / ORC_NH_PARTS) * (int32_t)part;
+          for (int32_t t = 0; t < ne; ++t) list[n++] = all[start + t];
+          part++;
+        } else {
+          /* non-homogeneous: the lines whose filtered magnitude exceeds the threshold,
+           * in raster order (GenNonHomList, Extra.cpp:1013-1033) */
+          orc_nhicd_filter(mag, e->g.N_z, e->g.N_x, filt);
+          double thr = orc_nhicd_threshold(filt, L);
+          for (int32_t l = 0; l < L; ++l)
+            if (filt[l] > thr) list[n++] = l;
+        }
+        nh_log_call(e, list, n);
+        status = nh_call(e, list, n, outer, inner, eff, prev_mag, mag);
+        if (eff % ORC_NH_PARTS == 0) prev_mag = roi_volume_sum(e); /* :730-735 */
+        eff++;
+      } else {
+        double st[4];
+        status = orc_engine_sweep(e, outer, inner, st);
+      }
       ns++;
       if (status == 0) break;
       PUSH_COST();
@@ -1175,6 +1393,10 @@ static int32_t bf_run(orc_engine* e, double* costs, int32_t cap, int32_t* n_cost
       }
     }
   }
+  free(all);
+  free(list);
+  free(mag);
+  free(filt);
 #undef PUSH_COST
   if (n_costs) *n_costs = nc;
   if (n_sweeps) *n_sweeps = ns;

@@ -113,11 +113,20 @@ int32_t orc_engine_sweep(orc_engine* e, int32_t outer, int32_t inner, double* st
 /* the same sweep with the reference's y-slab thread decomposition */
 int32_t orc_engine_sweep_threads(orc_engine* e, int32_t outer, int32_t inner, int32_t nthreads,
                                  double* stats);
+/* voxel updates in a given order of units (line, y0, ny), even local slices before odd */
+void orc_engine_update_units(orc_engine* e, const int32_t* units, int32_t n, int32_t outer,
+                             int32_t inner, double* stats);
 int32_t orc_engine_joint_estimation(orc_engine* e);
 void orc_engine_update_weights(orc_engine* e);
 /* estimateSigmaX then updatePriorModel; returns the estimate, the new SigmaX in *sigma_x */
 double orc_engine_estimate_sigma_x(orc_engine* e, double* sigma_x);
 double orc_engine_get_sigma_x(const orc_engine* e);
+/* bright-field NHICD helpers: ComputeVSC's 5 x 5 Hamming filter and SetNonHomThreshold */
+void orc_nhicd_filter(const double* mag, int32_t Nz, int32_t Nx, double* filt);
+double orc_nhicd_threshold(const double* v, int32_t n);
+/* the lines of every NHICD call of the last run (bf_nhicd): returns the call count */
+int32_t orc_engine_nhicd_log(const orc_engine* e, int64_t* call_ptr, int32_t cap_calls,
+                             int32_t* lines, int64_t cap_lines);
 double orc_engine_cost(orc_engine* e);
 double orc_engine_cost_terms(orc_engine* e, double* terms3);
 /* full outer/inner loop of execute(); costs[] receives one value per cost evaluation */

@@ -82,9 +82,14 @@ def load():
         "orc_engine_sweep": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _D]),
         "orc_engine_sweep_threads": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _D]),
         "orc_engine_joint_estimation": (C.c_int32, [C.c_void_p]),
+        "orc_engine_update_units": (None, [C.c_void_p, _I, C.c_int32, C.c_int32, C.c_int32, _D]),
         "orc_engine_update_weights": (None, [C.c_void_p]),
         "orc_engine_estimate_sigma_x": (C.c_double, [C.c_void_p, _D]),
         "orc_engine_get_sigma_x": (C.c_double, [C.c_void_p]),
+        "orc_nhicd_filter": (None, [_D, C.c_int32, C.c_int32, _D]),
+        "orc_nhicd_threshold": (C.c_double, [_D, C.c_int32]),
+        "orc_engine_nhicd_log": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32, _I,
+                                             C.c_int64]),
         "orc_engine_cost": (C.c_double, [C.c_void_p]),
         "orc_engine_cost_terms": (C.c_double, [C.c_void_p, _D]),
         "orc_engine_run": (C.c_int32, [C.c_void_p, _D, C.c_int32, _I, _I]),
@@ -191,6 +196,13 @@ class Engine:
             s = lib().orc_engine_sweep(self.h, outer, inner, dp(st))
         return s, st
 
+    def update_units(self, units, outer=0, inner=0):
+        """Voxel updates in the order of units [(line, y0, ny), ...]."""
+        u = np.ascontiguousarray(units, np.int32).reshape(-1, 3)
+        st = np.zeros(4)
+        lib().orc_engine_update_units(self.h, u.ctypes.data_as(_I), len(u), outer, inner, dp(st))
+        return st
+
     def joint_estimation(self):
         return lib().orc_engine_joint_estimation(self.h)
 
@@ -205,6 +217,16 @@ class Engine:
 
     def sigma_x(self):
         return lib().orc_engine_get_sigma_x(self.h)
+
+    def nhicd_lists(self):
+        """The lines of every NHICD call of the last run, one array per call."""
+        n = lib().orc_engine_nhicd_log(self.h, (C.c_int64 * 1)(), 0, None, 0)
+        ptr = (C.c_int64 * (n + 1))()
+        lib().orc_engine_nhicd_log(self.h, ptr, n, None, 0)
+        total = ptr[n] if n else 0
+        lines = np.zeros(max(total, 1), np.int32)
+        lib().orc_engine_nhicd_log(self.h, ptr, n, lines.ctypes.data_as(_I), total)
+        return [lines[ptr[c]:ptr[c + 1]].copy() for c in range(n)]
 
     def cost(self, terms=False):
         t = np.zeros(3)

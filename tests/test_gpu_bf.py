@@ -111,6 +111,38 @@ def test_bf_run_matches_oracle(gpu_ctx):
     np.testing.assert_allclose(alg, al, rtol=5e-2)
 
 
+def test_bf_nhicd_lists_and_stop_call_match_oracle(gpu_ctx):
+    """The NHICD schedule on the GPU against the oracle's restatement, same seed.  The
+    homogeneous calls (even calls) take the same lines: one list of all lines drawn by
+    the reference's list algorithm from the same MT19937 seed.  A non-homogeneous call
+    takes the lines above the (n - n/20)-th filtered update magnitude; the magnitudes
+    differ between the two only through the update order inside a call and float32
+    against FP64, so the lists must overlap by at least 90 %.  The stop test (every 20
+    calls) must fire at the same call."""
+    g, ang, vol, counts = bf_problem(bragg_frac=0.03)
+    offs = np.full(len(ang), -np.log(2000.0))
+    kw = dict(BF, bf_nhicd=1, num_outer=1, num_inner=2000, stop_threshold=2e-2,
+              zero_skipping=0, seed=77)
+    eng = O.Engine(g, ang, counts, O.params(**kw), offsets=offs)
+    _, n_r = eng.run()
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
+    job = capi.Job(gpu_ctx, geom, ang, counts.astype(np.float32), init_offset=offs, **kw)
+    job.set_nhicd_log(True)
+    rs, _ = job.run()
+    lr, lg = eng.nhicd_lists(), job.nhicd_lists()
+    assert len(lr) == n_r and len(lg) == rs.sweeps
+    assert n_r < 2000 and (n_r - 1) % 20 == 0
+    assert rs.sweeps == n_r, (rs.sweeps, n_r)
+    worst = 1.0
+    for c, (a, b) in enumerate(zip(lr, lg)):
+        if c % 2 == 0:
+            np.testing.assert_array_equal(np.sort(b), np.sort(a))
+        else:
+            inter = len(np.intersect1d(a, b))
+            worst = min(worst, inter / max(len(a), len(b), 1))
+    assert worst >= 0.9, worst
+
+
 def test_bf_nhicd_converges_to_the_full_sweep_minimum(gpu_ctx):
     """The NHICD schedule (homogeneous twentieths alternating with the filtered top-5 %
     lists) only reorders the updates.  With one outer iteration the Bragg threshold

@@ -111,6 +111,28 @@ def test_first_sweep_statistics_match_oracle(gpu_ctx):
     assert st.sum_abs_update > 0 and ost[0] > 0
 
 
+def test_sweep_equals_the_sequential_order_of_its_schedule(gpu_ctx):
+    """A GPU sweep updates the units of each conflict-free step at once.  The oracle,
+    replaying the same units one after another (the exported schedule, steps in order),
+    must reach the same volume and error sinogram up to float32 rounding: any two units
+    of one step that shared a detector bin or were prior neighbours would make the
+    parallel result depart from every sequential order."""
+    g, ang, vol, y = small_problem()
+    job = _job(gpu_ctx, g, ang, y, default_recon=0.1)
+    eng = O.Engine(g, ang, y, O.params(), default_recon=0.1)
+    for it in range(3):
+        job.sweep(0, it)
+        steps = job.schedule()
+        units = np.concatenate(steps)
+        assert len(units) >= g.N_z * g.N_x * 0.9  # (lines with empty columns are dropped)
+        eng.update_units(units, 0, it)
+        xr, xg = eng.x(), job.volume()
+        assert np.max(np.abs(xg - xr)) <= 2e-5 * np.abs(xr).max(), it
+        er = eng.e()
+        assert np.max(np.abs(job.error_sinogram() - er)) <= 2e-5 * np.abs(y).max(), it
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+
+
 def test_converged_volume_matches_oracle(gpu_ctx):
     g, ang, vol, y = small_problem()
     eng = _engine(g, ang, y)

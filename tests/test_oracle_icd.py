@@ -261,3 +261,44 @@ def test_run_costs_never_increase():
     # the first weight update changes the cost's noise term from its initial variance,
     # so compare within each constant-weight stretch and after each nuisance step
     assert np.all(np.diff(costs[1:]) <= 1e-8 * np.abs(costs[1:-1]))
+
+
+def _sweep_line_order(g, seed):
+    """The line order of one oracle sweep (UpdateYSlice's draw, ..._UpdateVoxels.cpp:
+    213-231: Index = next() % remaining, Counter[Index] = Counter[last])."""
+    raw = np.random.RandomState(seed).randint(0, 2 ** 32, size=g.N_z * g.N_x, dtype=np.uint64)
+    counter = list(range(g.N_z * g.N_x))
+    out = []
+    for t in range(g.N_z * g.N_x):
+        idx = int(raw[t]) % len(counter)
+        out.append(counter[idx])
+        counter[idx] = counter[-1]
+        counter.pop()
+    return out
+
+
+def test_unit_order_replay_equals_the_sweep():
+    """update_units replays a given order of voxel updates.  Units of one slice each, in
+    a sweep's own order (lines as drawn, slices ascending), must reproduce the sweep bit
+    for bit; and a unit of many slices is its even slices then its odd slices."""
+    g, ang, vol, y = small_problem()
+    a = O.Engine(g, ang, y, O.params(seed=99))
+    b = O.Engine(g, ang, y, O.params(seed=99))
+    a.sweep(0, 0)  # sweep_counter 0: seed 99
+    order = _sweep_line_order(g, 99)
+    b.update_units([(l, yy, 1) for l in order for yy in range(g.N_y)])
+    np.testing.assert_array_equal(a.x(), b.x())
+    np.testing.assert_array_equal(a.e(), b.e())
+    c = O.Engine(g, ang, y, O.params(seed=99), default_recon=0.2)
+    d = O.Engine(g, ang, y, O.params(seed=99), default_recon=0.2)
+    y0, ny = 1, g.N_y - 3
+    lines = order[:50]
+    c.update_units([(l, y0, ny) for l in lines])
+    d.update_units([(l, y0 + k, 1) for l in lines
+                    for k in list(range(0, ny, 2)) + list(range(1, ny, 2))])
+    np.testing.assert_array_equal(c.x(), d.x())
+    x = c.x()
+    touched = np.zeros(x.shape, bool)
+    for l in lines:
+        touched[l // g.N_x, l % g.N_x, y0:y0 + ny] = True
+    assert np.all(x[~touched] == 0.2) and np.any(x[touched] != 0.2)
