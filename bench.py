@@ -34,6 +34,100 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# Algorithmic HBM bytes of the line update per voxel visit (float32): an updated voxel
+# reads e and w for its theta sums (8 B per column entry and detector row) and
+# read-modify-writes e (8 B), plus its own read and write (8 B); a zero-skipped voxel is
+# only read (4 B).  The column table and the neighbour lines (cache-resident) are not
+# counted.
+BYTES_MODEL = "16 B per column entry and detector row of an updated voxel + 8 B per updated voxel + 4 B per zero-skipped voxel"
+
+
+def algorithmic_bytes(nnz_updated, updated, zero_skipped):
+    return 16.0 * nnz_updated + 8.0 * updated + 4.0 * zero_skipped
+
+
+def traffic_probe(args):
+    """Run under rocprofv3 --pmc by measure_traffic: a 512 MiB device copy (the counter's
+    scale), then three sweeps of the workload's job."""
+    import torch
+
+    from mbirtem import capi, workloads
+
+    n = (512 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    ctx = capi.Context(0)
+    w = workloads.make(ctx, workloads.WORKLOADS[args.workload])
+    for it in range(3):
+        w["job"].sweep(it, 0)
+    w["job"].close()
+    ctx.close()
+
+
+def _pmc_rows(d):
+    import csv
+    out = []
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                with open(os.path.join(root, f)) as fh:
+                    out.extend(csv.DictReader(fh))
+    return out
+
+
+def measure_traffic(workload):
+    """HBM bytes per k_icd_step launch, measured now: two rocprofv3 --pmc passes
+    (FETCH_SIZE, then WRITE_SIZE) over a child process running traffic_probe, each
+    counter scaled by its reading on the 512 MiB copy of the same process."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    per_launch = {}
+    factors = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="mbir_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-probe", "--workload", workload]
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=240,
+                               env=dict(os.environ, TMPDIR="/tmp"))
+        except subprocess.TimeoutExpired:
+            return None, f"rocprofv3 --pmc {ctr} timed out"
+        rows = _pmc_rows(d)
+        shutil.rmtree(d, ignore_errors=True)
+        if r.returncode != 0 and not rows:
+            return None, f"rocprofv3 --pmc {ctr} failed ({r.returncode})"
+        disp = {}
+        for row in rows:
+            if row.get("Counter_Name") != ctr:
+                continue
+            k = (row.get("Kernel_Name", ""), row.get("Dispatch_Id", ""))
+            disp[k] = disp.get(k, 0.0) + float(row.get("Counter_Value", 0) or 0)
+        icd = [v for (name, _), v in disp.items() if "k_icd_step" in name]
+        # the copy: a kernel named for copying, else the run's largest mover of bytes
+        cp = sorted(v for (name, _), v in disp.items() if "copy" in name.lower())
+        big = cp or sorted(v for (name, _), v in disp.items() if "mbir::" not in name)
+        if not icd or not big:
+            return None, f"no {ctr} rows for the kernels"
+        copy_kib = big[-1]
+        factors[ctr] = copy_kib * 1024.0 / (512 << 20)
+        per_launch[ctr] = sum(icd) / len(icd) * 1024.0
+    bytes_per_launch = (per_launch["FETCH_SIZE"] / factors["FETCH_SIZE"] +
+                        per_launch["WRITE_SIZE"] / factors["WRITE_SIZE"])
+    note = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE child runs of this bench (3 sweeps), "
+            f"scaled by a 512 MiB copy: factors {factors['FETCH_SIZE']:.3f} / "
+            f"{factors['WRITE_SIZE']:.3f}")
+    return round(bytes_per_launch, 1), note
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,17 +151,38 @@ def parse():
     ap.add_argument("--bf-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "icd_step_traffic.json"))
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 --pmc child runs that measure HBM traffic")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
-def cpu_baseline(wl, ang, y, threads):
-    """Time one steady-state sweep of the FP64 oracle (bounded sample: one sweep)."""
+def _host_info():
+    model = ""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def cpu_baseline(wl, ang, y, threads, ctx=None):
+    """The FP64 oracle (a C restatement of the reference's sweep, with its y-slab thread
+    decomposition) timed on this host, bounded samples:
+    - the configured workload: one steady-state sweep on `threads` threads, and the
+      y-slab [0, 8) of a steady-state sweep on one thread;
+    - the 128x16x64 two-sphere workload run end to end (20 sweeps) on `threads` threads
+      and on one, beside the same run on the GPU."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import numpy as np
 
     import oracle_py as O
 
+    nproc, model = _host_info()
     g = O.geometry(wl.n_r, wl.n_t, ang, wl.thickness)
     prm = O.params(sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3, joint_estimation=wl.joint_estimation,
                    noise_model=wl.noise_model, stop_threshold=wl.stop_threshold, seed=wl.seed)
@@ -76,13 +191,56 @@ def cpu_baseline(wl, ang, y, threads):
     log(f"cpu: oracle setup {time.time() - t0:.1f}s")
     eng.sweep(0, 0, threads=threads)  # first sweep (no zero skipping), untimed
     t0 = time.time()
-    _, st = eng.sweep(1, 0, threads=threads)
+    eng.sweep(1, 0, threads=threads)
     dt = time.time() - t0
     V = g.N_x * g.N_y * g.N_z
     log(f"cpu: steady sweep {dt:.2f}s on {threads} threads, {V / dt:.3g} visits/s")
-    return {"value": V / dt, "unit": "voxel-updates/s", "cores": threads, "kind": "port",
-            "sample": f"one steady-state ICD sweep (second sweep, zero skipping on) of the FP64 "
-                      f"oracle over the full {g.N_x}x{g.N_y}x{g.N_z} volume, {threads} y-slab threads"}
+    ys = min(8, g.N_y)
+    t0 = time.time()
+    eng.sweep_slab(2, 0, 0, ys)
+    dt1 = time.time() - t0
+    v1 = g.N_x * ys * g.N_z / dt1
+    log(f"cpu: slab [0, {ys}) of a steady sweep on 1 thread {dt1:.2f}s, {v1:.3g} visits/s")
+    out = {"value": V / dt, "unit": "voxel-updates/s", "cores": threads, "kind": "port",
+           "sample": f"one steady-state ICD sweep (second sweep, zero skipping on) of the FP64 "
+                     f"oracle over the full {g.N_x}x{g.N_y}x{g.N_z} volume, {threads} y-slab "
+                     f"threads (the reference's TBB split)",
+           "host_nproc": nproc, "host_cpu": model,
+           "one_core": {"value": v1, "unit": "voxel-updates/s", "cores": 1,
+                        "sample": f"the y-slab [0, {ys}) of a steady-state sweep "
+                                  f"({g.N_x}x{ys}x{g.N_z} voxel visits) on one thread"}}
+    del eng
+    # the small two-sphere workload end to end, CPU and GPU
+    from mbirtem import workloads
+    s1 = workloads.HAADF_128x16x64
+    d1 = workloads.make(ctx, s1)
+    g1 = O.geometry(s1.n_r, s1.n_t, d1["angles"], s1.thickness)
+    runs = {}
+    for nt in (threads, 1):
+        e1 = O.Engine(g1, d1["angles"], np.asarray(d1["sinogram"], np.float64),
+                      O.params(sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3, num_outer=1, num_inner=20,
+                               stop_threshold=s1.stop_threshold, compute_cost=0, seed=1))
+        t0 = time.time()
+        n = 0
+        for it in range(20):
+            st, _ = e1.sweep(0, it, threads=nt)
+            n += 1
+            if st == 0:
+                break
+        runs[nt] = (time.time() - t0, n)
+    t0 = time.time()
+    rs, _ = d1["job"].run(cap=0)
+    tg = time.time() - t0
+    d1["job"].close()
+    V1 = s1.n_r * s1.n_t * int(2 * s1.thickness)
+    out["haadf_128x16x64_61v"] = {
+        "cpu_seconds": runs[threads][0], "cpu_sweeps": runs[threads][1], "cpu_threads": threads,
+        "cpu_voxel_updates_per_s": V1 * runs[threads][1] / runs[threads][0],
+        "cpu_1core_seconds": runs[1][0], "cpu_1core_voxel_updates_per_s": V1 * runs[1][1] / runs[1][0],
+        "gpu_seconds": tg, "gpu_sweeps": rs.sweeps,
+        "gpu_voxel_updates_per_s": V1 * rs.sweeps / tg}
+    log(f"cpu: 128x16x64 run {out['haadf_128x16x64_61v']}")
+    return out
 
 
 def batched(ctx, wl, n, steps, single_value):
@@ -211,6 +369,32 @@ def tune_on(ctx, inputs, k=16):
     return out
 
 
+def batched_roofline(ctx, inputs):
+    """The batched line kernel over all the slab problems at once (mbir_batch_sweep: each
+    launch carries one conflict-free step of every job): algorithmic bytes of two steady
+    sweeps over the device time of their launches (HIP events around each batch sweep)."""
+    from mbirtem import capi
+    jobs = [capi.Job(ctx, w["geom"], w["angles"], w["sinogram"], **w["job_kw"]) for w in inputs]
+    n = len(jobs)
+    capi.batch_sweep(ctx, jobs, inner=[0] * n)
+    jobs[0].set_timing(True)
+    nbytes, ms, launches = 0.0, 0.0, 0
+    for k in range(2):
+        sts = capi.batch_sweep(ctx, jobs, inner=[1 + k] * n)
+        nbytes += sum(algorithmic_bytes(st.nnz_updated, st.updated, st.zero_skipped) for st in sts)
+        ms += sts[0].kernel_ms
+        launches += max(st.steps for st in sts)
+    for j in jobs:
+        j.close()
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    log(f"batched roofline: {nbytes / 1e9:.2f} GB in {ms:.1f} ms -> {achieved:.0f} GB/s")
+    return {"kernel": "k_icd_batch_step", "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "bytes_model": BYTES_MODEL, "bytes_per_sweep": nbytes / 2, "kernel_ms_per_sweep": ms / 2,
+            "steps_per_sweep": launches // 2,
+            "timing": "HIP events around each batched sweep's launches"}
+
+
 def slab_figures(ctx, n, rate, big_frac, only_batched=False):
     """batched_slabs (all n problems at once) and served (Poisson arrivals) figures."""
     import numpy as np
@@ -272,6 +456,7 @@ def slab_figures(ctx, n, rate, big_frac, only_batched=False):
         "peak_resident": int(st.peak_resident), "peak_device_gb": st.peak_reserved_bytes / 1e9,
         "autotuned_forms": tuning,
     }
+    batched["roofline"] = batched_roofline(ctx, inputs)
     log(f"batched slabs: {batched['value']:.4g} visits/s vs plain {batched['plain_value']:.4g} "
         f"({batched['vs_plain']:.2f}x), p99 {batched['time_to_converged_p99_s']:.2f}s vs "
         f"{batched['plain_time_to_converged_p99_s']:.2f}s")
@@ -393,6 +578,9 @@ def bf_figures(ctx, steps):
 
 def main():
     args = parse()
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine measures one GPU; --gpus N>1 is not run "
                                    "(multi-GPU sharding is a design, not a measured path)"}))
@@ -440,19 +628,20 @@ def main():
     rl_bytes, rl_ms, rl_launches = 0.0, 0.0, 0
     for _ in range(2):
         st = step()
-        rl_bytes += 12.0 * st.nnz_updated + 8.0 * st.updated + 4.0 * st.zero_skipped
+        rl_bytes += algorithmic_bytes(st.nnz_updated, st.updated, st.zero_skipped)
         rl_ms += st.kernel_ms
         rl_launches += st.steps
     job.set_timing(False)
     achieved = rl_bytes / (rl_ms * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(args.traffic_json):
-        with open(args.traffic_json) as f:
-            traffic = json.load(f).get(wl.name)
+    traffic, traffic_note = None, "not measured (--no-traffic)"
+    if not args.no_traffic:
+        traffic, traffic_note = measure_traffic(wl.name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_icd_step", "avg_launch_us": round(1e3 * rl_ms / rl_launches, 3),
-                "bytes_per_launch": round(rl_bytes / rl_launches, 1)}
+                "bytes_per_launch": round(rl_bytes / rl_launches, 1),
+                "bytes_model": BYTES_MODEL, "timing": "HIP events around each sweep's launches "
+                "on the job's stream, divided by the launches", "traffic_source": traffic_note}
     cost = job.cost()
     out = {
         "metric": "voxel-updates/s",
@@ -486,8 +675,9 @@ def main():
     if not args.no_bf:
         out["bright_field"] = bf_figures(ctx, args.bf_steps)
     if not args.no_cpu:
+        # every core of the host's share (16 per GPU on the benchmark boxes)
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(wl, w["angles"], w["sinogram"], threads)
+        out["cpu_baseline"] = cpu_baseline(wl, w["angles"], w["sinogram"], threads, ctx)
     else:
         out["cpu_baseline"] = None
     job.close()

@@ -138,7 +138,14 @@ typedef struct mbir_problem {
    * the volume (mbir_job_estimate_prior).  HAADF single-job runs only (the server refuses
    * such jobs with -3). */
   int32_t estimate_prior;
-  int32_t pad2_;
+  /* y-slabs of the conflict-free schedule (no reference counterpart; the reference splits
+   * y into one slab per thread, HAADF/HAADF_ReconstructionEngine_UpdateVoxels.cpp:671-715):
+   * 1 = each step updates whole lines; n >= 2 = the slices are cut into n slabs and a step
+   * updates one conflict-free row class per slab, the classes of neighbouring slabs at
+   * least two planes apart in z, so the units of a step still touch disjoint detector
+   * rows and are never prior neighbours (every step equals a sequential ICD order);
+   * 0 = chosen from the sizes.  Schedule 0 only; bright-field NHICD lists run whole lines. */
+  int32_t y_slabs;
 } mbir_problem;
 
 /* Per-sweep statistics (updateVoxels, HAADF_ReconstructionEngine_UpdateVoxels.cpp:717-786) */

@@ -228,6 +228,16 @@ extern "C" int32_t mbir_geometry_init(int32_t N_r, int32_t N_t, int32_t N_theta,
 
 namespace {
 
+// Slabs of the automatic y split: whole lines up to 64 slices (a wave's lanes), else
+// units of at most 64 slices, so that every job runs the one-lane-row form of the line
+// update (its launch time grows with the slices per lane, while up to about 150
+// workgroups of units cost no more than one of lines).
+int auto_y_slabs(int Ny, int Nz)
+{
+  if (Nz < 4 || Ny <= 64) return 1;
+  return (Ny + 63) / 64;
+}
+
 // The reference's random list (GenRandList, BrightField/BFReconstructionEngine_Extra.cpp:
 // 1072-1105) of 0 .. n - 1 with a seeded MT19937 in place of its clock-seeded generator:
 // entry t is Counter[Index], Index = next() % remaining, then Counter[Index] takes the last
@@ -296,6 +306,18 @@ int setup_job(mbir_job* j)
   int S = cmin > 1e-6 ? (int)std::ceil(need / cmin - 1e-9) : g.N_x;
   S = std::max(2, std::min(S, g.N_x));
   j->spacing = S;
+  // y-slabs: units of a step lie in different slabs, the row classes of neighbouring slabs
+  // zshift >= 2 planes apart (and N_z - zshift >= 2 across the wrap), so they are never
+  // prior neighbours; slices of different slabs touch disjoint detector rows
+  {
+    int ns = P.y_slabs;
+    if (ns == 0) ns = auto_y_slabs(g.N_y, g.N_z);
+    ns = std::max(1, std::min(ns, g.N_y));
+    if (P.schedule != 0 || P.bf_nhicd || g.N_z < 4) ns = 1;
+    j->ysplit = ns;
+    j->zshift = ns > 1 ? std::max(2, std::min(g.N_z / ns, g.N_z - 2)) : 0;
+    j->J.unit_max = (g.N_y + ns - 1) / ns;
+  }
 
   // ROI mask (…_Extra.cpp:227-248)
   std::vector<uint8_t> mask(L);
@@ -349,7 +371,9 @@ int setup_job(mbir_job* j)
   HIP_TRY(j->cost.alloc(4 + 2 * 1024));
   HIP_TRY(j->mask.alloc(L));
   HIP_TRY(j->line_nnz.alloc(L));
-  HIP_TRY(j->sched.alloc(L));
+  HIP_TRY(j->sched.alloc(j->sched_cap()));
+  HIP_TRY(j->ustat.alloc(j->sched_cap()));
+  HIP_TRY(j->red.alloc(5 * kReduceBlocks));
   HIP_TRY(j->acc.alloc(1));
   HIP_TRY(j->colv.alloc(kMaxSlots));
   HIP_TRY(j->colr.alloc(kMaxSlots));
@@ -431,7 +455,7 @@ int setup_job(mbir_job* j)
   J.spare_b = (uint32_t)(M * 4);
   J.x = j->x.p; J.e = j->e.p; J.y = j->y.p; J.wfull = nullptr; J.wview = j->wview.p;
   J.I0 = j->I0.p; J.mu = j->mu.p;
-  J.sched = j->sched.p; J.acc = j->acc.p;
+  J.sched = j->sched.p; J.ustat = j->ustat.p;
   // one job alone: the atomic e-update is the faster form (the batch and server paths
   // switch to the plain form for conflict-free schedules)
   J.e_atomic = 1;
@@ -471,9 +495,10 @@ int setup_job(mbir_job* j)
   j->nnz_total = tot;
   {
     // LDS of the line update: the longest column padded to a multiple of 4 P entries,
-    // P = 8 * 64 / TY streams for TY = min(64, the power of two >= N_y)
+    // P = 8 * 64 / TY streams for TY = min(64, the power of two >= the shortest unit)
+    const int umin = g.N_y / j->ysplit;
     int ty = 1;
-    while (ty < g.N_y && ty < 64) ty <<= 1;
+    while (ty < umin && ty < 64) ty <<= 1;
     const int q = 4 * kIcdWaves * (64 / ty);
     int nmax = 1;
     for (size_t l = 0; l < L; ++l) nmax = std::max(nmax, j->nnz_h[l]);
@@ -527,30 +552,37 @@ void build_schedule(mbir_job* j, bool drop_empty_steps)
   std::shuffle(steps.begin(), steps.end(), rng);
   j->sched_h.clear();
   j->step_ptr.assign(1, 0);
+  const int ns = j->ysplit, Ny = j->g.N_y;
   for (auto& st : steps) {
     for (int k = 0; k < R; ++k) {
-      const int z = st.first + k * Zs;
-      if (z >= Nz) continue;
-      for (int x = st.second; x < Nx; x += S)
-        if (j->nnz_h[(size_t)z * Nx + x] > 0) {  // (:245-249)
-          const int l = z * Nx + x;
-          j->sched_h.push_back(LineRef{l, j->cptr_h[l], j->nnz_h[l], 0});
-        }
+      for (int sl = 0; sl < ns; ++sl) {
+        // slab sl: slices [sl Ny / ns, (sl + 1) Ny / ns), the row class moved sl zshift
+        // planes (a bijection of the classes, so each unit comes once per sweep)
+        const int zc = st.first + k * Zs;
+        if (zc >= Nz) continue;
+        const int z = ns > 1 ? (zc + sl * j->zshift) % Nz : zc;
+        const int ya = (int)((int64_t)sl * Ny / ns), yb = (int)((int64_t)(sl + 1) * Ny / ns);
+        const int pad = ns > 1 ? (ya | ((yb - ya) << 16)) : 0;
+        for (int x = st.second; x < Nx; x += S)
+          if (j->nnz_h[(size_t)z * Nx + x] > 0) {  // (:245-249)
+            const int l = z * Nx + x;
+            j->sched_h.push_back(LineRef{l, j->cptr_h[l], j->nnz_h[l], pad});
+          }
+      }
     }
     if (drop_empty_steps && (int)j->sched_h.size() == j->step_ptr.back()) continue;
     j->step_ptr.push_back((int)j->sched_h.size());
   }
 }
 
-// Upload the sweep's schedule (from sched_src, default the host schedule) and
-// clear its accumulators, on stream s.
+// Upload the sweep's schedule (from sched_src, default the host schedule) on stream s
+// (the accumulators are formed from the units' statistics after the sweep's last step).
 int prepare_sweep_on(mbir_job* j, hipStream_t s, const LineRef* sched_src)
 {
   if (!sched_src) sched_src = j->sched_h.data();
   if (!j->sched_h.empty())
     HIP_TRY(hipMemcpyAsync(j->sched.p, sched_src, j->sched_h.size() * sizeof(LineRef),
                            hipMemcpyHostToDevice, s));
-  HIP_TRY(launch_fill_f64((double*)j->acc.p, 0.0, sizeof(SweepAccum) / sizeof(double), s));
   return MBIR_OK;
 }
 
@@ -634,8 +666,7 @@ int nuisance_update_on(mbir_job* j, hipStream_t s)
 
 namespace {
 
-// Schedule of the job's next sweep, uploaded on the job stream with its
-// accumulators cleared.
+// Schedule of the job's next sweep, uploaded on the job stream.
 int prepare_sweep(mbir_job* j, bool drop_empty_steps = false)
 {
   build_schedule(j, drop_empty_steps);
@@ -667,10 +698,11 @@ int run_sweep_steps(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
   }
   for (int k = 0; k < nsteps; ++k) {
     const int b = j->step_ptr[k], n = j->step_ptr[k + 1] - b;
-    HIP_TRY(launch_icd_step(j->J, j->sched.p + b, n, outer, inner, j->prob.zero_skipping,
-                            j->acc.p, s, j->sched_h.data() + b));
+    HIP_TRY(launch_icd_step(j->J, j->sched.p + b, n, b, outer, inner, j->prob.zero_skipping, s,
+                            j->sched_h.data() + b));
   }
   if (j->timing) HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(launch_sweep_reduce(j->sched.p, j->ustat.p, (int)j->sched_h.size(), j->acc.p, j->red.p, s));
   HIP_TRY(hipStreamSynchronize(s));
   float ms = 0;
   if (j->timing) {
@@ -750,9 +782,10 @@ int run_list_steps(mbir_job* j, const std::vector<int>& lines, int outer, int in
   const int nsteps = (int)j->step_ptr.size() - 1;
   for (int k = 0; k < nsteps; ++k) {
     const int b = j->step_ptr[k], n = j->step_ptr[k + 1] - b;
-    HIP_TRY(launch_icd_step(j->J, j->sched.p + b, n, outer, inner, j->prob.zero_skipping,
-                            j->acc.p, s, j->sched_h.data() + b));
+    HIP_TRY(launch_icd_step(j->J, j->sched.p + b, n, b, outer, inner, j->prob.zero_skipping, s,
+                            j->sched_h.data() + b));
   }
+  HIP_TRY(launch_sweep_reduce(j->sched.p, j->ustat.p, (int)j->sched_h.size(), j->acc.p, j->red.p, s));
   return finish_sweep(j, 0.0, nullptr);
 }
 
@@ -780,8 +813,8 @@ int fetch_magmap(mbir_job* j)
 
 // kernel_mode 3: the step launches of a sweep replayed from a HIP graph.  The graph
 // is captured once per job (again when the job descriptor or the schedule's shape
-// changes); a sweep uploads its permuted schedule into the padded slots, clears the
-// accumulators with a kernel and replays the graph.  The first sweep (zero skipping
+// changes); a sweep uploads its permuted schedule into the padded slots, replays the
+// graph and sums the units' statistics by padded slot.  The first sweep (zero skipping
 // off) runs with direct launches.
 int run_sweep_graph(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
 {
@@ -791,13 +824,18 @@ int run_sweep_graph(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
   const int nsteps = (int)j->step_ptr.size() - 1;
   int width = 1;
   for (int k = 0; k < nsteps; ++k) width = std::max(width, j->step_ptr[k + 1] - j->step_ptr[k]);
-  const bool stale = !j->graph_exec || nsteps != j->graph_steps || width != j->graph_lines ||
-                     std::memcmp(&j->graph_J, &j->J, sizeof(DevJob)) != 0;
+  const bool stale0 = !j->graph_exec || nsteps != j->graph_steps || width != j->graph_lines;
   const size_t slots = (size_t)nsteps * width;
   if (j->gsched.n < slots) {
     j->gsched.release();
     HIP_TRY(j->gsched.alloc(slots));
   }
+  if (j->ustat.n < slots) {  // statistics by padded slot (the descriptor changes: recapture)
+    j->ustat.release();
+    HIP_TRY(j->ustat.alloc(slots));
+    j->J.ustat = j->ustat.p;
+  }
+  const bool stale = stale0 || std::memcmp(&j->graph_J, &j->J, sizeof(DevJob)) != 0;
   if (stale) {
     if (j->graph_exec) (void)hipGraphExecDestroy(j->graph_exec);
     if (j->graph) (void)hipGraphDestroy(j->graph);
@@ -807,8 +845,8 @@ int run_sweep_graph(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipError_t le = hipSuccess;
     for (int k = 0; k < nsteps && le == hipSuccess; ++k)
-      le = launch_icd_step(j->J, j->gsched.p + (size_t)k * width, width, 1, 1,
-                           j->prob.zero_skipping, j->acc.p, s);
+      le = launch_icd_step(j->J, j->gsched.p + (size_t)k * width, width, k * width, 1, 1,
+                           j->prob.zero_skipping, s);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(s, &g);
     HIP_TRY(le);
@@ -825,7 +863,6 @@ int run_sweep_graph(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
       j->gsched_h[(size_t)k * width + (i - j->step_ptr[k])] = j->sched_h[i];
   HIP_TRY(hipMemcpyAsync(j->gsched.p, j->gsched_h.data(), slots * sizeof(LineRef),
                          hipMemcpyHostToDevice, s));
-  HIP_TRY(launch_fill_f64((double*)j->acc.p, 0.0, sizeof(SweepAccum) / sizeof(double), s));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (j->timing) {
     HIP_TRY(hipEventCreate(&e0));
@@ -834,6 +871,7 @@ int run_sweep_graph(mbir_job* j, int outer, int inner, mbir_sweep_stats* st)
   }
   HIP_TRY(hipGraphLaunch(j->graph_exec, s));
   if (j->timing) HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(launch_sweep_reduce(j->gsched.p, j->ustat.p, (int)slots, j->acc.p, j->red.p, s));
   HIP_TRY(hipStreamSynchronize(s));
   float ms = 0;
   if (j->timing) {
@@ -860,7 +898,7 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
     if (j->cancel.load()) return MBIR_ERR_CANCELLED;
     // bright field: one detector row per slice, full sweeps (NHICD lists run alone);
     // HAADF with a bright-field sinogram: single-job paths only
-    if (j->J.N_y > 128 || j->J.vlr_max > 2 || j->J.bfs ||
+    if (j->J.unit_max > 128 || j->J.vlr_max > 2 || j->J.bfs ||
         (j->J.wfull && (j->prob.bf_nhicd || j->J.vlr_max > 1)))
       return MBIR_ERR_UNSUPPORTED;
     int rc = prepare_sweep(j);
@@ -909,7 +947,7 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
         cur_fw = e.fw;
       }
       e.wg_off = wg;
-      e.acc = j->acc.p;
+      e.pos = b;
       ent.push_back(e);
       wg += cnt;
     }
@@ -971,6 +1009,8 @@ int run_batch(mbir_ctx* ctx, const std::vector<mbir_job*>& jobs, const int* oute
     }
   }
   if (timing) HIP_TRY(hipEventRecord(e1, s));
+  for (mbir_job* j : jobs)
+    HIP_TRY(launch_sweep_reduce(j->sched.p, j->ustat.p, (int)j->sched_h.size(), j->acc.p, j->red.p, s));
   HIP_TRY(hipStreamSynchronize(s));
   float ms = 0;
   if (timing) {

@@ -75,6 +75,10 @@ struct mbir_job {
   DevJob J{};
   double offsetR = 0, offsetT = 0;
   int spacing = 1;  // conflict-free x spacing of lines in one step
+  int ysplit = 1;   // y-slabs of the schedule: a step updates one unit (line, slab) per
+                    // line of its row class in every slab (1 = whole lines)
+  int zshift = 0;   // z offset between the row classes of neighbouring slabs in a step
+  size_t sched_cap() const { return (size_t)g.N_z * g.N_x * ysplit; }
   std::vector<double> angles_rad, H;
   std::vector<double> I0h, muh, alphah;
   // device buffers
@@ -91,6 +95,8 @@ struct mbir_job {
   int64_t nnz_total = 0;
   DevBuf<uint8_t> mask;
   DevBuf<SweepAccum> acc;
+  DevBuf<mbir::UnitStat> ustat;  // per schedule position: the unit's sweep statistics
+  DevBuf<double> red;             // block sums of the statistics (5 kReduceBlocks)
   std::vector<LineRef> sched_h;
   std::vector<int> step_ptr;
   std::vector<int> cptr_h;  // column-table offsets of all lines (host copy)
@@ -123,7 +129,7 @@ struct mbir_job {
     cosv.release(); sinv.release(); Hd.release(); I0.release(); mu.release(); alpha.release();
     I0b.release(); mub.release(); sums6.release(); yest.release(); cost.release(); prj.release();
     vlr_t0.release(); vlr_n.release(); line_nnz.release(); sched.release(); colr.release();
-    mask.release(); acc.release(); col_ptr.release(); col_row.release();
+    mask.release(); acc.release(); ustat.release(); red.release(); col_ptr.release(); col_row.release();
     col_a.release(); col_ka.release(); col_kw.release(); wfull.release(); bfs.release(); alpha_b.release();
     gsched.release();
     magmap.release();

@@ -31,9 +31,24 @@ struct LineRef {
 // The lines of one step carried in the kernel arguments (single-job launches of at
 // most kInlineLines lines): they arrive with the job descriptor in the launch's first
 // memory round trip instead of one dependent load later.
-constexpr int kInlineLines = 192;
+// Each entry is 8 bytes, {line | nc << 20, p0} (whole lines of jobs with fewer than
+// 2^20 lines), and the launch takes the smallest of two capacities that holds the step:
+// every kilobyte of kernel arguments costs about half a microsecond per launch.
+constexpr int kInlineSmall = 96;
+constexpr int kInlineLines = 256;
+template <int CAP>
 struct StepLines {
-  LineRef l[kInlineLines];
+  int2 l[CAP > 0 ? CAP : 1];
+};
+
+// A unit's share of its sweep's stop-test statistics, stored by the line update at the
+// unit's schedule position (no atomics) and summed in schedule order once per sweep:
+// sums of |x_new - x_old| and |x_old| over its ROI slices, and the counts zero-skipped
+// + 2^10 updated + 2^20 detector rows of its updated slices.
+struct UnitStat {
+  float su, sv;
+  int cnt;
+  int pad;
 };
 
 struct SweepAccum {  // device accumulators of one sweep
@@ -85,6 +100,8 @@ struct DevJob {
   int e_atomic;          // 1: error-sinogram update by fp32 atomics (required when lines of
                          // a step may share detector bins: relaxed schedule), 0: plain
                          // read-modify-write (conflict-free steps only)
+  int unit_max;          // slices of the job's longest schedule unit (N_y for whole lines):
+                         // the line kernels' shape
   int col_slots;         // padded column entries of this job's longest line (a multiple of
                          // 4 P, P = 8 * 64 / TY streams): the line update's dynamic LDS
   uint32_t spare_b;      // M * 4: byte offset of the spare detector row (N_t zeros) that
@@ -93,8 +110,8 @@ struct DevJob {
   const float* bfs;      // HAADF with a bright-field sinogram: per-element scale (counts +
                          // BF_OFFSET), spare row 0; null without
                          // (NHICD), nullptr = not kept
-  const LineRef* sched;  // [N_z * N_x] lines of the current sweep, step by step
-  SweepAccum* acc;       // the current sweep's accumulators
+  const LineRef* sched;  // [N_z * N_x * y slabs] units of the current sweep, step by step
+  UnitStat* ustat;       // per schedule position: the unit's statistics of the sweep
 };
 
 
@@ -107,7 +124,7 @@ struct BatchEntry {
   int flags;         // bit 0: first sweep, bit 1: zero skipping on
   int ypl, nv;       // kernel shape
   int fw;            // bright field (per-entry weights): kernels of their own
-  SweepAccum* acc;
+  int pos;           // schedule position of the entry's first unit in its job's sweep
 };
 
 // One job's share of one request-server launch: lines sched[line0 .. line0 + nlines)
@@ -160,9 +177,15 @@ hipError_t launch_upsample2(const float* coarse, float* fine, const int* line_nn
 // host_lines: the same nlines entries in host memory; steps of at most kInlineLines
 // lines then travel in the kernel arguments (not for graph capture, whose step slots
 // are rewritten in device memory between replays).
-hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, int outer, int inner,
-                           int zero_skipping, SweepAccum* acc, hipStream_t s,
+// pos0: the schedule position of lines[0] (the units' statistics go to J.ustat[pos0 + i])
+hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, int pos0, int outer,
+                           int inner, int zero_skipping, hipStream_t s,
                            const LineRef* host_lines = nullptr);
+// acc = the sums of the n units' statistics of sched / ustat (entries with line < 0 are
+// padding), in schedule order; part holds 5 kReduceBlocks doubles
+constexpr int kReduceBlocks = 128;
+hipError_t launch_sweep_reduce(const LineRef* sched, const UnitStat* ustat, int n, SweepAccum* acc,
+                               double* part, hipStream_t s);
 void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv);
 // one step of several jobs of one kernel shape: entries [e0, e0 + ne) of ent,
 // nwg workgroups in total

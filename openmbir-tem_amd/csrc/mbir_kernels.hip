@@ -426,7 +426,8 @@ template <int YPL>
 struct IcdSmem {
   static constexpr int NY = 64 * YPL;
   float nbl[9][NY + 2];  // lines (z+dz, x+dx), y = -1 .. N_y (0 outside the volume)
-  float x[NY];           // this line; the even phase writes its new values here
+  float xp[NY + 2];      // this unit's slices at xp[1 ..], the slices just below and above
+                         // it at the ends; the even phase writes its new values here
   float dx[NY];
   double T1[NY], T2[NY];
   // theta partial sums per wave and slice (FP64)
@@ -475,8 +476,8 @@ constexpr int icd_waves_per_eu(bool pf)
 // latency, more registers: the form for launches that do not fill the device).
 // SB: HAADF with a bright-field sinogram, every entry's element scaled by J.bfs
 template <int YPL, int NV, bool FW, bool PF, bool SB = false>
-__device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_on,
-                         SweepAccum* acc, IcdSmem<YPL>& sm, const IcdCol col)
+__device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_on, int pos,
+                         IcdSmem<YPL>& sm, const IcdCol col)
 {
   // entries per lane and batch, a multiple of four (about 24 loads in flight per lane)
   constexpr int UR = 24 / (YPL * NV);
@@ -487,7 +488,13 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   const int line = __builtin_amdgcn_readfirstlane(ref.line);
   const int z = line / J.N_x, xi = line % J.N_x;
   const bool roi = gp(J.mask)[line] != 0;  // used by the solve
-  const size_t lbase = (size_t)line * Ny;
+  // the unit: slices y0 .. y0 + ny - 1 of the line (ref.pad = y0 | ny << 16; 0 = the
+  // whole line).  Units of one launch lie on disjoint detector rows and are never prior
+  // neighbours, so the slices just outside a unit are read, not written, by this launch.
+  const int upad = __builtin_amdgcn_readfirstlane(ref.pad);
+  const int ny = (upad >> 16) ? (upad >> 16) : Ny;
+  const int y0 = (upad >> 16) ? (upad & 0xffff) : 0;
+  const size_t lbase = (size_t)line * Ny + y0;
   const bool zs_active = zs_on && !first;
   const int p0 = __builtin_amdgcn_readfirstlane(ref.p0);
   const int nc = __builtin_amdgcn_readfirstlane(ref.nc);
@@ -498,7 +505,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   // point at the spare detector row behind the error sinogram with zero gains, so
   // whole groups of four entries are loaded and updated without per-entry tests ----
   int lg = 0;
-  while ((1 << lg) < Ny && lg < 6) ++lg;
+  while ((1 << lg) < ny && lg < 6) ++lg;
   const int TY = 1 << lg, RPW = 64 >> lg, sub = lane >> lg, yl = lane & (TY - 1);
   const int P = kIcdWaves * RPW, p = wave * RPW + sub;
   const int C = (((nc + P - 1) >> (3 + 6 - lg)) + 3) & ~3;  // entries per stream, 4 | C
@@ -512,7 +519,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
     const int y = yl + TY * jy;
-    const int yc = y < Ny ? y : 0;
+    const int yc = y < ny ? y0 + y : 0;
     vt0_[jy] = gp(J.vlr_t0)[yc];
     vn_[jy] = gp(J.vlr_n)[yc];
 #pragma unroll
@@ -525,20 +532,25 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   // into the arrays, values selected afterwards) and nothing computes on a loaded
   // value before the last load is issued: a load under a per-lane branch, or one
   // whose value is used at once, is waited for before the next load goes out ----
-  float nbv[2][YPL];
-  bool nbin[2][YPL];
+  float nbv[2][YPL], nbh[2];
+  bool nbin[2][YPL], nbhin[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int l = wave + 8 * h;
     const int zz = z + l / 3 - 1, xx = xi + l % 3 - 1;
     const bool in = l < 9 && zz >= 0 && zz < J.N_z && xx >= 0 && xx < J.N_x;
-    const gptr<const float> src = gp(J.x) + ((size_t)(in ? zz : 0) * J.N_x + (in ? xx : 0)) * Ny;
+    const gptr<const float> src =
+        gp(J.x) + ((size_t)(in ? zz : 0) * J.N_x + (in ? xx : 0)) * Ny + (in ? y0 : 0);
 #pragma unroll
     for (int jy = 0; jy < YPL; ++jy) {
       const int y = lane + 64 * jy;
-      nbv[h][jy] = src[y < Ny ? y : Ny - 1];
-      nbin[h][jy] = in && y < Ny;
+      nbv[h][jy] = src[y < ny ? y : ny - 1];
+      nbin[h][jy] = in && y < ny;
     }
+    // the slice below (even lanes) and above (odd lanes) the unit, 0 outside the volume
+    const int hy = (lane & 1) ? ny : -1;
+    nbhin[h] = in && y0 + hy >= 0 && y0 + hy < Ny;
+    nbh[h] = src[nbhin[h] ? hy : 0];
   }
   constexpr int kColIt = kMaxSlots / kIcdBlock;
   uint32_t rw_[kColIt];
@@ -558,10 +570,10 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
 #pragma unroll
     for (int u = 0; u < U; ++u) r0[u] = (uint32_t)gp(J.col_row)[p0 + min(s0 + u, nc - 1)];
   }
-  const int vn_t = gp(J.vlr_n)[tid < Ny ? tid : 0];
-  float vl_t[NV];  // the slice's voxel-line response values (thread tid < N_y)
+  const int vn_t = gp(J.vlr_n)[tid < ny ? y0 + tid : 0];
+  float vl_t[NV];  // the slice's voxel-line response values (thread tid < ny)
 #pragma unroll
-  for (int k = 0; k < NV; ++k) vl_t[k] = gp(J.vlr_val)[(tid < Ny ? tid : 0) * kVlrCap + k];
+  for (int k = 0; k < NV; ++k) vl_t[k] = gp(J.vlr_val)[(tid < ny ? y0 + tid : 0) * kVlrCap + k];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int l = wave + 8 * h;
@@ -569,10 +581,10 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
 #pragma unroll
       for (int jy = 0; jy < YPL; ++jy) {
         const int y = lane + 64 * jy;
-        if (y < Ny) sm.nbl[l][y + 1] = nbin[h][jy] ? nbv[h][jy] : 0.f;
+        if (y < ny) sm.nbl[l][y + 1] = nbin[h][jy] ? nbv[h][jy] : 0.f;
       }
-      if (lane == 0) sm.nbl[l][0] = 0.f;
-      if (lane == 1) sm.nbl[l][Ny + 1] = 0.f;
+      if (lane == 0) sm.nbl[l][0] = nbhin[h] ? nbh[h] : 0.f;
+      if (lane == 1) sm.nbl[l][ny + 1] = nbhin[h] ? nbh[h] : 0.f;
     }
   }
   // theta2 of slice y is (sum_k vlr_yk^2) * sum over the column of (I0 a)(I0 a w):
@@ -595,7 +607,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   bool vok[YPL][NV];
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
-    const int n = yl + TY * jy < Ny ? vn_[jy] : 0;
+    const int n = yl + TY * jy < ny ? vn_[jy] : 0;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       vok[jy][k] = k < n;
@@ -642,9 +654,14 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
     if (lane == 0) sm.kk[wave] = kk;
   }
   __syncthreads();  // A: neighbour lines and column visible
-  if (tid < Ny) {  // read after barrier D
-    sm.x[tid] = sm.nbl[4][tid + 1];
+  float* const smx = sm.xp + 1;  // this unit's slices, smx[-1] and smx[ny] outside it
+  if (tid < ny) {  // read after barrier D
+    smx[tid] = sm.nbl[4][tid + 1];
     sm.vns[tid] = vn_t;
+  }
+  if (tid == kIcdBlock - 1) {  // (ny < kIcdBlock)
+    smx[-1] = sm.nbl[4][0];
+    smx[ny] = sm.nbl[4][ny + 1];
   }
   double vsq_t = 0.0;  // sum_k vlr^2 of slice tid: theta2 / column factor
 #pragma unroll
@@ -776,7 +793,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       if constexpr (FW || SB) sm.pd2[wave][yl + TY * jy] = d2[jy];
     }
   __syncthreads();  // D
-  if (tid < Ny) {
+  if (tid < ny) {
     double T1 = 0.0, T2 = 0.0;
 #pragma unroll
     for (int w = 0; w < kIcdWaves; ++w) T1 += sm.pd1[w][tid];
@@ -804,7 +821,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   int ndy[NS];
   bool nok[NS];
   float nw[NS];
-  const float* nsrc[NS];  // slice 0 of the neighbour line in LDS (this line: sm.x)
+  const float* nsrc[NS];  // the unit's first slice of the neighbour line in LDS (this line: smx)
 #pragma unroll
   for (int q = 0; q < NS; ++q) {
     const int n = gl + q * G;
@@ -814,7 +831,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
     ndy[q] = dy;
     nok[q] = n < 26 && z + dz >= 0 && z + dz < J.N_z && xi + dx >= 0 && xi + dx < J.N_x;
     nw[q] = filter_weight(dy, dz, dx);
-    nsrc[q] = l == 4 ? sm.x : &sm.nbl[l < 9 ? l : 0][1];
+    nsrc[q] = l == 4 ? smx : &sm.nbl[l < 9 ? l : 0][1];
   }
   // this lane's statistics over both phases (group leaders only): |dx| and |x| in
   // the ROI, |dx| everywhere, and the counts zero-skipped + 2^10 updated + 2^20
@@ -822,10 +839,10 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   float lsu = 0.f, lsv = 0.f, lmg = 0.f;
   int lcnt = 0;
   for (int phase = 0; phase < 2; ++phase) {
-    const int vpp = (Ny + 1 - phase) / 2;  // slices of this parity
-    const int y = 2 * vi + phase;
+    const int vpp = (ny + 1 - phase) / 2;  // slices of this parity
+    const int y = 2 * vi + phase;           // within the unit
     const bool active = vi < vpp;
-    const float v = active ? sm.x[y] : 0.f;
+    const float v = active ? smx[y] : 0.f;
     const double T1 = active ? sm.T1[y] : 0.0, T2 = active ? sm.T2[y] : 0.0;
     float s1 = 0.f, s2 = 0.f, lo = 0.f, hi = 0.f;  // lo/hi include the zeroed centre
     if (active) {
@@ -833,9 +850,9 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       bool ex[NS];
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
-        const int yy = y + ndy[q];
-        ex[q] = nok[q] && yy >= 0 && yy < Ny;
-        const float t = nsrc[q][yy < 0 ? 0 : (yy < Ny ? yy : Ny - 1)];
+        const int yy = y + ndy[q];  // -1 .. ny: inside the unit or just outside it
+        ex[q] = nok[q] && y0 + yy >= 0 && y0 + yy < Ny;
+        const float t = nsrc[q][yy];
         nbv[q] = ex[q] ? t : 0.f;
       }
 #pragma unroll
@@ -880,7 +897,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
         }
       }
       sm.vst[y] = (unsigned char)vst;
-      sm.x[y] = xnew;
+      smx[y] = xnew;
       sm.dx[y] = xnew - v;
     }
     lsu += su;
@@ -903,21 +920,19 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   }
 
   // ---- 6. sweep statistics: the per-wave shares (visible since the last solve
-  // barrier) summed in a fixed order, one set of global atomics per line; issued
-  // before the error-sinogram update where the line's latency counts (PF), after it
-  // where the device is full ----
+  // barrier) summed in a fixed order and stored at the unit's schedule position (one
+  // 16-byte store; the sweep's sums are formed in schedule order after its last step,
+  // so no two units meet at one address); issued before the error-sinogram update where
+  // the line's latency counts (PF), after it where the device is full ----
   auto line_stats = [&]() {
     if (wave == 0) {
       const float su = sum8(lane < kIcdWaves ? sm.wsu[lane] : 0.f);
       const float sv = sum8(lane < kIcdWaves ? sm.wsv[lane] : 0.f);
       const int cn = sum8(lane < kIcdWaves ? sm.wcnt[lane] : 0);
-      const int nz = cn & 1023, nu = (cn >> 10) & 1023, nn = nc * (cn >> 20);
-      if (lane == 0 && su != 0.f) atomic_add_g(&acc->sum_abs_update, (double)su);
-      if (lane == 0 && sv != 0.f) atomic_add_g(&acc->sum_abs_value, (double)sv);
-      if (lane == 0 && nz) atomic_add_g(&acc->zero_skipped, (unsigned long long)nz);
-      if (lane == 0 && nu) atomic_add_g(&acc->updated, (unsigned long long)nu);
-      if (lane == 0 && nn) atomic_add_g(&acc->nnz_updated, (unsigned long long)nn);
-      if (J.magmap) {  // the line's update magnitude (BFUpdateYSlice.cpp:171, 303-304)
+      typedef float f32x4 __attribute__((ext_vector_type(4)));
+      if (lane == 0) *(gptr<f32x4>)(J.ustat + pos) = f32x4{su, sv, __int_as_float(cn), 0.f};
+      if (J.magmap) {  // the line's update magnitude (BFUpdateYSlice.cpp:171, 303-304); NHICD
+                       // lists run whole lines, so the unit is the line
         const float mgl = sum8(lane < kIcdWaves ? sm.wmg[lane] : 0.f);
         if (lane == 0) gp(J.magmap)[line] = mgl;
       }
@@ -932,8 +947,8 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
 #pragma unroll
   for (int jy = 0; jy < YPL; ++jy) {
     const int y = yl + TY * jy;
-    d[jy] = y < Ny ? sm.dx[y] : 0.f;
-    upd[jy] = y < Ny && sm.vst[y] == 1;
+    d[jy] = y < ny ? sm.dx[y] : 0.f;
+    upd[jy] = y < ny && sm.vst[y] == 1;
 #pragma unroll
     for (int k = 0; k < NV; ++k) vd[jy][k] = __fmul_rn(vv[jy][k], d[jy]);
   }
@@ -1051,15 +1066,15 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
 // INL: the step's lines come in the kernel arguments (`inl`, exactly one per
 // workgroup, no padding), so no branch precedes the argument loads and the line's
 // entry arrives with the job descriptor.
-template <int YPL, int NV, bool FW = false, bool SB = false, bool INL = false>
+template <int YPL, int NV, bool FW = false, bool SB = false, int CAP = 0>
 __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef* lines, int nlines,
-                                                        int first, int zs_on, SweepAccum* acc,
-                                                        StepLines inl)
+                                                        int pos0, int flags, StepLines<CAP> inl)
 {
   __shared__ IcdSmem<YPL> sm;
   LineRef ref;
-  if constexpr (INL) {
-    ref = inl.l[blockIdx.x];
+  if constexpr (CAP > 0) {
+    const int2 c = inl.l[blockIdx.x];
+    ref = LineRef{c.x & 0xfffff, c.y, (int)((unsigned)c.x >> 20), 0};
     // the arguments the line's first loads depend on, fetched in the same round trip
     // as its entry (an empty asm that takes them as operands: no instruction)
     asm volatile(";" ::"s"(ref.line), "s"(ref.p0), "s"(ref.nc), "s"(J.N_x), "s"(J.N_y), "s"(J.NB),
@@ -1071,7 +1086,8 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef*
     ref = lines[blockIdx.x];
     if (__builtin_amdgcn_readfirstlane(ref.line) < 0) return;  // padding of a graph's step slot
   }
-  icd_line<YPL, NV, FW, true, SB>(J, ref, first, zs_on, acc, sm, icd_col(J.col_slots));
+  icd_line<YPL, NV, FW, true, SB>(J, ref, flags & 1, (flags >> 1) & 1, pos0 + (int)blockIdx.x, sm,
+                                  icd_col(J.col_slots));
 }
 
 // One conflict-free step of every job of a batch that has kernel shape
@@ -1097,7 +1113,8 @@ void k_icd_batch_step(const DevJob* __restrict__ jobs,
   if (i < 0 || i >= E.nlines) return;
   const DevJob J = jobs[__builtin_amdgcn_readfirstlane(E.job)];
   const LineRef ref = load_ref(E.lines + i);
-  icd_line<YPL, NV, FW, PF>(J, ref, E.flags & 1, (E.flags >> 1) & 1, E.acc, sm, icd_col(slots));
+  icd_line<YPL, NV, FW, PF>(J, ref, E.flags & 1, (E.flags >> 1) & 1,
+                            __builtin_amdgcn_readfirstlane(E.pos) + i, sm, icd_col(slots));
 }
 
 // One launch of the request server: entries travel by value in the kernel
@@ -1127,7 +1144,7 @@ void k_icd_tick(const DevJob* __restrict__ jobs, const TickArgs args)
   if (i < 0 || i >= nl) return;
   const DevJob J = jobs[job];
   const LineRef ref = load_ref(J.sched + l0 + i);
-  icd_line<YPL, NV, FW, PF>(J, ref, flags & 1, (flags >> 1) & 1, J.acc, sm, icd_col(args.slots));
+  icd_line<YPL, NV, FW, PF>(J, ref, flags & 1, (flags >> 1) & 1, l0 + i, sm, icd_col(args.slots));
 }
 
 // jobs[slot] = J: a job descriptor written in stream order.
@@ -1607,66 +1624,132 @@ hipError_t launch_export_column(const DevJob& J, int z, int x, float* vals, int*
 
 void icd_shape(const DevJob& J, int max_vlr, int* ypl, int* nv)
 {
-  *ypl = J.N_y <= 64 ? 1 : (J.N_y <= 128 ? 2 : 4);
+  const int ny = J.unit_max > 0 ? J.unit_max : J.N_y;
+  *ypl = ny <= 64 ? 1 : (ny <= 128 ? 2 : 4);
   *nv = max_vlr <= 1 ? 1 : (max_vlr == 2 ? 2 : 4);
 }
 
-template <int YPL, bool INL>
-static void launch_step_nv(const DevJob& J, const LineRef* lines, int nlines, int first, int zs,
-                           SweepAccum* acc, hipStream_t s, const StepLines& inl)
+template <int YPL, int CAP>
+static void launch_step_nv(const DevJob& J, const LineRef* lines, int nlines, int pos0, int flags,
+                           hipStream_t s, const StepLines<CAP>& inl)
 {
   const size_t dyn = icd_col_bytes(J.col_slots);
+  const dim3 grid(nlines), block(kIcdBlock);
   if (J.wfull && J.vlr_max <= 1)  // bright field: per-entry weights and selector
-    hipLaunchKernelGGL((k_icd_step<YPL, 1, true, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
-                       s, J, lines, nlines, first, zs, acc, inl);
+    hipLaunchKernelGGL((k_icd_step<YPL, 1, true, false, CAP>), grid, block, dyn, s, J, lines,
+                       nlines, pos0, flags, inl);
   else if (J.wfull)  // bright field, two detector rows per slice (coarse voxels)
-    hipLaunchKernelGGL((k_icd_step<YPL, 2, true, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
-                       s, J, lines, nlines, first, zs, acc, inl);
+    hipLaunchKernelGGL((k_icd_step<YPL, 2, true, false, CAP>), grid, block, dyn, s, J, lines,
+                       nlines, pos0, flags, inl);
   else if (J.bfs && J.vlr_max <= 1)  // HAADF with a bright-field sinogram
-    hipLaunchKernelGGL((k_icd_step<YPL, 1, false, true, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
-                       s, J, lines, nlines, first, zs, acc, inl);
+    hipLaunchKernelGGL((k_icd_step<YPL, 1, false, true, CAP>), grid, block, dyn, s, J, lines,
+                       nlines, pos0, flags, inl);
   else if (J.bfs)
-    hipLaunchKernelGGL((k_icd_step<YPL, 2, false, true, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
-                       s, J, lines, nlines, first, zs, acc, inl);
+    hipLaunchKernelGGL((k_icd_step<YPL, 2, false, true, CAP>), grid, block, dyn, s, J, lines,
+                       nlines, pos0, flags, inl);
   else if (J.vlr_max <= 1)
-    hipLaunchKernelGGL((k_icd_step<YPL, 1, false, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
-                       s, J, lines, nlines, first, zs, acc, inl);
+    hipLaunchKernelGGL((k_icd_step<YPL, 1, false, false, CAP>), grid, block, dyn, s, J, lines,
+                       nlines, pos0, flags, inl);
   else if (J.vlr_max == 2)
-    hipLaunchKernelGGL((k_icd_step<YPL, 2, false, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
-                       s, J, lines, nlines, first, zs, acc, inl);
+    hipLaunchKernelGGL((k_icd_step<YPL, 2, false, false, CAP>), grid, block, dyn, s, J, lines,
+                       nlines, pos0, flags, inl);
   else
-    hipLaunchKernelGGL((k_icd_step<YPL, 4, false, false, INL>), dim3(nlines), dim3(kIcdBlock), dyn,
-                       s, J, lines, nlines, first, zs, acc, inl);
+    hipLaunchKernelGGL((k_icd_step<YPL, 4, false, false, CAP>), grid, block, dyn, s, J, lines,
+                       nlines, pos0, flags, inl);
 }
 
-template <bool INL>
-static void launch_step_ypl(const DevJob& J, const LineRef* lines, int nlines, int first, int zs,
-                            SweepAccum* acc, hipStream_t s, const StepLines& inl)
+template <int CAP>
+static void launch_step_ypl(const DevJob& J, const LineRef* lines, int nlines, int pos0,
+                            int flags, hipStream_t s, const StepLines<CAP>& inl)
 {
   int ypl, nv;
   icd_shape(J, J.vlr_max, &ypl, &nv);
   if (ypl == 1)
-    launch_step_nv<1, INL>(J, lines, nlines, first, zs, acc, s, inl);
+    launch_step_nv<1, CAP>(J, lines, nlines, pos0, flags, s, inl);
   else if (ypl == 2)
-    launch_step_nv<2, INL>(J, lines, nlines, first, zs, acc, s, inl);
+    launch_step_nv<2, CAP>(J, lines, nlines, pos0, flags, s, inl);
   else
-    launch_step_nv<4, INL>(J, lines, nlines, first, zs, acc, s, inl);
+    launch_step_nv<4, CAP>(J, lines, nlines, pos0, flags, s, inl);
 }
 
-hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, int outer, int inner,
-                           int zero_skipping, SweepAccum* acc, hipStream_t s,
-                           const LineRef* host_lines)
+template <int CAP>
+static void launch_inline(const DevJob& J, const LineRef* lines, const LineRef* host_lines,
+                          int nlines, int pos0, int flags, hipStream_t s)
+{
+  StepLines<CAP> inl;  // entries past nlines are never read
+  for (int i = 0; i < nlines; ++i)
+    inl.l[i] = make_int2(host_lines[i].line | (host_lines[i].nc << 20), host_lines[i].p0);
+  launch_step_ypl<CAP>(J, lines, nlines, pos0, flags, s, inl);
+}
+
+hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, int pos0, int outer,
+                           int inner, int zero_skipping, hipStream_t s, const LineRef* host_lines)
 {
   if (nlines <= 0) return hipSuccess;
-  const int first = (outer == 0 && inner == 0) ? 1 : 0;
-  if (host_lines && nlines <= kInlineLines) {
-    StepLines inl;  // entries past nlines are never read
-    std::memcpy(inl.l, host_lines, (size_t)nlines * sizeof(LineRef));
-    launch_step_ypl<true>(J, lines, nlines, first, zero_skipping, acc, s, inl);
-  } else {
-    static const StepLines none{};
-    launch_step_ypl<false>(J, lines, nlines, first, zero_skipping, acc, s, none);
+  const int flags = ((outer == 0 && inner == 0) ? 1 : 0) | (zero_skipping ? 2 : 0);
+  // the entries travel in the arguments when they are whole lines of a job of fewer
+  // than 2^20 lines (8 bytes each)
+  bool inl = host_lines && nlines <= kInlineLines && (int64_t)J.N_z * J.N_x < (1 << 20);
+  for (int i = 0; inl && i < nlines; ++i) inl = host_lines[i].pad == 0 && host_lines[i].nc < 4096;
+  if (inl && nlines <= kInlineSmall)
+    launch_inline<kInlineSmall>(J, lines, host_lines, nlines, pos0, flags, s);
+  else if (inl)
+    launch_inline<kInlineLines>(J, lines, host_lines, nlines, pos0, flags, s);
+  else {
+    static const StepLines<0> none{};
+    launch_step_ypl<0>(J, lines, nlines, pos0, flags, s, none);
   }
+  return hipGetLastError();
+}
+
+// The sweep's stop-test sums (…_UpdateVoxels.cpp:417-421, 765-770) from its units' shares
+// in schedule order, FP64: block b sums positions [b chunk, (b + 1) chunk), then one
+// block adds the blocks' sums in order (the same sums whichever unit finished first).
+__global__ __launch_bounds__(kBlock) void k_sweep_partials(const LineRef* sched, const UnitStat* us,
+                                                           int n, int chunk, double* part)
+{
+  __shared__ double red[kBlock];
+  double su = 0, sv = 0, nz = 0, nu = 0, nn = 0;
+  const int b0 = blockIdx.x * chunk, b1 = min(n, b0 + chunk);
+  for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    const LineRef r = sched[i];
+    if (r.line < 0) continue;  // a graph step's padding
+    const UnitStat u = us[i];
+    su += (double)u.su;
+    sv += (double)u.sv;
+    nz += (double)(u.cnt & 1023);
+    nu += (double)((u.cnt >> 10) & 1023);
+    nn += (double)r.nc * (double)(u.cnt >> 20);
+  }
+  const double v[5] = {su, sv, nz, nu, nn};
+  for (int q = 0; q < 5; ++q) {
+    const double t = block_sum_d(v[q], red);
+    if (threadIdx.x == 0) part[blockIdx.x * 5 + q] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sweep_final(const double* part, int nb, SweepAccum* acc)
+{
+  if (threadIdx.x != 0) return;
+  double t[5] = {0, 0, 0, 0, 0};
+  for (int b = 0; b < nb; ++b)
+    for (int q = 0; q < 5; ++q) t[q] += part[b * 5 + q];
+  SweepAccum a{};
+  a.sum_abs_update = t[0];
+  a.sum_abs_value = t[1];
+  a.zero_skipped = (unsigned long long)t[2];  // counts: exact in FP64 below 2^53
+  a.updated = (unsigned long long)t[3];
+  a.nnz_updated = (unsigned long long)t[4];
+  *acc = a;
+}
+
+hipError_t launch_sweep_reduce(const LineRef* sched, const UnitStat* ustat, int n, SweepAccum* acc,
+                               double* part, hipStream_t s)
+{
+  const int chunk = std::max(2048, (n + kReduceBlocks - 1) / kReduceBlocks);
+  const int nb = std::max(1, (n + chunk - 1) / chunk);
+  hipLaunchKernelGGL(k_sweep_partials, dim3(nb), dim3(kBlock), 0, s, sched, ustat, n, chunk, part);
+  hipLaunchKernelGGL(k_sweep_final, dim3(1), dim3(64), 0, s, (const double*)part, nb, acc);
   return hipGetLastError();
 }
 

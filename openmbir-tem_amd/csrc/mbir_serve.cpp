@@ -163,7 +163,7 @@ void mbir_server::setup_loop()
 
     mbir_job* job = nullptr;
     int rc = mbir_job_create(ctx, &r->prob, &job);
-    const size_t L = rc == 0 ? (size_t)job->g.N_z * job->g.N_x : 0;
+    const size_t L = rc == 0 ? job->sched_cap() : 0;
     if (rc == 0 && (hipHostMalloc((void**)&r->hacc, sizeof(SweepAccum)) != hipSuccess ||
                     hipHostMalloc((void**)&r->hsched, std::max<size_t>(L, 1) * sizeof(LineRef)) !=
                         hipSuccess ||
@@ -171,7 +171,7 @@ void mbir_server::setup_loop()
       rc = MBIR_ERR_HIP;
     // bright field: one detector row per slice and full sweeps (NHICD lists run
     // as single jobs)
-    if (rc == 0 && (job->J.N_y > 128 || job->J.vlr_max > 2 ||
+    if (rc == 0 && (job->J.unit_max > 128 || job->J.vlr_max > 2 ||
                     (job->J.wfull && (job->prob.bf_nhicd || job->J.vlr_max > 1))))
       rc = MBIR_ERR_UNSUPPORTED;
 
@@ -217,7 +217,7 @@ void mbir_server::finish(Req* r, int state, int error)
 }
 
 // Schedule of the job's next sweep: seeded step order, uploaded from pinned
-// memory on the launch stream with the accumulators cleared.
+// memory on the launch stream.
 int mbir_server::start_sweep(Req* r)
 {
   mbir_job* j = r->job;
@@ -237,6 +237,7 @@ int mbir_server::start_sweep(Req* r)
   r->nsteps = (int)j->step_ptr.size() - 1;
   r->waiting = false;
   if (r->nsteps == 0) {  // nothing to update: the sweep ends at once
+    HIP_TRY(launch_sweep_reduce(j->sched.p, j->ustat.p, 0, j->acc.p, j->red.p, stream));
     HIP_TRY(hipMemcpyAsync(r->hacc, j->acc.p, sizeof(SweepAccum), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipEventRecord(r->ev, stream));
     r->waiting = true;
@@ -446,7 +447,10 @@ void mbir_server::run_loop()
           n_wg += wg;
           launched = true;
           for (Req* r : last) {
-            if (hipMemcpyAsync(r->hacc, r->job->acc.p, sizeof(SweepAccum), hipMemcpyDeviceToHost,
+            mbir_job* lj = r->job;
+            if (launch_sweep_reduce(lj->sched.p, lj->ustat.p, (int)lj->sched_h.size(), lj->acc.p,
+                                    lj->red.p, stream) != hipSuccess ||
+                hipMemcpyAsync(r->hacc, lj->acc.p, sizeof(SweepAccum), hipMemcpyDeviceToHost,
                                stream) != hipSuccess ||
                 hipEventRecord(r->ev, stream) != hipSuccess)
               return false;

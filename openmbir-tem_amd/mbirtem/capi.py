@@ -56,7 +56,7 @@ class Problem(C.Structure):
         ("schedule", C.c_int32), ("rows_per_step", C.c_int32), ("kernel_mode", C.c_int32),
         ("bf_mode", C.c_int32), ("bragg_T", C.c_double), ("bragg_delta", C.c_double),
         ("bf_offset", C.c_double), ("bf_nhicd", C.c_int32), ("pad_", C.c_int32),
-        ("bf_sinogram", _F), ("estimate_prior", C.c_int32), ("pad2_", C.c_int32),
+        ("bf_sinogram", _F), ("estimate_prior", C.c_int32), ("y_slabs", C.c_int32),
     ]
 
 
@@ -307,7 +307,7 @@ def make_problem(geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3
                  seed=12345, schedule=0, rows_per_step=1, kernel_mode=0, init_volume=None,
                  default_recon=0.0, init_gain=None, init_offset=None, init_variance=None,
                  bf_mode=0, bragg_T=3.0, bragg_delta=0.5, bf_offset=0.0, bf_nhicd=0,
-                 bf_sinogram=None, estimate_prior=0):
+                 bf_sinogram=None, estimate_prior=0, y_slabs=0):
     """mbir_problem for the arrays given; returns (Problem, arrays to keep alive)."""
     ang = np.ascontiguousarray(angles_deg, np.float64)
     cnt = np.ascontiguousarray(counts, np.float32)
@@ -340,6 +340,7 @@ def make_problem(geom, angles_deg, counts, *, sigma_x=0.1, mrf_q=1.2, mrf_c=1e-3
     p.bf_mode, p.bragg_T, p.bragg_delta, p.bf_offset = bf_mode, bragg_T, bragg_delta, bf_offset
     p.bf_nhicd = bf_nhicd
     p.estimate_prior = estimate_prior
+    p.y_slabs = y_slabs
     if bf_sinogram is not None:
         b = np.ascontiguousarray(bf_sinogram, np.float32)
         assert b.size == cnt.size

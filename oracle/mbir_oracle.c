@@ -820,6 +820,29 @@ void orc_engine_update_units(orc_engine* e, const int32_t* units, int32_t n, int
   }
 }
 
+/* One thread's share of a sweep: UpdateYSlice::execute over the y-slab [y0, y1) only
+ * (:149-231), the line order drawn as orc_engine_sweep's.  Does not count as a sweep (no
+ * stop test); for timing a bounded part of a sweep. */
+void orc_engine_sweep_slab(orc_engine* e, int32_t outer, int32_t inner, int32_t y0, int32_t y1,
+                           double* stats)
+{
+  slab_args a;
+  memset(&a, 0, sizeof(a));
+  a.e = e;
+  a.y0 = y0 < 0 ? 0 : y0;
+  a.y1 = y1 > e->g.N_y ? e->g.N_y : y1;
+  a.outer = outer;
+  a.inner = inner;
+  a.seed = e->p.seed + 7919u * (uint32_t)e->sweep_counter;
+  run_slab(&a);
+  if (stats) {
+    stats[0] = a.avg_update;
+    stats[1] = a.avg_mag;
+    stats[2] = a.zeros;
+    stats[3] = a.visits;
+  }
+}
+
 /* y split round-robin into per-thread counts, one slab per thread, :670-723 */
 int32_t orc_engine_sweep_threads(orc_engine* e, int32_t outer, int32_t inner, int32_t nthreads,
                                  double* stats)

@@ -113,6 +113,9 @@ int32_t orc_engine_sweep(orc_engine* e, int32_t outer, int32_t inner, double* st
 /* the same sweep with the reference's y-slab thread decomposition */
 int32_t orc_engine_sweep_threads(orc_engine* e, int32_t outer, int32_t inner, int32_t nthreads,
                                  double* stats);
+/* the y-slab [y0, y1) of a sweep, one thread (a bounded part of a sweep, for timing) */
+void orc_engine_sweep_slab(orc_engine* e, int32_t outer, int32_t inner, int32_t y0, int32_t y1,
+                           double* stats);
 /* voxel updates in a given order of units (line, y0, ny), even local slices before odd */
 void orc_engine_update_units(orc_engine* e, const int32_t* units, int32_t n, int32_t outer,
                              int32_t inner, double* stats);

@@ -81,6 +81,7 @@ def load():
         "orc_engine_destroy": (None, [C.c_void_p]),
         "orc_engine_sweep": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _D]),
         "orc_engine_sweep_threads": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _D]),
+        "orc_engine_sweep_slab": (None, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _D]),
         "orc_engine_joint_estimation": (C.c_int32, [C.c_void_p]),
         "orc_engine_update_units": (None, [C.c_void_p, _I, C.c_int32, C.c_int32, C.c_int32, _D]),
         "orc_engine_update_weights": (None, [C.c_void_p]),
@@ -201,6 +202,12 @@ class Engine:
         u = np.ascontiguousarray(units, np.int32).reshape(-1, 3)
         st = np.zeros(4)
         lib().orc_engine_update_units(self.h, u.ctypes.data_as(_I), len(u), outer, inner, dp(st))
+        return st
+
+    def sweep_slab(self, outer, inner, y0, y1):
+        """The y-slab [y0, y1) of a sweep on one thread (not counted as a sweep)."""
+        st = np.zeros(4)
+        lib().orc_engine_sweep_slab(self.h, outer, inner, y0, y1, dp(st))
         return st
 
     def joint_estimation(self):

@@ -155,9 +155,18 @@ def test_server_rejects_and_cancels(gpu_ctx):
 
 
 def test_server_unsupported_shape_fails_cleanly(gpu_ctx):
+    """Units of more than 128 slices are beyond the server's line kernels: a job of 160
+    slices fails with -3 as whole lines, and runs (equal to alone) when cut into y slabs
+    (the automatic choice for more than 64 slices)."""
     g, ang, vol, y = small_problem(n_r=16, n_t=160, n_views=11, thickness=4, seed=2)
     srv = capi.Server(gpu_ctx, max_jobs=2)
-    t = srv.submit(_geom(g, ang), ang, y, num_inner=2)
+    t = srv.submit(_geom(g, ang), ang, y, num_inner=2, y_slabs=1)
     rep = srv.wait(t, timeout=60.0)
-    assert rep.state == capi.JOB_FAILED and rep.error == -3  # N_y > 128
+    assert rep.state == capi.JOB_FAILED and rep.error == -3
+    kw = dict(num_inner=3, stop_threshold=0.0)
+    t = srv.submit(_geom(g, ang), ang, y, **kw)
+    rep = srv.wait(t, timeout=60.0)
+    ref = _alone(gpu_ctx, (g, ang, y, kw))
+    _check_same(srv.job(t), ref, rep)
+    srv.release(t)
     srv.close()

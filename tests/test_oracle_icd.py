@@ -302,3 +302,19 @@ def test_unit_order_replay_equals_the_sweep():
     for l in lines:
         touched[l // g.N_x, l % g.N_x, y0:y0 + ny] = True
     assert np.all(x[~touched] == 0.2) and np.any(x[touched] != 0.2)
+
+
+def test_sweep_slab_is_a_slab_of_the_sweep():
+    """sweep_slab (the bounded CPU-baseline sample) runs a sweep's own line order over a
+    y-range: over all slices it equals the sweep bit for bit; over [0, 4) it updates
+    exactly those slices."""
+    g, ang, vol, y = small_problem()
+    a = O.Engine(g, ang, y, O.params(seed=5), default_recon=0.2)
+    b = O.Engine(g, ang, y, O.params(seed=5), default_recon=0.2)
+    a.sweep(0, 0)
+    b.sweep_slab(0, 0, 0, g.N_y)
+    np.testing.assert_array_equal(a.x(), b.x())
+    c = O.Engine(g, ang, y, O.params(seed=5), default_recon=0.2)
+    st = c.sweep_slab(0, 0, 0, 4)
+    x = c.x()
+    assert st[3] > 0 and np.all(x[:, :, 4:] == 0.2) and np.any(x[:, :, :4] != 0.2)
