@@ -150,6 +150,8 @@ def parse():
     ap.add_argument("--no-bf", action="store_true", help="skip the bright-field figure")
     ap.add_argument("--bf-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the float32-vs-FP64 comparison at the full workload size")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 --pmc child runs that measure HBM traffic")
@@ -168,6 +170,54 @@ def _host_info():
     except Exception:
         pass
     return os.cpu_count() or 1, model
+
+
+def fp32_vs_fp64(ctx, wl, w):
+    """The float32 engine against the FP64 oracle from identical states at the full
+    workload size: setup (error sinogram = the forward projection of the phantom, cost),
+    one gain/offset + variance step, and one ICD sweep replayed by the oracle in the
+    GPU's own update order (the exported schedule)."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import numpy as np
+
+    import oracle_py as O
+    from mbirtem import capi
+
+    ang, y = w["angles"], np.asarray(w["sinogram"], np.float64)
+    x0 = np.asarray(w["volume"], np.float32)
+    kw = dict(w["job_kw"])
+    job = capi.Job(ctx, w["geom"], ang, w["sinogram"], init_volume=x0, **kw)
+    g = O.geometry(wl.n_r, wl.n_t, ang, wl.thickness)
+    prm = O.params(sigma_x=kw["sigma_x"], mrf_q=kw["mrf_q"], mrf_c=kw["mrf_c"],
+                   joint_estimation=wl.joint_estimation, noise_model=wl.noise_model)
+    t0 = time.time()
+    eng = O.Engine(g, ang, y, prm, x0=x0.astype(np.float64))
+    err = {}
+    er = eng.e()
+    err["error_sinogram_at_setup_rel"] = float(np.max(np.abs(job.error_sinogram() - er)) /
+                                               np.max(np.abs(y)))
+    err["cost_at_setup_rel"] = abs(job.cost() - eng.cost()) / abs(eng.cost())
+    eng.joint_estimation()
+    eng.update_weights()
+    job.nuisance_update()
+    I0r, mur, alr = eng.nuisance()
+    I0g, mug, alg = job.nuisance()
+    err["gain_step_rel"] = float(np.max(np.abs(I0g - I0r) / np.abs(I0r)))
+    err["offset_step_abs"] = float(np.max(np.abs(mug - mur)))
+    err["variance_step_rel"] = float(np.max(np.abs(alg - alr) / alr))
+    xs = job.volume().astype(np.float64)
+    job.sweep(0, 0)
+    steps = job.schedule()
+    eng.update_units(np.concatenate(steps), 0, 0)
+    xr, xg = eng.x(), job.volume().astype(np.float64)
+    d = np.abs(xg - xr)
+    err["sweep_volume_max_rel"] = float(d.max() / np.abs(xr).max())
+    err["sweep_volume_mean_rel"] = float(d.mean() / np.abs(xr).max())
+    err["cost_after_sweep_rel"] = abs(job.cost() - eng.cost()) / abs(eng.cost())
+    job.close()
+    log(f"fp32 vs fp64 ({time.time() - t0:.1f}s): {err}")
+    del xs
+    return err
 
 
 def cpu_baseline(wl, ang, y, threads, ctx=None):
@@ -672,6 +722,10 @@ def main():
     if args.slabs > 0:
         out["batched_slabs"], out["served"] = slab_figures(ctx, args.slabs, args.arrival_rate,
                                                            args.big_frac, args.slab_only_batched)
+    if not args.no_fp64:
+        out["fp32_vs_fp64"] = fp32_vs_fp64(ctx, wl, w)
+        out["fp32_vs_fp64_sample"] = (f"{wl.name} from identical states (the phantom volume "
+                                      "and its nuisance parameters), one sweep in the GPU's order")
     if not args.no_bf:
         out["bright_field"] = bf_figures(ctx, args.bf_steps)
     if not args.no_cpu:

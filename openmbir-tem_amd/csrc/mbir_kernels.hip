@@ -1672,6 +1672,11 @@ static void launch_step_ypl(const DevJob& J, const LineRef* lines, int nlines, i
     launch_step_nv<4, CAP>(J, lines, nlines, pos0, flags, s, inl);
 }
 
+// kernel arguments of the step kernel: the job descriptor, the entries and a few scalars
+// must stay within the 4 KB argument limit
+static_assert(sizeof(DevJob) + sizeof(StepLines<kInlineLines>) + 64 <= 4096,
+              "step kernel arguments exceed 4 KB");
+
 template <int CAP>
 static void launch_inline(const DevJob& J, const LineRef* lines, const LineRef* host_lines,
                           int nlines, int pos0, int flags, hipStream_t s)
