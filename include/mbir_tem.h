@@ -282,6 +282,16 @@ int32_t mbir_job_run(mbir_job* job, mbir_run_stats* stats, double* costs, int32_
  * [N_z][N_x][N_y]; NULL = the job's current volume), written as FP64 to out. */
 int32_t mbir_job_forward_project(mbir_job* job, const float* volume, double* out);
 
+/* mbir_job_forward_project without allocation.  No counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  The same kernels, with the FP64 projection accumulator and the uploaded
+ * volume in a library-wide device array of 64 MiB per GPU (held until the process
+ * ends): at most 4 Mi sinogram entries (N_theta N_r N_t) and, when volume is not NULL,
+ * 8 Mi voxels; larger jobs get -1.  Calls no hipMalloc or hipFree.  Every call on one GPU
+ * uses the same array, so calls that overlap on one GPU (from several host threads)
+ * race; the accumulator is cleared by a kernel at each call. */
+int32_t mbir_job_forward_project_noalloc(mbir_job* job, const float* volume, double* out);
+
 /* A-matrix column (z, x) as computed on the device
  * (HAADF/HAADF_ReconstructionEngine.cpp:1369-1524).  Returns the entry count
  * (entries beyond cap are not written) or a negative error. */

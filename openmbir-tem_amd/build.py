@@ -31,6 +31,8 @@ SOURCES = [
     (os.path.join(CSRC, "mbir_kernels.hip"), ["-x", "hip"] + DEVICE),
     (os.path.join(CSRC, "mbir_capi.cpp"), ["-x", "hip"] + DEVICE),
     (os.path.join(CSRC, "mbir_serve.cpp"), ["-x", "hip"] + DEVICE),
+    # the forward projection without allocation (library-wide device scratch)
+    (os.path.join(CSRC, "mbir_noalloc.hip"), ["-x", "hip"] + DEVICE),
     # host adapter with the reference engine's C++ interface (no device code)
     (os.path.join(ENGINE, "HAADF_ReconstructionEngine.cpp"), ["-x", "c++"]),
     # the engine's files (MRC tilt series in, MRC volume and binary outputs out)

@@ -190,6 +190,7 @@ _SIG = {
     "mbir_job_set_bragg_threshold": (C.c_int32, [C.c_void_p, C.c_double]),
     "mbir_job_estimate_prior": (C.c_int32, [C.c_void_p, _D, _D]),
     "mbir_job_get_sigma_x": (C.c_int32, [C.c_void_p, _D]),
+    "mbir_job_forward_project_noalloc": (C.c_int32, [C.c_void_p, _F, _D]),
     "mbir_job_get_schedule": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32,
                                           C.POINTER(C.c_int32), C.c_int64]),
     "mbir_job_set_nhicd_log": (C.c_int32, [C.c_void_p, C.c_int32]),
