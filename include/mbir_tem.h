@@ -111,7 +111,9 @@ typedef struct mbir_problem {
    * noise_model is NOISE_ESTIMATION (one variance for all views); the first outer
    * iteration's inner sweeps run with the threshold 5e10 when num_inner > 1.  The
    * reference's BF prior uses mrf_c = 0.01 (BrightField/BF_QGGMRFPriorModel.cpp:54).
-   * Needs schedule 0, N_y <= 256, one detector row per slice; single-job sweeps only. */
+   * Needs schedule 0 and N_y <= 256; up to four detector rows per slice (the coarse levels
+   * of the multi-resolution driver) in single-job sweeps; the batched sweep and the server
+   * take bright-field jobs of one row per slice. */
   int32_t bf_mode;
   double bragg_T;     /* Bragg threshold T (BFReconstructionArgsParser.cpp:176, default 3) */
   double bragg_delta; /* delta of the Bragg rejector (:179, default 0.5) */
@@ -228,7 +230,8 @@ int32_t mbir_set_tuning(mbir_ctx* ctx, const mbir_tuning* t);
  * the reference runs one job per process).  Launch k runs conflict-free step k of
  * every job that has one, so the launches carry many jobs' lines at once; jobs stay
  * independent (a job's result is the same alone and in a batch).  Jobs need
- * N_y <= 128 and at most 2 detector rows per slice.  outer/inner (may be NULL = 0)
+ * units (y-slabs of a line) of at most 128 slices and at most 2 detector rows per slice.
+ * outer/inner (may be NULL = 0)
  * are per-job iteration indices, stats (may be NULL) per-job results. */
 int32_t mbir_batch_sweep(mbir_ctx* ctx, mbir_job* const* jobs, int32_t n, const int32_t* outer,
                          const int32_t* inner, mbir_sweep_stats* stats);
@@ -336,7 +339,8 @@ int32_t mbir_job_set_timing(mbir_job* job, int32_t on);
  * each launch carries the next conflict-free ICD step of every resident job, so
  * a job joins at the next launch after its setup and leaves when it converges.
  * A job's result is the same as mbir_job_run's for the same problem.  Jobs need
- * N_y <= 128 and at most 2 detector rows per slice (else they fail with -3). */
+ * units of at most 128 slices and at most 2 detector rows per slice (else they fail
+ * with -3). */
 typedef struct mbir_server mbir_server;
 
 typedef struct mbir_server_opts {

@@ -384,8 +384,6 @@ int setup_job(mbir_job* j)
   // :328-385 with BF_MAX = 1)
   std::vector<float> bf_y, bf_w;
   if (P.bf_mode) {
-    for (int c : vn)
-      if (c > 2) return MBIR_ERR_UNSUPPORTED;  // at most two detector rows per slice
     if (P.schedule != 0) return MBIR_ERR_UNSUPPORTED;
     bf_y.resize(M);
     bf_w.resize(M);

@@ -1638,8 +1638,11 @@ static void launch_step_nv(const DevJob& J, const LineRef* lines, int nlines, in
   if (J.wfull && J.vlr_max <= 1)  // bright field: per-entry weights and selector
     hipLaunchKernelGGL((k_icd_step<YPL, 1, true, false, CAP>), grid, block, dyn, s, J, lines,
                        nlines, pos0, flags, inl);
-  else if (J.wfull)  // bright field, two detector rows per slice (coarse voxels)
+  else if (J.wfull && J.vlr_max <= 2)  // bright field, two detector rows per slice
     hipLaunchKernelGGL((k_icd_step<YPL, 2, true, false, CAP>), grid, block, dyn, s, J, lines,
+                       nlines, pos0, flags, inl);
+  else if (J.wfull)  // bright field, three or four rows per slice (the coarsest levels)
+    hipLaunchKernelGGL((k_icd_step<YPL, 4, true, false, CAP>), grid, block, dyn, s, J, lines,
                        nlines, pos0, flags, inl);
   else if (J.bfs && J.vlr_max <= 1)  // HAADF with a bright-field sinogram
     hipLaunchKernelGGL((k_icd_step<YPL, 1, false, true, CAP>), grid, block, dyn, s, J, lines,

@@ -256,6 +256,36 @@ def test_bf_jobs_in_a_batched_sweep_equal_alone(gpu_ctx):
         j.close()
 
 
+def test_bf_four_rows_per_slice_matches_oracle(gpu_ctx):
+    """Bright field at four detector rows per slice (level 0 of three resolution levels,
+    the reference's default for its bright-field reconstruction): setup against the
+    oracle (e, weights, cost), then the cost never rises over sweeps and e stays
+    y - A x - mu for the GPU's own volume."""
+    _, ang, _, counts = bf_problem(n_r=64, n_t=16, bragg_frac=0.05)
+    n_r, n_t = counts.shape[1], counts.shape[2]
+    g = O.geometry(n_r, n_t, ang, thickness=16, delta_xz=4.0, interpolate_factor=4.0)
+    offs = np.full(len(ang), -np.log(2000.0))
+    prm = dict(BF, zero_skipping=0)
+    eng = O.Engine(g, ang, counts, O.params(**prm), offsets=offs)
+    geom = capi.geometry_init(n_r, n_t, ang, g.LengthZ / 2, delta_xz_in=4.0,
+                              interpolate_factor=4.0)
+    assert (geom.N_x, geom.N_y, geom.N_z) == (g.N_x, g.N_y, g.N_z)
+    assert geom.N_y * 4 == n_t
+    job = capi.Job(gpu_ctx, geom, ang, counts.astype(np.float32), init_offset=offs, **prm)
+    ref_e = eng.e()
+    assert np.max(np.abs(job.error_sinogram() - ref_e)) <= 1e-5 * np.max(np.abs(ref_e))
+    np.testing.assert_allclose(job.weights(), eng.w(), rtol=1e-6)
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+    costs = [job.cost()]
+    for it in range(12):
+        job.sweep(0, it)
+        costs.append(job.cost())
+    assert all(b <= a * (1 + 1e-6) + 1e-9 for a, b in zip(costs, costs[1:])), costs
+    y = -np.log(counts)
+    e_ref = y - eng.project(job.volume().astype(np.float64)) - offs[:, None, None]
+    assert np.max(np.abs(job.error_sinogram() - e_ref)) <= 2e-5 * np.max(np.abs(e_ref))
+
+
 def test_bf_two_rows_per_slice_matches_oracle(gpu_ctx):
     """Voxels twice the detector pitch (a coarse level of the multi-resolution
     driver): every slice spans two detector rows.  Setup against the oracle, then

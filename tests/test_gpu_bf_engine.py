@@ -97,3 +97,26 @@ def test_bf_multiresolution_driver(gpu_ctx, tmp_path):
         assert os.path.exists(os.path.join(tmp, d, "FinalOffsetParameters.bin"))
     h, data = parse_mrc(out)
     assert np.isfinite(data).all() and data.max() > 0
+
+
+def test_bf_three_levels(gpu_ctx, tmp_path):
+    """The reference's default of three resolution levels (BFReconstructionArgsParser.cpp:
+    171): voxels of 4, 2 and 1 detector pitches, level 0 with four detector rows per
+    slice; each level hands its volume and offsets to the next through the files."""
+    from test_io import parse_mrc
+    g, ang, _, counts = bf_problem(n_r=64, n_t=16, bragg_frac=0.05)
+    mrc = _series(tmp_path, counts, ang)
+    tmp, out = str(tmp_path / "t"), str(tmp_path / "bf.mrc")
+    os.makedirs(tmp)
+    r, lv = _cli(mrc, 3, 16.0, tmp, out, 2, 4, 1e-3, 5, 0.0, False)
+    assert r.returncode == 0 and "error 0" in lv, r.stdout + r.stderr + lv
+    lines = [l.split() for l in lv.splitlines() if l.startswith("level")]
+    assert [float(l[3]) for l in lines] == [4.0, 2.0, 1.0]
+    dims = [tuple(int(v) for v in l[15:18]) for l in lines]
+    assert all(2 * a == b for a, b in zip(dims[0], dims[1]))
+    assert all(2 * a == b for a, b in zip(dims[1], dims[2]))
+    for d in ("4x", "2x", "1x"):
+        assert os.path.exists(os.path.join(tmp, d, "BraggSelector.mrc"))
+    h, data = parse_mrc(out)
+    assert np.isfinite(data).all() and data.max() > 0
+

@@ -162,6 +162,59 @@ def test_multiresolution_driver(gpu_ctx, tmp_path):
     assert rel < 0.35, rel
 
 
+def test_multiresolution_three_levels(gpu_ctx, tmp_path):
+    """The reference's default of three levels (HAADFReconstructionArgsParser.cpp:165):
+    voxels of 4, 2 and 1 detector pitches.  Level 0 (four detector rows per slice) is
+    exactly the engine's result for that level's inputs, and the final volume
+    reconstructs the phantom."""
+    from test_io import make_tilt_series
+    rng = np.random.default_rng(10)
+    nv = 31
+    g, ang, vol, y = small_problem(n_r=64, n_t=16, n_views=nv, thickness=16,
+                                   gains=rng.uniform(0.9, 1.1, nv),
+                                   offsets=rng.uniform(-0.03, 0.03, nv))
+    ang32 = ang.astype(np.float32)
+    y32 = y.astype(np.float32)
+    mrc = str(tmp_path / "tilts.mrc")
+    make_tilt_series(mrc, y32.transpose(0, 2, 1), mode=2, tilts=ang32, pixel=1e-9)
+    d = float(np.float32(1e-9)) * 1e9
+    tmp = str(tmp_path / "work")
+    out = str(tmp_path / "final.mrc")
+    r = subprocess.run([CLI, "--multires", mrc, "3", "16", "0.1", "0.2", "3", "4", "0.001", "5",
+                        tmp, out], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = open(os.path.join(tmp, "levels.txt")).read().split("\n")
+    assert lines[0] == "error 0"
+    lev = []
+    for l in lines[1:4]:
+        f = l.split()
+        lev.append(dict(delta=float(f[3]), sigma=float(f[5]), q=float(f[7]), stop=float(f[9]),
+                        inner=int(f[11]), dims=tuple(int(v) for v in f[15:18]), dir=f[-1]))
+    assert [v["delta"] for v in lev] == [4.0, 2.0, 1.0]
+    assert [v["inner"] for v in lev] == [4, 1, 1]
+    q = float(np.float32(0.2) + np.float32(1))
+    sig = float(np.float32(0.1))
+    for i, v in enumerate(lev):
+        assert v["sigma"] == pytest.approx(2 ** ((2 - i) * (1 - 3 / q)) * sig, rel=1e-12)
+    th = float(np.float32(16))
+    g0 = capi.geometry_init(g.N_r, g.N_t, ang32.astype(np.float64), th, delta_r=d, delta_t=d,
+                            delta_xz_in=4.0, interpolate_factor=4.0)
+    j0 = capi.Job(gpu_ctx, g0, ang32.astype(np.float64), y32, sigma_x=lev[0]["sigma"],
+                  mrf_q=q, stop_threshold=lev[0]["stop"], num_inner=4, num_outer=3,
+                  joint_estimation=1, noise_model=1, seed=5)
+    j0.run()
+    x0 = j0.volume()
+    nz0, nx0, ny0 = x0.shape
+    assert (nz0, nx0, ny0) == lev[0]["dims"] and ny0 * 4 == g.N_t
+    raw0 = np.fromfile(os.path.join(lev[0]["dir"], "ReconstructedObject.bin"), "<f8")
+    np.testing.assert_array_equal(raw0.reshape(ny0, nx0, nz0), x0.transpose(2, 1, 0))
+    nz, nx, ny = lev[2]["dims"]
+    x2 = np.fromfile(os.path.join(lev[2]["dir"], "ReconstructedObject.bin"), "<f8")
+    x2 = x2.reshape(ny, nx, nz).transpose(2, 1, 0)
+    rel = np.sqrt(np.mean((x2 - vol) ** 2)) / np.sqrt(np.mean(vol ** 2))
+    assert rel < 0.4, rel
+
+
 def test_upsample_volume_on_device(gpu_ctx):
     """mbir_job_upsample_volume gives the fine job exactly the state that a setup
     from the host-replicated coarse volume gives (volume, with lines of empty columns

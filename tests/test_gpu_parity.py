@@ -273,6 +273,45 @@ def test_prior_reestimation_matches_oracle(gpu_ctx):
     assert np.all(np.diff(c[:6]) <= 1e-6 * np.abs(c[1:6]))
 
 
+def test_four_rows_per_slice_matches_oracle(gpu_ctx):
+    """Voxels four detector pitches wide (level 0 of the reference's default three
+    resolution levels): every slice spans four detector rows (the four-row form of the
+    line update).  Setup against the oracle (e to 1e-5, cost to 1e-6), then sweeps: the
+    cost never rises, e stays y - A x - mu for the GPU's own volume, and after 40 sweeps
+    the volume lies within 3 of the oracle's seed-to-seed scatter of its mean."""
+    g0, ang, vol, y = small_problem(n_r=64, n_t=16, n_views=31, thickness=16, seed=41)
+    g = O.geometry(64, 16, ang, thickness=16, delta_xz=4.0, interpolate_factor=4.0)
+    geom = capi.geometry_init(64, 16, ang, g.LengthZ / 2, delta_xz_in=4.0,
+                              interpolate_factor=4.0)
+    assert (geom.N_x, geom.N_y, geom.N_z) == (g.N_x, g.N_y, g.N_z)
+    assert geom.N_y * 4 == 16
+    job = capi.Job(gpu_ctx, geom, ang, y)
+    eng = O.Engine(g, ang, y, O.params())
+    ref_e = eng.e()
+    assert np.max(np.abs(job.error_sinogram() - ref_e)) <= 1e-5 * np.max(np.abs(ref_e))
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+    costs = [job.cost()]
+    for it in range(40):
+        eng.sweep(0, it)
+        job.sweep(0, it)
+        if it < 8:
+            costs.append(job.cost())
+    assert all(b <= a * (1 + 1e-6) for a, b in zip(costs, costs[1:])), costs
+    e_ref = y.reshape(ref_e.shape) - eng.project(job.volume().astype(np.float64))
+    assert np.max(np.abs(job.error_sinogram() - e_ref)) <= 2e-5 * np.max(np.abs(y))
+    runs = [eng.x()]
+    for seed in (1, 2):
+        e2 = O.Engine(g, ang, y, O.params(seed=seed))
+        for it in range(40):
+            e2.sweep(0, it)
+        runs.append(e2.x())
+    xbar = np.mean(runs, axis=0)
+    rms = lambda a: np.sqrt(np.mean(a ** 2))
+    spread = max(rms(r - xbar) for r in runs)
+    xg = job.volume().astype(np.float64)
+    assert rms(xg - xbar) <= max(3.0 * spread, 1e-2 * rms(xbar)), (rms(xg - xbar), spread)
+
+
 def test_cancel_returns_minus_999(gpu_ctx):
     g, ang, vol, y = small_problem()
     job = _job(gpu_ctx, g, ang, y)
