@@ -440,6 +440,14 @@ int setup_job(mbir_job* j)
 
   DevJob& J = j->J;
   J.N_theta = NT; J.N_r = g.N_r; J.N_t = g.N_t; J.N_x = g.N_x; J.N_y = g.N_y; J.N_z = g.N_z;
+  {
+    // line / N_x by a multiply-high: m = ceil(2^32 / N_x) = (2^32 + r) / N_x is exact
+    // for every line l < N_z N_x when l r < 2^32 (the error l r / (N_x 2^32) then stays
+    // below 1 / N_x)
+    const uint64_t nx = (uint64_t)g.N_x, m = ((1ull << 32) + nx - 1) / nx;
+    const uint64_t r = m * nx - (1ull << 32), lmax = (uint64_t)g.N_z * nx;
+    J.nx_div = (m < (1ull << 32) && lmax * r < (1ull << 32)) ? (uint32_t)m : 0u;
+  }
   J.NB = NB;
   J.nslots = NT * NB;
   J.vlr_max = 1;
@@ -505,11 +513,14 @@ int setup_job(mbir_job* j)
   }
   j->cptr_h = cptr;
   HIP_TRY(j->col_ptr.alloc(L + 1));
-  HIP_TRY(j->col_row.alloc((size_t)tot));
+  // the row table has kColPad zero entries behind it (the line kernels read a batch of
+  // rows past a column's end without a bound per entry)
+  HIP_TRY(j->col_row.alloc((size_t)tot + kColPad));
+  HIP_TRY(launch_fill_u32(reinterpret_cast<unsigned int*>(j->col_row.p) + tot, 0u, kColPad, s));
   HIP_TRY(j->col_a.alloc((size_t)tot));
   HIP_TRY(j->col_ka.alloc((size_t)tot));
   HIP_TRY(j->col_kw.alloc((size_t)tot));
-  j->bytes += tot * 16 + (int64_t)(L + 1) * 4;
+  j->bytes += tot * 16 + (int64_t)kColPad * 4 + (int64_t)(L + 1) * 4;
   HIP_TRY(hipMemcpyAsync(j->col_ptr.p, cptr.data(), (L + 1) * sizeof(int), hipMemcpyHostToDevice, s));
   J.col_ptr = j->col_ptr.p;
   J.col_row = j->col_row.p;

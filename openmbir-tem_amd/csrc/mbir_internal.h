@@ -16,6 +16,9 @@ constexpr int kIcdBlock = 512;        // threads per ICD line workgroup (8 wave6
 constexpr int kIcdWaves = kIcdBlock / 64;
 constexpr int kMaxNy = 256;           // voxels per line handled by one workgroup
 constexpr int kMaxSlots = 2048;       // N_theta * NB per column
+// zero entries behind the column table: a line's first batch reads rows past its own
+// column (at most one column's slots plus a batch) without a bound per entry
+constexpr int kColPad = kMaxSlots + 64;
 constexpr int kMaxNB = 12;            // detector bins per view per column (voxels up to 5.5 delta_r)
 constexpr int kVlrCap = 4;            // detector rows per voxel slice
 
@@ -107,9 +110,11 @@ struct DevJob {
   uint32_t spare_b;      // M * 4: byte offset of the spare detector row (N_t zeros) that
                          // follows e and wfull, the target of padding column entries
   float* magmap;         // [N_z * N_x] per-line sum of |x_new - x_old| of the last update
+                         // (NHICD), nullptr = not kept
   const float* bfs;      // HAADF with a bright-field sinogram: per-element scale (counts +
                          // BF_OFFSET), spare row 0; null without
-                         // (NHICD), nullptr = not kept
+  uint32_t nx_div;       // ceil(2^32 / N_x): line / N_x = umulhi(line, nx_div) for every
+                         // line of the volume (checked on the host), 0 = divide
   const LineRef* sched;  // [N_z * N_x * y slabs] units of the current sweep, step by step
   UnitStat* ustat;       // per schedule position: the unit's statistics of the sweep
 };

@@ -128,14 +128,15 @@ __device__ inline float filter_weight(int dy, int dz, int dx)
 //   rho'(d)/d = (P - (P-Q) t1 / t3) / (t3 sigma^P);   P / (sigma^P c) at d = 0.
 __device__ inline float qggmrf_b(float d, const DevJob& J)
 {
-  if (d == 0.f) return J.b0;
   // |d|^(2-q) through the hardware log2 / exp2 (v_log_f32, v_exp_f32, ~1 ulp) and
   // reciprocals through v_rcp_f32 (1 ulp): the coefficient of a quadratic
   // surrogate, where 1e-6 relative error is far below the fp32 solve's own.
+  // Evaluated for every d and selected (no branch: callers interleave several).
   const float a = fabsf(d) < 1e-30f ? 1e-30f : fabsf(d);
   const float t1 = __builtin_amdgcn_exp2f(J.pq2 * __builtin_amdgcn_logf(a)) * J.inv_sigma_pq;
   const float r3 = __builtin_amdgcn_rcpf(J.mrf_c + t1);
-  return (2.f - J.pq2 * t1 * r3) * r3 * J.inv_sigma_p;
+  const float b = (2.f - J.pq2 * t1 * r3) * r3 * J.inv_sigma_p;
+  return d == 0.f ? J.b0 : b;
 }
 
 // Butterfly reduction inside aligned groups of G (1, 2, 4, 8 or 16) lanes with DPP
@@ -425,11 +426,13 @@ __device__ inline void atomic_add_g(T* p, T v)
 template <int YPL>
 struct IcdSmem {
   static constexpr int NY = 64 * YPL;
-  float nbl[9][NY + 2];  // lines (z+dz, x+dx), y = -1 .. N_y (0 outside the volume)
-  float xp[NY + 2];      // this unit's slices at xp[1 ..], the slices just below and above
-                         // it at the ends; the even phase writes its new values here
+  // lines (z+dz, x+dx) over the unit's slices y = -1 .. ny (0 outside the volume);
+  // nbl[4] is the unit's own line, where the even phase writes its new values
+  float nbl[9][NY + 2];
   float dx[NY];
-  double T1[NY], T2[NY];
+  // per slice: the prior terms of the 24 neighbours in the other eight lines (not
+  // written by the launch, so formed before the solve): sum b x_j, sum b, min, max
+  float ps1[NY], ps2[NY], plo[NY], phi[NY];
   // theta partial sums per wave and slice (FP64)
   double pd1[kIcdWaves][NY], pd2[kIcdWaves][NY];
   double kk[kIcdWaves];  // per wave: sum over the column of I0 a * I0 a w (theta2 factor)
@@ -439,7 +442,6 @@ struct IcdSmem {
   float wsu[kIcdWaves], wsv[kIcdWaves], wmg[kIcdWaves];  // wmg: |x_new - x_old|, all slices
   int wcnt[kIcdWaves];
   unsigned char vst[NY];      // per slice: 1 updated, 2 zero-skipped
-  unsigned char vns[NY];      // per slice: detector rows (VLR size)
 };
 
 // The line's column in dynamic LDS, entries padded to a multiple of four per stream:
@@ -474,8 +476,10 @@ constexpr int icd_waves_per_eu(bool pf)
 
 // PF: issue the first theta batch's loads before the LDS staging (shorter line
 // latency, more registers: the form for launches that do not fill the device).
-// SB: HAADF with a bright-field sinogram, every entry's element scaled by J.bfs
-template <int YPL, int NV, bool FW, bool PF, bool SB = false>
+// SB: HAADF with a bright-field sinogram, every entry's element scaled by J.bfs.
+// KR (with PF, YPL NV <= 4): keep the first batch's values in registers through the
+// solve and write the new error sinogram from them when a stream fits one batch.
+template <int YPL, int NV, bool FW, bool PF, bool SB = false, bool KR = false>
 __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_on, int pos,
                          IcdSmem<YPL>& sm, const IcdCol col)
 {
@@ -486,7 +490,8 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int Ny = J.N_y;
   const int line = __builtin_amdgcn_readfirstlane(ref.line);
-  const int z = line / J.N_x, xi = line % J.N_x;
+  const int z = J.nx_div ? (int)__umulhi((unsigned)line, J.nx_div) : line / J.N_x;
+  const int xi = line - z * J.N_x;
   const bool roi = gp(J.mask)[line] != 0;  // used by the solve
   // the unit: slices y0 .. y0 + ny - 1 of the line (ref.pad = y0 | ny << 16; 0 = the
   // whole line).  Units of one launch lie on disjoint detector rows and are never prior
@@ -504,8 +509,8 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   // entries [p C, p C + cnt) of the column; the entries [nc, P C) are padding that
   // point at the spare detector row behind the error sinogram with zero gains, so
   // whole groups of four entries are loaded and updated without per-entry tests ----
-  int lg = 0;
-  while ((1 << lg) < ny && lg < 6) ++lg;
+  // lg = log2 of the power of two >= ny, at most 6
+  const int lg = ny <= 1 ? 0 : min(6, 32 - __clz(ny - 1));
   const int TY = 1 << lg, RPW = 64 >> lg, sub = lane >> lg, yl = lane & (TY - 1);
   const int P = kIcdWaves * RPW, p = wave * RPW + sub;
   const int C = (((nc + P - 1) >> (3 + 6 - lg)) + 3) & ~3;  // entries per stream, 4 | C
@@ -536,6 +541,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   bool nbin[2][YPL], nbhin[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
+    if (h == 1 && wave != 0) break;  // (uniform) the ninth line is wave 0's
     const int l = wave + 8 * h;
     const int zz = z + l / 3 - 1, xx = xi + l % 3 - 1;
     const bool in = l < 9 && zz >= 0 && zz < J.N_z && xx >= 0 && xx < J.N_x;
@@ -565,15 +571,28 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       kw_[it] = gp(J.col_kw)[ic];
     }
   }
+  // (PF) the rows of the stream's first batch, read straight from the table without a
+  // per-entry bound: entries past the stream's count belong to the next stream, the
+  // next line or the zero pad behind the table (p0 + s0 + u < p0 + P C + U), so every
+  // read is in bounds; they are used only as addresses of loads whose values meet
+  // zero gains, never as store targets
   uint32_t r0[PF ? U : 1];
   if constexpr (PF) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) r0[u] = (uint32_t)gp(J.col_row)[p0 + min(s0 + u, nc - 1)];
+    for (int u = 0; u < U; ++u) r0[u] = (uint32_t)gp(J.col_row)[p0 + s0 + u];
   }
-  const int vn_t = gp(J.vlr_n)[tid < ny ? y0 + tid : 0];
-  float vl_t[NV];  // the slice's voxel-line response values (thread tid < ny)
+  // the solving lane's slices 2 tid (even phase) and 2 tid + 1 (odd phase): detector
+  // rows and voxel-line response values (theta2 = column factor * sum_k vlr^2)
+  int vn_s[2];
+  float vl_s[2][NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) vl_t[k] = gp(J.vlr_val)[(tid < ny ? y0 + tid : 0) * kVlrCap + k];
+  for (int ph = 0; ph < 2; ++ph) {
+    const int ys = 2 * tid + ph;
+    const int yc = ys < ny ? y0 + ys : 0;
+    vn_s[ph] = gp(J.vlr_n)[yc];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) vl_s[ph][k] = gp(J.vlr_val)[yc * kVlrCap + k];
+  }
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int l = wave + 8 * h;
@@ -643,30 +662,57 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       }
     }
   };
-  // padding entries (u >= cnt) read the spare row: in bounds, zero gains
+  // entries u >= cnt read some detector row of the error sinogram (in bounds); their
+  // gains in the LDS column are zero, so they add nothing (and are never stored)
   float ev0[PF ? U : 1][YPL][NV], wf0[PF ? U : 1][YPL][NV];
   if constexpr (PF) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) load_entry(u < cnt ? r0[u] * 4u : J.spare_b, ev0[u], wf0[u]);
+    for (int u = 0; u < U; ++u) load_entry(r0[u] * 4u, ev0[u], wf0[u]);
   }
   if constexpr (!FW && !SB) {
     kk = wave_sum(kk);
     if (lane == 0) sm.kk[wave] = kk;
   }
   __syncthreads();  // A: neighbour lines and column visible
-  float* const smx = sm.xp + 1;  // this unit's slices, smx[-1] and smx[ny] outside it
-  if (tid < ny) {  // read after barrier D
-    smx[tid] = sm.nbl[4][tid + 1];
-    sm.vns[tid] = vn_t;
-  }
-  if (tid == kIcdBlock - 1) {  // (ny < kIcdBlock)
-    smx[-1] = sm.nbl[4][0];
-    smx[ny] = sm.nbl[4][ny + 1];
-  }
-  double vsq_t = 0.0;  // sum_k vlr^2 of slice tid: theta2 / column factor
+  float* const smx = &sm.nbl[4][1];  // this unit's slices, smx[-1] and smx[ny] outside it
+
+  // ---- 3b. the prior terms of each slice's 24 neighbours in the other eight lines
+  // (QGGMRF_Functions.cpp:90-147 with the FILTER weights): those lines are not
+  // updated by this launch, and a slice's own value changes only in its own phase,
+  // so these terms are formed for every slice here, while the error-sinogram loads
+  // are in flight.  GP = 8 / YPL lanes per slice, NQ neighbours per lane, summed by
+  // DPP in the group; the two neighbours in the line itself are added in the solve ----
+  {
+    constexpr int GP = 8 / YPL, NQ = 24 / GP;
+    const int vy = tid / GP, gq = tid % GP;
+    float s1 = 0.f, s2 = 0.f, lo = 0.f, hi = 0.f;  // lo/hi include the zeroed centre
+    if (vy < ny) {
+      const float v = smx[vy];
 #pragma unroll
-  for (int k = 0; k < NV; ++k)
-    if (k < vn_t) vsq_t = __fma_rn((double)vl_t[k], (double)vl_t[k], vsq_t);
+      for (int q = 0; q < NQ; ++q) {
+        const int n = gq + q * GP;  // 0 .. 23: line l8 of the eight, dy
+        const int l8 = n / 3, dy = n % 3 - 1;
+        const int l = l8 < 4 ? l8 : l8 + 1;
+        const int dz = l / 3 - 1, dxl = l % 3 - 1;
+        const bool ex = z + dz >= 0 && z + dz < J.N_z && xi + dxl >= 0 && xi + dxl < J.N_x &&
+                        y0 + vy + dy >= 0 && y0 + vy + dy < Ny;
+        const float t = sm.nbl[l][vy + dy + 1];
+        const float nb = ex ? t : 0.f;
+        lo = fminf(lo, nb);
+        hi = fmaxf(hi, nb);
+        const float b = filter_weight(dy, dz, dxl) * qggmrf_b(v - nb, J);
+        s1 = ex ? s1 + b * nb : s1;
+        s2 = ex ? s2 + b : s2;
+      }
+    }
+    group_reduce(GP, s1, s2, lo, hi);
+    if (vy < ny && gq == 0) {
+      sm.ps1[vy] = s1;
+      sm.ps2[vy] = s2;
+      sm.plo[vy] = lo;
+      sm.phi[vy] = hi;
+    }
+  }
 
   // ---- 4. theta1 / theta2 (:343-369): every slice of the line (a zero-skipped
   // slice's sums are simply unused).  Plain weights: theta1 of slice y is
@@ -756,6 +802,12 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   if constexpr (PF) {
     accumulate(0, ev0, wf0);
     base0 = U;
+    // the first batch's loads have all been consumed (or were padding); saying so
+    // here, where no store is pending yet, keeps the compiler from waiting on them
+    // again before each later store (stores share the load counter on this
+    // architecture, so such a wait would stall on the acknowledgement of earlier
+    // stores): s_waitcnt vmcnt(0)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
   }
   for (int base = base0; base < cnt; base += U) {
     float ev[U][YPL][NV], wf[U][YPL][NV];
@@ -793,95 +845,73 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       if constexpr (FW || SB) sm.pd2[wave][yl + TY * jy] = d2[jy];
     }
   __syncthreads();  // D
-  if (tid < ny) {
-    double T1 = 0.0, T2 = 0.0;
-#pragma unroll
-    for (int w = 0; w < kIcdWaves; ++w) T1 += sm.pd1[w][tid];
-    if constexpr (FW || SB) {
-#pragma unroll
-      for (int w = 0; w < kIcdWaves; ++w) T2 += sm.pd2[w][tid];
-    } else {
-      double K = 0.0;
-#pragma unroll
-      for (int w = 0; w < kIcdWaves; ++w) K += sm.kk[w];
-      T2 = K * vsq_t;
-    }
-    sm.T1[tid] = -T1;  // THETA1 *= -1 (:369)
-    sm.T2[tid] = T2;
-  }
-  __syncthreads();  // E
 
-  // ---- 5. voxel solve on lane groups, even slices then odd slices.  G = 16 / YPL
-  // lanes per voxel (a DPP row at most: every slice of one parity fits the
-  // workgroup), each lane holding NS of the 26 neighbours, the same ones in both
-  // phases: their offsets, weights and whether they lie inside the volume in z and x
-  // are set once, and a phase issues all its LDS reads together ----
-  constexpr int G = 16 / YPL, NS = (26 + G - 1) / G;
-  const int vi = tid / G, gl = tid % G;
-  int ndy[NS];
-  bool nok[NS];
-  float nw[NS];
-  const float* nsrc[NS];  // the unit's first slice of the neighbour line in LDS (this line: smx)
+  // ---- 5. voxel solve (:311-421), even slices then odd slices, one lane per slice
+  // (lane tid: slices 2 tid and 2 tid + 1): theta1 / theta2 from the per-wave
+  // partials, the prior terms from 3b plus the two neighbours in the line itself, the
+  // only ones the even phase changes ----
+  const int nsw = __builtin_amdgcn_readfirstlane((((ny + 1) >> 1) + 63) >> 6);  // solving waves
+  double K = 0.0;  // plain weights: the column factor of theta2
+  if constexpr (!FW && !SB) {
 #pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    const int n = gl + q * G;
-    const int m = n < 13 ? n : n + 1;  // the 27-point index without the centre
-    const int dy = m / 9 - 1, dz = (m / 3) % 3 - 1, dx = m % 3 - 1;
-    const int l = (dz + 1) * 3 + (dx + 1);
-    ndy[q] = dy;
-    nok[q] = n < 26 && z + dz >= 0 && z + dz < J.N_z && xi + dx >= 0 && xi + dx < J.N_x;
-    nw[q] = filter_weight(dy, dz, dx);
-    nsrc[q] = l == 4 ? smx : &sm.nbl[l < 9 ? l : 0][1];
+    for (int w = 0; w < kIcdWaves; ++w) K += sm.kk[w];
   }
-  // this lane's statistics over both phases (group leaders only): |dx| and |x| in
-  // the ROI, |dx| everywhere, and the counts zero-skipped + 2^10 updated + 2^20
-  // detector rows of the updated slices
+  // this lane's statistics over both phases: |dx| and |x| in the ROI, |dx|
+  // everywhere, and the counts zero-skipped + 2^10 updated + 2^20 detector rows of
+  // the updated slices
+  // both phases' theta1 / theta2 first: independent of the solve order, so the odd
+  // phase's sums are off its critical path
+  double T1s[2], T2s[2];
+#pragma unroll
+  for (int ph = 0; ph < 2; ++ph) {
+    const int y = 2 * tid + ph;
+    double T1 = 0.0, T2 = 0.0;
+    if (y < ny) {
+#pragma unroll
+      for (int w = 0; w < kIcdWaves; ++w) T1 += sm.pd1[w][y];
+      if constexpr (FW || SB) {
+#pragma unroll
+        for (int w = 0; w < kIcdWaves; ++w) T2 += sm.pd2[w][y];
+      } else {
+        double vsq = 0.0;  // sum_k vlr^2 of the slice
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+          if (k < vn_s[ph]) vsq = __fma_rn((double)vl_s[ph][k], (double)vl_s[ph][k], vsq);
+        T2 = K * vsq;
+      }
+    }
+    T1s[ph] = -T1;  // THETA1 *= -1 (:369)
+    T2s[ph] = T2;
+  }
   float lsu = 0.f, lsv = 0.f, lmg = 0.f;
   int lcnt = 0;
   for (int phase = 0; phase < 2; ++phase) {
-    const int vpp = (ny + 1 - phase) / 2;  // slices of this parity
-    const int y = 2 * vi + phase;           // within the unit
-    const bool active = vi < vpp;
-    const float v = active ? smx[y] : 0.f;
-    const double T1 = active ? sm.T1[y] : 0.0, T2 = active ? sm.T2[y] : 0.0;
-    float s1 = 0.f, s2 = 0.f, lo = 0.f, hi = 0.f;  // lo/hi include the zeroed centre
-    if (active) {
-      float nbv[NS];
-      bool ex[NS];
+    const int y = 2 * tid + phase;  // within the unit
+    if (y < ny) {
+      const double T1 = T1s[phase], T2 = T2s[phase];
+      const float v = smx[y];
+      float s1 = sm.ps1[y], s2 = sm.ps2[y], lo = sm.plo[y], hi = sm.phi[y];
 #pragma unroll
-      for (int q = 0; q < NS; ++q) {
-        const int yy = y + ndy[q];  // -1 .. ny: inside the unit or just outside it
-        ex[q] = nok[q] && y0 + yy >= 0 && y0 + yy < Ny;
-        const float t = nsrc[q][yy];
-        nbv[q] = ex[q] ? t : 0.f;
+      for (int h = 0; h < 2; ++h) {
+        const int dy = 2 * h - 1;  // the slices y - 1 and y + 1 of this line
+        const bool ex = y0 + y + dy >= 0 && y0 + y + dy < Ny;
+        const float t = smx[y + dy];
+        const float nb = ex ? t : 0.f;
+        lo = fminf(lo, nb);
+        hi = fmaxf(hi, nb);
+        const float b = filter_weight(dy, 0, 0) * qggmrf_b(v - nb, J);
+        s1 = ex ? s1 + b * nb : s1;
+        s2 = ex ? s2 + b : s2;
       }
-#pragma unroll
-      for (int q = 0; q < NS; ++q) {
-        lo = fminf(lo, nbv[q]);
-        hi = fmaxf(hi, nbv[q]);
-        if (ex[q]) {
-          const float b = nw[q] * qggmrf_b(v - nbv[q], J);
-          s1 += b * nbv[q];
-          s2 += b;
-        }
-      }
-    }
-    group_reduce(G, s1, s2, lo, hi);
-    const bool anypos = hi > 0.f;
-    float su = 0.f, sv = 0.f, mg = 0.f;  // this lane's voxel (group leaders only)
-    int vst = 0, nrow = 0;
-    if (active && gl == 0) {
       float xnew = v;
-      vst = 2;
-      if (zs_active && v == 0.f && !anypos) {
+      int vst = 2;
+      if (zs_active && v == 0.f && !(hi > 0.f)) {
         // zero-skipped (:311-338)
       } else {
-        double dlo = lo, dhi = hi;
-        if (T2 != 0.0) {
-          const double c = (double)v - T1 / T2;
-          dlo = fmin(dlo, c);
-          dhi = fmax(dhi, c);
-        }
+        // the surrogate's minimiser and the clip bounds: two independent divisions
+        const double c = (double)v - T1 / (T2 != 0.0 ? T2 : 1.0);
+        const double dlo = T2 != 0.0 ? fmin((double)lo, c) : (double)lo;
+        const double dhi = T2 != 0.0 ? fmax((double)hi, c) : (double)hi;
         const double u = ((double)s1 + T2 * (double)v - T1) / ((double)s2 + T2);
         double nv = (double)v + (double)J.alpha * (u - (double)v);
         nv = nv < dlo ? dlo : (nv > dhi ? dhi : nv);
@@ -889,22 +919,18 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
         xnew = (float)nv;
         gp(J.x)[lbase + y] = xnew;
         vst = 1;
-        nrow = sm.vns[y];
-        mg = fabsf(xnew - v);
+        lmg += fabsf(xnew - v);
         if (roi) {
-          su = fabsf(xnew - v);
-          sv = fabsf(v);
+          lsu += fabsf(xnew - v);
+          lsv += fabsf(v);
         }
       }
+      lcnt += vst == 2 ? 1 : (1 << 10) + ((NV == 1 ? 1 : vn_s[phase]) << 20);
       sm.vst[y] = (unsigned char)vst;
       smx[y] = xnew;
       sm.dx[y] = xnew - v;
     }
-    lsu += su;
-    lsv += sv;
-    lmg += mg;
-    lcnt += (vst == 2 ? 1 : 0) + (vst == 1 ? (1 << 10) + ((NV == 1 ? 1 : nrow) << 20) : 0);
-    if (phase == 1) {
+    if (phase == 1 && wave < nsw) {
       // this wave's share of the sweep statistics (:412-421), summed in a fixed order
       const float wsu = wave_sum(lsu), wsv = wave_sum(lsv);
       const float wmg = J.magmap ? wave_sum(lmg) : 0.f;
@@ -926,14 +952,14 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   // the line's latency counts (PF), after it where the device is full ----
   auto line_stats = [&]() {
     if (wave == 0) {
-      const float su = sum8(lane < kIcdWaves ? sm.wsu[lane] : 0.f);
-      const float sv = sum8(lane < kIcdWaves ? sm.wsv[lane] : 0.f);
-      const int cn = sum8(lane < kIcdWaves ? sm.wcnt[lane] : 0);
+      const float su = sum8(lane < nsw ? sm.wsu[lane] : 0.f);
+      const float sv = sum8(lane < nsw ? sm.wsv[lane] : 0.f);
+      const int cn = sum8(lane < nsw ? sm.wcnt[lane] : 0);
       typedef float f32x4 __attribute__((ext_vector_type(4)));
       if (lane == 0) *(gptr<f32x4>)(J.ustat + pos) = f32x4{su, sv, __int_as_float(cn), 0.f};
       if (J.magmap) {  // the line's update magnitude (BFUpdateYSlice.cpp:171, 303-304); NHICD
                        // lists run whole lines, so the unit is the line
-        const float mgl = sum8(lane < kIcdWaves ? sm.wmg[lane] : 0.f);
+        const float mgl = sum8(lane < nsw ? sm.wmg[lane] : 0.f);
         if (lane == 0) gp(J.magmap)[line] = mgl;
       }
     }
@@ -959,9 +985,51 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   auto enew_s = [&](float e, float ka, int jy, int k, float sc) {
     return __fadd_rn(e, -__fmul_rn(__fmul_rn(ka, vd[jy][k]), sc));
   };
+  // PF with one batch per stream: the stream's error-sinogram values (and the
+  // bright-field weights or the scales) are still in registers from the theta pass,
+  // and no other line of the launch touches these bins, so the new values go out as
+  // plain stores with no second round trip
+  bool from_regs = false;
+  if constexpr (PF && KR && YPL * NV <= 4) {  // (wider shapes would spill)
+    if (C <= U) {  // uniform
+      from_regs = true;
+#pragma unroll
+      for (int q = 0; q < U / 4; ++q) {
+        if (4 * q < cnt) {
+          const float4 a4 = ka4(4 * q);
+          const float aa[4] = {a4.x, a4.y, a4.z, a4.w};
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int u = 4 * q + t;
+            const uint32_t rb = r0[u] * 4u;
+            auto at = [&](float* a, int jy, int k) { return pto(a, rb + vtb[jy][k]); };
+#pragma unroll
+            for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+              for (int k = 0; k < NV; ++k) {
+                if constexpr (FW) {
+                  if (u < cnt && upd[jy] && vok[jy][k]) {
+                    const float en = enew(ev0[u][jy][k], aa[t], jy, k);
+                    const float aw = fabsf(wf0[u][jy][k]);
+                    *at(J.e, jy, k) = en;
+                    *at(J.wfull, jy, k) = fabsf(en * sqrtf(aw)) < J.bragg ? aw : -aw;
+                  }
+                } else if constexpr (SB) {
+                  if (u < cnt && d[jy] != 0.f && vok[jy][k])
+                    *at(J.e, jy, k) = enew_s(ev0[u][jy][k], aa[t], jy, k, wf0[u][jy][k]);
+                } else {
+                  if (u < cnt && d[jy] != 0.f && vok[jy][k])
+                    *at(J.e, jy, k) = enew(ev0[u][jy][k], aa[t], jy, k);
+                }
+              }
+          }
+        }
+      }
+    }
+  }
   // padding entries (zero gains, spare row) are updated like the others: they leave
   // the spare row at zero
-  for (int base = 0; base < cnt; base += U) {
+  for (int base = 0; !from_regs && base < cnt; base += U) {
     if constexpr (FW) {
       // bright field: every updated slice (even with dx = 0) refreshes the selector
       // of the entries it touches, |e sqrt(w)| < T
@@ -1086,8 +1154,8 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef*
     ref = lines[blockIdx.x];
     if (__builtin_amdgcn_readfirstlane(ref.line) < 0) return;  // padding of a graph's step slot
   }
-  icd_line<YPL, NV, FW, true, SB>(J, ref, flags & 1, (flags >> 1) & 1, pos0 + (int)blockIdx.x, sm,
-                                  icd_col(J.col_slots));
+  icd_line<YPL, NV, FW, true, SB, true>(J, ref, flags & 1, (flags >> 1) & 1, pos0 + (int)blockIdx.x,
+                                        sm, icd_col(J.col_slots));
 }
 
 // One conflict-free step of every job of a batch that has kernel shape
