@@ -993,38 +993,47 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   if constexpr (PF && KR && YPL * NV <= 4) {  // (wider shapes would spill)
     if (C <= U) {  // uniform
       from_regs = true;
+      // per lane and row: the slice changed (updated, bright field) and has that row
+      bool pr[YPL][NV];
 #pragma unroll
-      for (int q = 0; q < U / 4; ++q) {
-        if (4 * q < cnt) {
-          const float4 a4 = ka4(4 * q);
-          const float aa[4] = {a4.x, a4.y, a4.z, a4.w};
+      for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const int u = 4 * q + t;
-            const uint32_t rb = r0[u] * 4u;
-            auto at = [&](float* a, int jy, int k) { return pto(a, rb + vtb[jy][k]); };
+        for (int k = 0; k < NV; ++k) pr[jy][k] = (FW ? upd[jy] : d[jy] != 0.f) && vok[jy][k];
+      // the stores of entries [4 q, 4 q + 4) that lie below cnt (every one in a full group)
+      auto store4 = [&](int q, bool full, int jy, int k) {
+        const float4 a4 = ka4(4 * q);
+        const float aa[4] = {a4.x, a4.y, a4.z, a4.w};
 #pragma unroll
-            for (int jy = 0; jy < YPL; ++jy)
-#pragma unroll
-              for (int k = 0; k < NV; ++k) {
-                if constexpr (FW) {
-                  if (u < cnt && upd[jy] && vok[jy][k]) {
-                    const float en = enew(ev0[u][jy][k], aa[t], jy, k);
-                    const float aw = fabsf(wf0[u][jy][k]);
-                    *at(J.e, jy, k) = en;
-                    *at(J.wfull, jy, k) = fabsf(en * sqrtf(aw)) < J.bragg ? aw : -aw;
-                  }
-                } else if constexpr (SB) {
-                  if (u < cnt && d[jy] != 0.f && vok[jy][k])
-                    *at(J.e, jy, k) = enew_s(ev0[u][jy][k], aa[t], jy, k, wf0[u][jy][k]);
-                } else {
-                  if (u < cnt && d[jy] != 0.f && vok[jy][k])
-                    *at(J.e, jy, k) = enew(ev0[u][jy][k], aa[t], jy, k);
-                }
-              }
+        for (int t = 0; t < 4; ++t) {
+          const int u = 4 * q + t;
+          if (!full && u >= cnt) continue;
+          const uint32_t rb = r0[u] * 4u + vtb[jy][k];
+          if constexpr (FW) {
+            const float en = enew(ev0[u][jy][k], aa[t], jy, k);
+            const float aw = fabsf(wf0[u][jy][k]);
+            *pto(J.e, rb) = en;
+            *pto(J.wfull, rb) = fabsf(en * sqrtf(aw)) < J.bragg ? aw : -aw;
+          } else if constexpr (SB) {
+            *pto(J.e, rb) = enew_s(ev0[u][jy][k], aa[t], jy, k, wf0[u][jy][k]);
+          } else {
+            *pto(J.e, rb) = enew(ev0[u][jy][k], aa[t], jy, k);
           }
         }
-      }
+      };
+#pragma unroll
+      for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          if (pr[jy][k]) {  // one mask for the whole stream
+#pragma unroll
+            for (int q = 0; q < U / 4; ++q) {
+              if (4 * q + 4 <= cnt)
+                store4(q, true, jy, k);
+              else if (4 * q < cnt)
+                store4(q, false, jy, k);
+            }
+          }
+        }
     }
   }
   // padding entries (zero gains, spare row) are updated like the others: they leave
