@@ -434,7 +434,9 @@ struct IcdSmem {
   // written by the launch, so formed before the solve): sum b x_j, sum b, min, max
   float ps1[NY], ps2[NY], plo[NY], phi[NY];
   // theta partial sums per wave and slice (FP64)
-  double pd1[kIcdWaves][NY], pd2[kIcdWaves][NY];
+  // (16-byte aligned: the solve reads the partials of two adjacent slices at once)
+  alignas(16) double pd1[kIcdWaves][NY];
+  alignas(16) double pd2[kIcdWaves][NY];
   double kk[kIcdWaves];  // per wave: sum over the column of I0 a * I0 a w (theta2 factor)
   // per wave: ROI sums of |x_new - x_old| and |x_old| (fp32, at most 256 non-negative
   // terms per line), and the counts zero-skipped + 2^10 updated + 2^20 detector rows
@@ -859,109 +861,140 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   // this lane's statistics over both phases: |dx| and |x| in the ROI, |dx|
   // everywhere, and the counts zero-skipped + 2^10 updated + 2^20 detector rows of
   // the updated slices
-  // both phases' theta1 / theta2 first: independent of the solve order, so the odd
-  // phase's sums are off its critical path
-  double T1s[2], T2s[2];
+  // both phases' theta1 / theta2 first (slices 2 tid and 2 tid + 1, one 16-byte LDS
+  // read per wave partial): independent of the solve order, so the odd slice's sums
+  // are off its critical path
+  double T1s[2] = {0.0, 0.0}, T2s[2] = {0.0, 0.0};
+  if (2 * tid < ny) {
 #pragma unroll
-  for (int ph = 0; ph < 2; ++ph) {
-    const int y = 2 * tid + ph;
-    double T1 = 0.0, T2 = 0.0;
-    if (y < ny) {
+    for (int w = 0; w < kIcdWaves; ++w) {
+      const double2 p1 = *reinterpret_cast<const double2*>(&sm.pd1[w][2 * tid]);
+      T1s[0] += p1.x;
+      T1s[1] += p1.y;
+    }
+    if constexpr (FW || SB) {
 #pragma unroll
-      for (int w = 0; w < kIcdWaves; ++w) T1 += sm.pd1[w][y];
-      if constexpr (FW || SB) {
+      for (int w = 0; w < kIcdWaves; ++w) {
+        const double2 p2 = *reinterpret_cast<const double2*>(&sm.pd2[w][2 * tid]);
+        T2s[0] += p2.x;
+        T2s[1] += p2.y;
+      }
+    } else {
 #pragma unroll
-        for (int w = 0; w < kIcdWaves; ++w) T2 += sm.pd2[w][y];
-      } else {
+      for (int ph = 0; ph < 2; ++ph) {
         double vsq = 0.0;  // sum_k vlr^2 of the slice
 #pragma unroll
         for (int k = 0; k < NV; ++k)
           if (k < vn_s[ph]) vsq = __fma_rn((double)vl_s[ph][k], (double)vl_s[ph][k], vsq);
-        T2 = K * vsq;
+        T2s[ph] = K * vsq;
       }
     }
-    T1s[ph] = -T1;  // THETA1 *= -1 (:369)
-    T2s[ph] = T2;
+    T1s[0] = -T1s[0];  // THETA1 *= -1 (:369)
+    T1s[1] = -T1s[1];
   }
   float lsu = 0.f, lsv = 0.f, lmg = 0.f;
   int lcnt = 0;
-  for (int phase = 0; phase < 2; ++phase) {
-    const int y = 2 * tid + phase;  // within the unit
-    if (y < ny) {
-      const double T1 = T1s[phase], T2 = T2s[phase];
-      const float v = smx[y];
-      float s1 = sm.ps1[y], s2 = sm.ps2[y], lo = sm.plo[y], hi = sm.phi[y];
+  // the update of slice y (parity ph) from its in-line neighbours xl (y - 1) and xr
+  // (y + 1): writes x, the slice's state and dx; returns the new value (the old one
+  // when zero-skipped)
+  auto solve = [&](int y, int ph, float xl, float xr) -> float {
+    const double T1 = T1s[ph], T2 = T2s[ph];
+    const float v = smx[y];
+    float s1 = sm.ps1[y], s2 = sm.ps2[y], lo = sm.plo[y], hi = sm.phi[y];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int dy = 2 * h - 1;  // the slices y - 1 and y + 1 of this line
-        const bool ex = y0 + y + dy >= 0 && y0 + y + dy < Ny;
-        const float t = smx[y + dy];
-        const float nb = ex ? t : 0.f;
-        lo = fminf(lo, nb);
-        hi = fmaxf(hi, nb);
-        const float b = filter_weight(dy, 0, 0) * qggmrf_b(v - nb, J);
-        s1 = ex ? s1 + b * nb : s1;
-        s2 = ex ? s2 + b : s2;
+    for (int h = 0; h < 2; ++h) {
+      const int dy = 2 * h - 1;
+      const bool ex = y0 + y + dy >= 0 && y0 + y + dy < Ny;
+      const float nb = ex ? (h ? xr : xl) : 0.f;
+      lo = fminf(lo, nb);
+      hi = fmaxf(hi, nb);
+      const float b = filter_weight(dy, 0, 0) * qggmrf_b(v - nb, J);
+      s1 = ex ? s1 + b * nb : s1;
+      s2 = ex ? s2 + b : s2;
+    }
+    float xnew = v;
+    int vst = 2;
+    if (zs_active && v == 0.f && !(hi > 0.f)) {
+      // zero-skipped (:311-338)
+    } else {
+      // the surrogate's minimiser and the clip bounds: two independent divisions
+      const double c = (double)v - T1 / (T2 != 0.0 ? T2 : 1.0);
+      const double dlo = T2 != 0.0 ? fmin((double)lo, c) : (double)lo;
+      const double dhi = T2 != 0.0 ? fmax((double)hi, c) : (double)hi;
+      const double u = ((double)s1 + T2 * (double)v - T1) / ((double)s2 + T2);
+      double nv = (double)v + (double)J.alpha * (u - (double)v);
+      nv = nv < dlo ? dlo : (nv > dhi ? dhi : nv);
+      if (nv < 0.0) nv = 0.0;  // POSITIVITY_CONSTRAINT (:394-400)
+      xnew = (float)nv;
+      gp(J.x)[lbase + y] = xnew;
+      vst = 1;
+      lmg += fabsf(xnew - v);
+      if (roi) {
+        lsu += fabsf(xnew - v);
+        lsv += fabsf(v);
       }
-      float xnew = v;
-      int vst = 2;
-      if (zs_active && v == 0.f && !(hi > 0.f)) {
-        // zero-skipped (:311-338)
-      } else {
-        // the surrogate's minimiser and the clip bounds: two independent divisions
-        const double c = (double)v - T1 / (T2 != 0.0 ? T2 : 1.0);
-        const double dlo = T2 != 0.0 ? fmin((double)lo, c) : (double)lo;
-        const double dhi = T2 != 0.0 ? fmax((double)hi, c) : (double)hi;
-        const double u = ((double)s1 + T2 * (double)v - T1) / ((double)s2 + T2);
-        double nv = (double)v + (double)J.alpha * (u - (double)v);
-        nv = nv < dlo ? dlo : (nv > dhi ? dhi : nv);
-        if (nv < 0.0) nv = 0.0;  // POSITIVITY_CONSTRAINT (:394-400)
-        xnew = (float)nv;
-        gp(J.x)[lbase + y] = xnew;
-        vst = 1;
-        lmg += fabsf(xnew - v);
-        if (roi) {
-          lsu += fabsf(xnew - v);
-          lsv += fabsf(v);
+    }
+    lcnt += vst == 2 ? 1 : (1 << 10) + ((NV == 1 ? 1 : vn_s[ph]) << 20);
+    sm.vst[y] = (unsigned char)vst;
+    sm.dx[y] = xnew - v;
+    return xnew;
+  };
+  // this unit's statistics (:412-421): |dx| and |x| in the ROI, the counts, |dx|
+  float su_u = 0.f, sv_u = 0.f, mg_u = 0.f;
+  int cn_u = 0;
+  if constexpr (YPL <= 2) {  // one wave holds every pair of slices (ny <= 128)
+    if (wave == 0) {
+      // even slice 2 lane, then odd slice 2 lane + 1, whose upper neighbour is the next
+      // lane's new even value (or the slice above the unit): no barrier between
+      const int ye = 2 * lane, yo = ye + 1;
+      float xe = 0.f;
+      if (ye < ny) xe = solve(ye, 0, smx[ye - 1], smx[ye + 1]);
+      const float xe_next = __shfl_down(xe, 1);
+      if (yo < ny) solve(yo, 1, xe, yo + 1 < ny ? xe_next : smx[yo + 1]);
+      // sums in a fixed order
+      su_u = wave_sum(lsu);
+      sv_u = wave_sum(lsv);
+      mg_u = J.magmap ? wave_sum(lmg) : 0.f;
+      cn_u = wave_sum(lcnt);
+    }
+    __syncthreads();  // G: x, dx and the slice states visible
+  } else {
+    for (int phase = 0; phase < 2; ++phase) {
+      const int y = 2 * tid + phase;  // within the unit
+      if (y < ny) smx[y] = solve(y, phase, smx[y - 1], smx[y + 1]);
+      if (phase == 1 && wave < nsw) {
+        // this wave's share of the sweep statistics, summed in a fixed order
+        const float wsu = wave_sum(lsu), wsv = wave_sum(lsv);
+        const float wmg = J.magmap ? wave_sum(lmg) : 0.f;
+        const int wcnt = wave_sum(lcnt);
+        if (lane == 0) {
+          sm.wsu[wave] = wsu;
+          sm.wsv[wave] = wsv;
+          sm.wmg[wave] = wmg;
+          sm.wcnt[wave] = wcnt;
         }
       }
-      lcnt += vst == 2 ? 1 : (1 << 10) + ((NV == 1 ? 1 : vn_s[phase]) << 20);
-      sm.vst[y] = (unsigned char)vst;
-      smx[y] = xnew;
-      sm.dx[y] = xnew - v;
+      __syncthreads();  // F, G
     }
-    if (phase == 1 && wave < nsw) {
-      // this wave's share of the sweep statistics (:412-421), summed in a fixed order
-      const float wsu = wave_sum(lsu), wsv = wave_sum(lsv);
-      const float wmg = J.magmap ? wave_sum(lmg) : 0.f;
-      const int wcnt = wave_sum(lcnt);
-      if (lane == 0) {
-        sm.wsu[wave] = wsu;
-        sm.wsv[wave] = wsv;
-        sm.wmg[wave] = wmg;
-        sm.wcnt[wave] = wcnt;
-      }
+    if (wave == 0) {
+      su_u = sum8(lane < nsw ? sm.wsu[lane] : 0.f);
+      sv_u = sum8(lane < nsw ? sm.wsv[lane] : 0.f);
+      mg_u = J.magmap ? sum8(lane < nsw ? sm.wmg[lane] : 0.f) : 0.f;
+      cn_u = sum8(lane < nsw ? sm.wcnt[lane] : 0);
     }
-    __syncthreads();  // F, G
   }
 
-  // ---- 6. sweep statistics: the per-wave shares (visible since the last solve
-  // barrier) summed in a fixed order and stored at the unit's schedule position (one
-  // 16-byte store; the sweep's sums are formed in schedule order after its last step,
-  // so no two units meet at one address); issued before the error-sinogram update where
-  // the line's latency counts (PF), after it where the device is full ----
+  // ---- 6. sweep statistics, stored at the unit's schedule position (one 16-byte
+  // store; the sweep's sums are formed in schedule order after its last step, so no two
+  // units meet at one address); issued before the error-sinogram update where the
+  // line's latency counts (PF), after it where the device is full ----
   auto line_stats = [&]() {
-    if (wave == 0) {
-      const float su = sum8(lane < nsw ? sm.wsu[lane] : 0.f);
-      const float sv = sum8(lane < nsw ? sm.wsv[lane] : 0.f);
-      const int cn = sum8(lane < nsw ? sm.wcnt[lane] : 0);
+    if (wave == 0 && lane == 0) {
       typedef float f32x4 __attribute__((ext_vector_type(4)));
-      if (lane == 0) *(gptr<f32x4>)(J.ustat + pos) = f32x4{su, sv, __int_as_float(cn), 0.f};
-      if (J.magmap) {  // the line's update magnitude (BFUpdateYSlice.cpp:171, 303-304); NHICD
-                       // lists run whole lines, so the unit is the line
-        const float mgl = sum8(lane < nsw ? sm.wmg[lane] : 0.f);
-        if (lane == 0) gp(J.magmap)[line] = mgl;
-      }
+      *(gptr<f32x4>)(J.ustat + pos) = f32x4{su_u, sv_u, __int_as_float(cn_u), 0.f};
+      // the line's update magnitude (BFUpdateYSlice.cpp:171, 303-304); NHICD lists run
+      // whole lines, so the unit is the line
+      if (J.magmap) gp(J.magmap)[line] = mg_u;
     }
   };
   if constexpr (PF) line_stats();
@@ -990,7 +1023,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   // and no other line of the launch touches these bins, so the new values go out as
   // plain stores with no second round trip
   bool from_regs = false;
-  if constexpr (PF && KR && YPL * NV <= 4) {  // (wider shapes would spill)
+  if constexpr (PF && KR && YPL * NV <= (FW || SB ? 2 : 4)) {  // (wider shapes would spill)
     if (C <= U) {  // uniform
       from_regs = true;
       // per lane and row: the slice changed (updated, bright field) and has that row
