@@ -529,6 +529,10 @@ int setup_job(mbir_job* j)
   J.col_kw = j->col_kw.p;
   HIP_TRY(launch_build_columns(J, j->col_row.p, j->col_a.p, s));
   HIP_TRY(launch_column_weights(J, j->col_ka.p, j->col_kw.p, (int)tot, s));
+  HIP_TRY(j->line_kk.alloc(L));
+  j->bytes += (int64_t)L * 8;
+  J.line_kk = j->line_kk.p;
+  HIP_TRY(launch_line_kk(J, j->line_kk.p, s));
   // initial forward projection and error sinogram (:646-691)
   HIP_TRY(j->yest.alloc(M));
   HIP_TRY(launch_fill_f64(j->yest.p, 0.0, M, s));
@@ -668,6 +672,7 @@ int nuisance_update_on(mbir_job* j, hipStream_t s)
   }
   if (j->prob.noise_model) HIP_TRY(launch_noise_update(j->J, j->alpha.p, j->wview.p, s));
   HIP_TRY(launch_column_weights(j->J, j->col_ka.p, j->col_kw.p, (int)j->nnz_total, s));
+  HIP_TRY(launch_line_kk(j->J, j->line_kk.p, s));
   return MBIR_OK;
 }
 

@@ -87,6 +87,7 @@ struct mbir_job {
   DevBuf<int> vlr_t0, vlr_n, line_nnz, colr, col_ptr, col_row;
   DevBuf<LineRef> sched;
   DevBuf<float> col_a, col_ka, col_kw;
+  DevBuf<double> line_kk;  // per line: the column factor of theta2 (plain weights)
   DevBuf<float> wfull;       // bright field: per-entry weights, sign = selector
   DevBuf<float> bfs;         // HAADF with a bright-field sinogram: per-element scale
   DevBuf<double> alpha_b;    // bright field: the next shared variance

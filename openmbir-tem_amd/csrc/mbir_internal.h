@@ -86,6 +86,8 @@ struct DevJob {
   const float* col_a;    // A value a
   const float* col_ka;   // I0[th] * a            (rebuilt when I0 changes)
   const float* col_kw;   // I0[th] * a * w[th]    (rebuilt when I0 or w changes)
+  const double* line_kk; // [N_z * N_x] sum over the column of col_ka col_kw in FP64 (plain
+                         // weights: the column factor of theta2; rebuilt with col_kw)
   float* x;              // [N_z][N_x][N_y]
   float* e;              // [N_theta][N_r][N_t] error sinogram
   const float* y;        // [N_theta][N_r][N_t] measurements
@@ -198,6 +200,7 @@ hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const Batc
                                  int e0, int ne, int nwg, int slots, int pf, hipStream_t s, int fw = 0);
 hipError_t launch_build_columns(const DevJob& J, int* row, float* a, hipStream_t s);
 hipError_t launch_column_weights(const DevJob& J, float* ka, float* kw, int nnz, hipStream_t s);
+hipError_t launch_line_kk(const DevJob& J, double* kk, hipStream_t s);
 hipError_t launch_export_column(const DevJob& J, int z, int x, float* vals, int* rows,
                                 hipStream_t s);
 // joint estimation / weights / cost (FP64)

@@ -351,6 +351,21 @@ __global__ void k_column_weights(DevJob J, float* ka, float* kw, int nnz)
   }
 }
 
+// The column factor of theta2 per line with plain weights: sum over the line's column
+// of (I0 a)(I0 a w), exact products summed in FP64 (lanes stride the entries, then a
+// fixed-order DPP sum); one wave per line.
+__global__ __launch_bounds__(kBlock) void k_line_kk(DevJob J, const int* col_ptr, double* kk, int L)
+{
+  const int lane = threadIdx.x & 63;
+  const int l = (int)(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+  if (l >= L) return;  // (uniform per wave)
+  double a = 0.0;
+  for (int i = col_ptr[l] + lane; i < col_ptr[l + 1]; i += 64)
+    a = __fma_rn((double)J.col_ka[i], (double)J.col_kw[i], a);
+  a = wave_sum(a);
+  if (lane == 63) kk[l] = a;
+}
+
 // ---------------------------------------------------------------------------
 // ICD update of one line (z, x): all N_y voxels of the line.  Restates
 // UpdateYSlice::execute for one line (HAADF/HAADF_ReconstructionEngine_UpdateVoxels.cpp:260-455):
@@ -437,7 +452,6 @@ struct IcdSmem {
   // (16-byte aligned: the solve reads the partials of two adjacent slices at once)
   alignas(16) double pd1[kIcdWaves][NY];
   alignas(16) double pd2[kIcdWaves][NY];
-  double kk[kIcdWaves];  // per wave: sum over the column of I0 a * I0 a w (theta2 factor)
   // per wave: ROI sums of |x_new - x_old| and |x_old| (fp32, at most 256 non-negative
   // terms per line), and the counts zero-skipped + 2^10 updated + 2^20 detector rows
   // of the updated slices
@@ -495,6 +509,9 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   const int z = J.nx_div ? (int)__umulhi((unsigned)line, J.nx_div) : line / J.N_x;
   const int xi = line - z * J.N_x;
   const bool roi = gp(J.mask)[line] != 0;  // used by the solve
+  // plain weights: theta2 of slice y is (sum_k vlr_yk^2) times the line's column factor
+  // sum (I0 a)(I0 a w), formed when the gains or weights change
+  const double K = (!FW && !SB) ? gp(J.line_kk)[line] : 0.0;
   // the unit: slices y0 .. y0 + ny - 1 of the line (ref.pad = y0 | ny << 16; 0 = the
   // whole line).  Units of one launch lie on disjoint detector rows and are never prior
   // neighbours, so the slices just outside a unit are read, not written, by this launch.
@@ -608,9 +625,6 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       if (lane == 1) sm.nbl[l][ny + 1] = nbhin[h] ? nbh[h] : 0.f;
     }
   }
-  // theta2 of slice y is (sum_k vlr_yk^2) * sum over the column of (I0 a)(I0 a w):
-  // the column part once per line, exact products summed in FP64
-  double kk = 0.0;
 #pragma unroll
   for (int it = 0; it < kColIt; ++it) {
     const int i = tid + it * kIcdBlock;
@@ -619,8 +633,6 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       col.rb[i] = in ? rw_[it] * 4u : J.spare_b;
       col.ka[i] = in ? ka_[it] : 0.f;
       col.kw[i] = in ? kw_[it] : 0.f;
-      if constexpr (!FW && !SB)
-        if (in) kk = __fma_rn((double)ka_[it], (double)kw_[it], kk);
     }
   }
   uint32_t vtb[YPL][NV];  // byte offset of the lane's detector row position t
@@ -671,10 +683,6 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
 #pragma unroll
     for (int u = 0; u < U; ++u) load_entry(r0[u] * 4u, ev0[u], wf0[u]);
   }
-  if constexpr (!FW && !SB) {
-    kk = wave_sum(kk);
-    if (lane == 0) sm.kk[wave] = kk;
-  }
   __syncthreads();  // A: neighbour lines and column visible
   float* const smx = &sm.nbl[4][1];  // this unit's slices, smx[-1] and smx[ny] outside it
 
@@ -682,29 +690,33 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   // (QGGMRF_Functions.cpp:90-147 with the FILTER weights): those lines are not
   // updated by this launch, and a slice's own value changes only in its own phase,
   // so these terms are formed for every slice here, while the error-sinogram loads
-  // are in flight.  GP = 8 / YPL lanes per slice, NQ neighbours per lane, summed by
-  // DPP in the group; the two neighbours in the line itself are added in the solve ----
+  // are in flight.  GP = 8 / YPL lanes per slice, each with 8 / GP of the lines and
+  // all three dy, summed by DPP in the group; the two neighbours in the line itself are
+  // added in the solve ----
   {
-    constexpr int GP = 8 / YPL, NQ = 24 / GP;
+    constexpr int GP = 8 / YPL, NL = 8 / GP;  // lanes per slice, lines per lane
     const int vy = tid / GP, gq = tid % GP;
     float s1 = 0.f, s2 = 0.f, lo = 0.f, hi = 0.f;  // lo/hi include the zeroed centre
     if (vy < ny) {
       const float v = smx[vy];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int n = gq + q * GP;  // 0 .. 23: line l8 of the eight, dy
-        const int l8 = n / 3, dy = n % 3 - 1;
+      for (int m = 0; m < NL; ++m) {
+        const int l8 = gq + m * GP;  // one of the eight other lines, all three dy
         const int l = l8 < 4 ? l8 : l8 + 1;
         const int dz = l / 3 - 1, dxl = l % 3 - 1;
-        const bool ex = z + dz >= 0 && z + dz < J.N_z && xi + dxl >= 0 && xi + dxl < J.N_x &&
-                        y0 + vy + dy >= 0 && y0 + vy + dy < Ny;
-        const float t = sm.nbl[l][vy + dy + 1];
-        const float nb = ex ? t : 0.f;
-        lo = fminf(lo, nb);
-        hi = fmaxf(hi, nb);
-        const float b = filter_weight(dy, dz, dxl) * qggmrf_b(v - nb, J);
-        s1 = ex ? s1 + b * nb : s1;
-        s2 = ex ? s2 + b : s2;
+        const bool lin = z + dz >= 0 && z + dz < J.N_z && xi + dxl >= 0 && xi + dxl < J.N_x;
+        const float* src = &sm.nbl[l][vy + 1];
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy) {
+          const bool ex = lin && y0 + vy + dy >= 0 && y0 + vy + dy < Ny;
+          const float t = src[dy];
+          const float nb = ex ? t : 0.f;
+          lo = fminf(lo, nb);
+          hi = fmaxf(hi, nb);
+          const float b = filter_weight(dy, dz, dxl) * qggmrf_b(v - nb, J);
+          s1 = ex ? s1 + b * nb : s1;
+          s2 = ex ? s2 + b : s2;
+        }
       }
     }
     group_reduce(GP, s1, s2, lo, hi);
@@ -853,11 +865,6 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   // partials, the prior terms from 3b plus the two neighbours in the line itself, the
   // only ones the even phase changes ----
   const int nsw = __builtin_amdgcn_readfirstlane((((ny + 1) >> 1) + 63) >> 6);  // solving waves
-  double K = 0.0;  // plain weights: the column factor of theta2
-  if constexpr (!FW && !SB) {
-#pragma unroll
-    for (int w = 0; w < kIcdWaves; ++w) K += sm.kk[w];
-  }
   // this lane's statistics over both phases: |dx| and |x| in the ROI, |dx|
   // everywhere, and the counts zero-skipped + 2^10 updated + 2^20 detector rows of
   // the updated slices
@@ -1722,6 +1729,13 @@ hipError_t launch_column_weights(const DevJob& J, float* ka, float* kw, int nnz,
   if (nnz <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_column_weights, dim3(grid_for((size_t)nnz)), dim3(kBlock), 0, s, J, ka,
                      kw, nnz);
+  return hipGetLastError();
+}
+
+hipError_t launch_line_kk(const DevJob& J, double* kk, hipStream_t s)
+{
+  const int L = J.N_z * J.N_x, per = kBlock / 64;
+  hipLaunchKernelGGL(k_line_kk, dim3((L + per - 1) / per), dim3(kBlock), 0, s, J, J.col_ptr, kk, L);
   return hipGetLastError();
 }
 

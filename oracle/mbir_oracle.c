@@ -1472,6 +1472,9 @@ int32_t orc_engine_run(orc_engine* e, double* costs, int32_t cap, int32_t* n_cos
 
 void orc_engine_get_x(const orc_engine* e, double* x) { memcpy(x, e->x, sizeof(double) * vol_size(&e->g)); }
 void orc_engine_set_x(orc_engine* e, const double* x) { memcpy(e->x, x, sizeof(double) * vol_size(&e->g)); }
+/* The error sinogram as given (a state to continue from, e.g. another engine's);
+ * the volume and e are no longer tied by e = y - I0 A x - mu afterwards */
+void orc_engine_set_e(orc_engine* e, const double* err) { memcpy(e->e, err, sizeof(double) * sino_size(&e->g)); }
 
 /* The HAADF engine's bright-field sinogram (HAADF/HAADF_ReconstructionEngine_Extra.cpp:
  * 89-175): s = counts + BF_OFFSET (HAADF/HAADFConstants.h:49) per element, then the

@@ -139,6 +139,7 @@ int32_t orc_engine_run(orc_engine* e, double* costs, int32_t cap, int32_t* n_cos
 /* getters (copies) */
 void orc_engine_get_x(const orc_engine* e, double* x);
 void orc_engine_set_x(orc_engine* e, const double* x);
+void orc_engine_set_e(orc_engine* e, const double* err);
 /* HAADF with a bright-field sinogram (BF_Flag): projection entries scaled by
  * bf_counts + ORC_BF_OFFSET per element, e re-derived, no gain/offset estimation */
 void orc_engine_set_bf_sinogram(orc_engine* e, const double* bf_counts);

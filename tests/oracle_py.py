@@ -96,6 +96,7 @@ def load():
         "orc_engine_run": (C.c_int32, [C.c_void_p, _D, C.c_int32, _I, _I]),
         "orc_engine_get_x": (None, [C.c_void_p, _D]),
         "orc_engine_set_x": (None, [C.c_void_p, _D]),
+        "orc_engine_set_e": (None, [C.c_void_p, _D]),
         "orc_engine_set_bf_sinogram": (None, [C.c_void_p, _D]),
         "orc_engine_get_e": (None, [C.c_void_p, _D]),
         "orc_engine_get_w": (None, [C.c_void_p, _D]),
@@ -258,6 +259,11 @@ class Engine:
     def set_x(self, x):
         x = np.ascontiguousarray(x, np.float64)
         lib().orc_engine_set_x(self.h, dp(x))
+
+    def set_e(self, err):
+        err = np.ascontiguousarray(err, np.float64)
+        assert err.size == int(np.prod(self.sshape))
+        lib().orc_engine_set_e(self.h, dp(err))
 
     def set_bf_sinogram(self, bf_counts):
         """HAADF with a bright-field sinogram: entries scaled by counts + 32768."""
