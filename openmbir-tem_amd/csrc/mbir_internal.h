@@ -31,17 +31,18 @@ struct LineRef {
   int pad;
 };
 
-// The lines of one step carried in the kernel arguments (single-job launches of at
-// most kInlineLines lines): they arrive with the job descriptor in the launch's first
+// The units of one step carried in the kernel arguments (single-job launches of at
+// most kInlineLines units): they arrive with the job descriptor in the launch's first
 // memory round trip instead of one dependent load later.
-// Each entry is 8 bytes, {line | nc << 20, p0} (whole lines of jobs with fewer than
-// 2^20 lines), and the launch takes the smallest of two capacities that holds the step:
-// every kilobyte of kernel arguments costs about half a microsecond per launch.
+// Each entry is 12 bytes, {line | nc << 20, p0, pad} (jobs with fewer than 2^20 lines,
+// columns of fewer than 4096 entries); the launch takes the smaller of two capacities
+// that holds the step (argument size changes a dependent launch's time by at most
+// 0.3 us up to 3.5 KB on this device).
 constexpr int kInlineSmall = 96;
 constexpr int kInlineLines = 256;
 template <int CAP>
 struct StepLines {
-  int2 l[CAP > 0 ? CAP : 1];
+  int3 l[CAP > 0 ? CAP : 1];
 };
 
 // A unit's share of its sweep's stop-test statistics, stored by the line update at the

@@ -1190,14 +1190,14 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef*
   __shared__ IcdSmem<YPL> sm;
   LineRef ref;
   if constexpr (CAP > 0) {
-    const int2 c = inl.l[blockIdx.x];
-    ref = LineRef{c.x & 0xfffff, c.y, (int)((unsigned)c.x >> 20), 0};
+    const int3 c = inl.l[blockIdx.x];
+    ref = LineRef{c.x & 0xfffff, c.y, (int)((unsigned)c.x >> 20), c.z};
     // the arguments the line's first loads depend on, fetched in the same round trip
     // as its entry (an empty asm that takes them as operands: no instruction)
     asm volatile(";" ::"s"(ref.line), "s"(ref.p0), "s"(ref.nc), "s"(J.N_x), "s"(J.N_y), "s"(J.NB),
                  "s"(J.N_z), "s"(J.mask), "s"(J.vlr_t0), "s"(J.vlr_n), "s"(J.col_row),
                  "s"(J.col_ka), "s"(J.col_kw), "s"(J.x), "s"(J.e), "s"(J.col_slots),
-                 "s"(J.spare_b));
+                 "s"(J.spare_b), "s"(ref.pad), "s"(J.line_kk), "s"(J.nx_div));
   } else {
     if ((int)blockIdx.x >= nlines) return;
     ref = lines[blockIdx.x];
@@ -1205,6 +1205,21 @@ __global__ __launch_bounds__(kIcdBlock) void k_icd_step(DevJob J, const LineRef*
   }
   icd_line<YPL, NV, FW, true, SB, true>(J, ref, flags & 1, (flags >> 1) & 1, pos0 + (int)blockIdx.x,
                                         sm, icd_col(J.col_slots));
+}
+
+// A step with more units than the device has compute units: the prefetching form
+// without the register-held batch, compiled for two resident workgroups per compute
+// unit (at most 128 registers), so the step's workgroups run in one round.
+template <int YPL, int NV, bool FW = false, bool SB = false>
+__global__ __launch_bounds__(kIcdBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void k_icd_step_wide(DevJob J, const LineRef* lines, int nlines, int pos0, int flags)
+{
+  __shared__ IcdSmem<YPL> sm;
+  if ((int)blockIdx.x >= nlines) return;
+  const LineRef ref = lines[blockIdx.x];
+  if (__builtin_amdgcn_readfirstlane(ref.line) < 0) return;  // padding of a graph's step slot
+  icd_line<YPL, NV, FW, true, SB, false>(J, ref, flags & 1, (flags >> 1) & 1,
+                                         pos0 + (int)blockIdx.x, sm, icd_col(J.col_slots));
 }
 
 // One conflict-free step of every job of a batch that has kernel shape
@@ -1799,6 +1814,57 @@ static void launch_step_ypl(const DevJob& J, const LineRef* lines, int nlines, i
     launch_step_nv<4, CAP>(J, lines, nlines, pos0, flags, s, inl);
 }
 
+template <int YPL>
+static void launch_step_wide(const DevJob& J, const LineRef* lines, int nlines, int pos0,
+                             int flags, hipStream_t s)
+{
+  const size_t dyn = icd_col_bytes(J.col_slots);
+  const dim3 grid(nlines), block(kIcdBlock);
+#define MBIR_WIDE(NV, FW, SB)                                                                 \
+  hipLaunchKernelGGL((k_icd_step_wide<YPL, NV, FW, SB>), grid, block, dyn, s, J, lines, nlines, \
+                     pos0, flags)
+  if (J.wfull && J.vlr_max <= 1)
+    MBIR_WIDE(1, true, false);
+  else if (J.wfull && J.vlr_max <= 2)
+    MBIR_WIDE(2, true, false);
+  else if (J.wfull)
+    MBIR_WIDE(4, true, false);
+  else if (J.bfs && J.vlr_max <= 1)
+    MBIR_WIDE(1, false, true);
+  else if (J.bfs)
+    MBIR_WIDE(2, false, true);
+  else if (J.vlr_max <= 1)
+    MBIR_WIDE(1, false, false);
+  else if (J.vlr_max == 2)
+    MBIR_WIDE(2, false, false);
+  else
+    MBIR_WIDE(4, false, false);
+#undef MBIR_WIDE
+}
+
+// compute units of the current device (all devices of a process alike), once
+static int device_cus()
+{
+  static int n = 0;
+  if (n == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      n = v;
+    else
+      n = 256;
+  }
+  return n;
+}
+
+// the wide form within 128 registers (lane rows times detector rows at most 4)
+static bool wide_ok(const DevJob& J)
+{
+  int ypl, nv;
+  icd_shape(J, J.vlr_max, &ypl, &nv);
+  return ypl * nv <= 4;
+}
+
 // kernel arguments of the step kernel: the job descriptor, the entries and a few scalars
 // must stay within the 4 KB argument limit
 static_assert(sizeof(DevJob) + sizeof(StepLines<kInlineLines>) + 64 <= 4096,
@@ -1810,7 +1876,8 @@ static void launch_inline(const DevJob& J, const LineRef* lines, const LineRef* 
 {
   StepLines<CAP> inl;  // entries past nlines are never read
   for (int i = 0; i < nlines; ++i)
-    inl.l[i] = make_int2(host_lines[i].line | (host_lines[i].nc << 20), host_lines[i].p0);
+    inl.l[i] = make_int3(host_lines[i].line | (host_lines[i].nc << 20), host_lines[i].p0,
+                         host_lines[i].pad);
   launch_step_ypl<CAP>(J, lines, nlines, pos0, flags, s, inl);
 }
 
@@ -1819,15 +1886,24 @@ hipError_t launch_icd_step(const DevJob& J, const LineRef* lines, int nlines, in
 {
   if (nlines <= 0) return hipSuccess;
   const int flags = ((outer == 0 && inner == 0) ? 1 : 0) | (zero_skipping ? 2 : 0);
-  // the entries travel in the arguments when they are whole lines of a job of fewer
-  // than 2^20 lines (8 bytes each)
+  // the units travel in the arguments for jobs of fewer than 2^20 lines and columns of
+  // fewer than 4096 entries (12 bytes each)
   bool inl = host_lines && nlines <= kInlineLines && (int64_t)J.N_z * J.N_x < (1 << 20);
-  for (int i = 0; inl && i < nlines; ++i) inl = host_lines[i].pad == 0 && host_lines[i].nc < 4096;
+  for (int i = 0; inl && i < nlines; ++i) inl = host_lines[i].nc < 4096;
   if (inl && nlines <= kInlineSmall)
     launch_inline<kInlineSmall>(J, lines, host_lines, nlines, pos0, flags, s);
-  else if (inl)
+  else if (inl && nlines <= device_cus())
     launch_inline<kInlineLines>(J, lines, host_lines, nlines, pos0, flags, s);
-  else {
+  else if (nlines > device_cus() && wide_ok(J)) {
+    int ypl, nv;
+    icd_shape(J, J.vlr_max, &ypl, &nv);
+    if (ypl == 1)
+      launch_step_wide<1>(J, lines, nlines, pos0, flags, s);
+    else if (ypl == 2)
+      launch_step_wide<2>(J, lines, nlines, pos0, flags, s);
+    else
+      launch_step_wide<4>(J, lines, nlines, pos0, flags, s);
+  } else {
     static const StepLines<0> none{};
     launch_step_ypl<0>(J, lines, nlines, pos0, flags, s, none);
   }

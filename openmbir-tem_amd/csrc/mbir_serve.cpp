@@ -419,11 +419,15 @@ void mbir_server::run_loop()
     // one tick per lane that may run ahead: the next step of every job of the lane
     // that has one, one launch per kernel shape
     bool launched_any = false, held = false;
+    bool lane_has[kLanes] = {false, false};
+    for (Req* r : act)
+      if (!r->waiting && !r->finishing && r->step < r->nsteps) lane_has[r->lane] = true;
     for (int lane = 0; lane < kLanes && !bad; ++lane) {
-      bool has = false;
-      for (Req* r : act)
-        if (r->lane == lane && !r->waiting && !r->finishing && r->step < r->nsteps) has = true;
-      if (!has) continue;
+      if (!lane_has[lane]) continue;
+      // a job alone in a launch takes the plain path's step kernel only while the other
+      // lane is idle: beside a full launch of the other lane, the batched form (fewer
+      // registers) shares the compute units better
+      const bool alone_ok = !lane_has[1 - lane];
       if (!lane_open(lane)) {
         held = true;
         continue;
@@ -437,11 +441,24 @@ void mbir_server::run_loop()
         args.slots = 4;
         int wg = 0;
         std::vector<Req*> last;  // jobs whose sweep ends with this launch
+        Req* first = nullptr;    // the launch's first job
         auto flush = [&]() -> bool {
           if (args.n == 0) return true;
-          if (launch_icd_tick(ypl, nv, djobs.p, args, wg, tune.choose(ypl, nv, wg), stream, fw) !=
-              hipSuccess)
+          if (args.n == 1 && alone_ok) {
+            // one job alone in the launch with the other lane idle: the plain path's
+            // step kernel, with the job descriptor and the step's units in the launch
+            // arguments (no entry search, no descriptor or unit loads before the
+            // line's own)
+            const TickEntry& E = args.e[0];
+            mbir_job* j = first->job;
+            if (launch_icd_step(tick_desc(j), j->sched.p + E.line0, E.nlines, E.line0, (E.flags & 1) ? 0 : 1,
+                                0, (E.flags >> 1) & 1, stream, j->sched_h.data() + E.line0) !=
+                hipSuccess)
+              return false;
+          } else if (launch_icd_tick(ypl, nv, djobs.p, args, wg, tune.choose(ypl, nv, wg), stream,
+                                     fw) != hipSuccess) {
             return false;
+          }
           n_launch++;
           n_entries += args.n;
           n_wg += wg;
@@ -467,6 +484,7 @@ void mbir_server::run_loop()
               r->finishing || r->step >= r->nsteps)
             continue;
           mbir_job* j = r->job;
+          if (args.n == 0) first = r;
           TickEntry& E = args.e[args.n++];
           args.slots = std::max(args.slots, j->J.col_slots);
           E.job = r->slot;
