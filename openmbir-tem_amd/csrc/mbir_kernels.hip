@@ -1135,43 +1135,78 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
       // full batches: read-modify-write, every load of a batch in flight before its
       // stores (lanes whose slices did not change load and store nothing)
       float ev[U][YPL][NV], sv[SB ? U : 1][YPL][NV];
+      if constexpr (YPL * NV == 1) {
+        // one detector row per lane: one lane mask for the whole batch
+        if (d[0] != 0.f && vok[0][0]) {
 #pragma unroll
-      for (int q = 0; q < U / 4; ++q) {
-        if (base + 4 * q < cnt) {
-          const uint4 r4 = rows4(base + 4 * q);
-          const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
+          for (int q = 0; q < U / 4; ++q) {
+            if (base + 4 * q < cnt) {
+              const uint4 r4 = rows4(base + 4 * q);
+              const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
 #pragma unroll
-          for (int t = 0; t < 4; ++t)
+              for (int t = 0; t < 4; ++t) {
+                ev[4 * q + t][0][0] = ldo(J.e, rr[t] + vtb[0][0]);
+                if constexpr (SB) sv[4 * q + t][0][0] = ldo(J.bfs, rr[t] + vtb[0][0]);
+              }
+            }
+          }
 #pragma unroll
-            for (int jy = 0; jy < YPL; ++jy)
+          for (int q = 0; q < U / 4; ++q) {
+            if (base + 4 * q < cnt) {
+              const uint4 r4 = rows4(base + 4 * q);
+              const float4 a4 = ka4(base + 4 * q);
+              const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
+              const float aa[4] = {a4.x, a4.y, a4.z, a4.w};
 #pragma unroll
-              for (int k = 0; k < NV; ++k)
-                if (d[jy] != 0.f && vok[jy][k]) {
-                  ev[4 * q + t][jy][k] = ldo(J.e, rr[t] + vtb[jy][k]);
-                  if constexpr (SB) sv[4 * q + t][jy][k] = ldo(J.bfs, rr[t] + vtb[jy][k]);
-                }
+              for (int t = 0; t < 4; ++t) {
+                if constexpr (SB)
+                  *pto(J.e, rr[t] + vtb[0][0]) =
+                      enew_s(ev[4 * q + t][0][0], aa[t], 0, 0, sv[4 * q + t][0][0]);
+                else
+                  *pto(J.e, rr[t] + vtb[0][0]) = enew(ev[4 * q + t][0][0], aa[t], 0, 0);
+              }
+            }
+          }
         }
-      }
+      } else {
 #pragma unroll
-      for (int q = 0; q < U / 4; ++q) {
-        if (base + 4 * q < cnt) {
-          const uint4 r4 = rows4(base + 4 * q);
-          const float4 a4 = ka4(base + 4 * q);
-          const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
-          const float aa[4] = {a4.x, a4.y, a4.z, a4.w};
+        for (int q = 0; q < U / 4; ++q) {
+          if (base + 4 * q < cnt) {
+            const uint4 r4 = rows4(base + 4 * q);
+            const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
 #pragma unroll
-          for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int jy = 0; jy < YPL; ++jy)
+              for (int jy = 0; jy < YPL; ++jy)
 #pragma unroll
-              for (int k = 0; k < NV; ++k)
-                if (d[jy] != 0.f && vok[jy][k]) {
-                  if constexpr (SB)
-                    *pto(J.e, rr[t] + vtb[jy][k]) =
-                        enew_s(ev[4 * q + t][jy][k], aa[t], jy, k, sv[4 * q + t][jy][k]);
-                  else
-                    *pto(J.e, rr[t] + vtb[jy][k]) = enew(ev[4 * q + t][jy][k], aa[t], jy, k);
-                }
+                for (int k = 0; k < NV; ++k)
+                  if (d[jy] != 0.f && vok[jy][k]) {
+                    ev[4 * q + t][jy][k] = ldo(J.e, rr[t] + vtb[jy][k]);
+                    if constexpr (SB) sv[4 * q + t][jy][k] = ldo(J.bfs, rr[t] + vtb[jy][k]);
+                  }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < U / 4; ++q) {
+          if (base + 4 * q < cnt) {
+            const uint4 r4 = rows4(base + 4 * q);
+            const float4 a4 = ka4(base + 4 * q);
+            const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
+            const float aa[4] = {a4.x, a4.y, a4.z, a4.w};
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+              for (int jy = 0; jy < YPL; ++jy)
+#pragma unroll
+                for (int k = 0; k < NV; ++k)
+                  if (d[jy] != 0.f && vok[jy][k]) {
+                    if constexpr (SB)
+                      *pto(J.e, rr[t] + vtb[jy][k]) =
+                          enew_s(ev[4 * q + t][jy][k], aa[t], jy, k, sv[4 * q + t][jy][k]);
+                    else
+                      *pto(J.e, rr[t] + vtb[jy][k]) = enew(ev[4 * q + t][jy][k], aa[t], jy, k);
+                  }
+          }
         }
       }
     }
