@@ -316,11 +316,24 @@ def batched(ctx, wl, n, steps, single_value):
                 j.nuisance_update()
     dt = _t.perf_counter() - t0
     val = n * V * steps / dt
+    # the batched line kernel on this batch: algorithmic bytes of one more sweep over
+    # the device time of its launches (HIP events around the sweep's launches)
+    jobs[0].set_timing(True)
+    sts = capi.batch_sweep(ctx, jobs, inner=[2 + steps] * n)
+    nbytes = sum(algorithmic_bytes(st.nnz_updated, st.updated, st.zero_skipped) for st in sts)
+    ms = sts[0].kernel_ms
+    achieved = nbytes / (ms * 1e-3) / 1e9
     for j in jobs:
         j.close()
-    log(f"batched: {n} jobs x {steps} steps in {dt:.3f}s -> {val:.4g} visits/s")
+    log(f"batched: {n} jobs x {steps} steps in {dt:.3f}s -> {val:.4g} visits/s; "
+        f"kernel {achieved:.0f} GB/s")
     return {"jobs": n, "value": val, "unit": "voxel-updates/s", "ms_per_step": 1e3 * dt / steps,
-            "vs_single_job": val / single_value}
+            "vs_single_job": val / single_value,
+            "roofline": {"kernel": "k_icd_batch_step", "bound": "hbm", "achieved": round(achieved, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_model": BYTES_MODEL,
+                         "bytes_per_sweep": nbytes, "kernel_ms_per_sweep": ms,
+                         "timing": "HIP events around one batched sweep's launches"}}
 
 
 def _pct(v, q):
