@@ -136,7 +136,11 @@ __device__ inline float qggmrf_b(float d, const DevJob& J)
   const float t1 = __builtin_amdgcn_exp2f(J.pq2 * __builtin_amdgcn_logf(a)) * J.inv_sigma_pq;
   const float r3 = __builtin_amdgcn_rcpf(J.mrf_c + t1);
   const float b = (2.f - J.pq2 * t1 * r3) * r3 * J.inv_sigma_p;
-  return d == 0.f ? J.b0 : b;
+  // b0 read first: with a member read in the conditional's arm the front end emits a
+  // branch, the evaluation above sinks into it, and the callers' evaluations then run
+  // one after another instead of interleaved
+  const float b0 = J.b0;
+  return d == 0.f ? b0 : b;
 }
 
 // Butterfly reduction inside aligned groups of G (1, 2, 4, 8 or 16) lanes with DPP
