@@ -153,20 +153,53 @@ def test_sweep_equals_the_sequential_order_of_its_schedule(gpu_ctx, y_slabs):
             keys = sorted(zs)
             for a, b in zip(keys, keys[1:]):
                 assert abs(next(iter(zs[a])) - next(iter(zs[b]))) >= 2, st
-        fwd = O.Engine(g, ang, y, prm, x0=x0)
-        rev = O.Engine(g, ang, y, prm, x0=x0)
-        fwd.set_e(e0)
-        rev.set_e(e0)
-        fwd.update_units(units, 0, it)
-        rev.update_units(np.concatenate([st[::-1] for st in steps]), 0, it)
-        np.testing.assert_array_equal(fwd.x(), rev.x())
-        np.testing.assert_array_equal(fwd.e(), rev.e())
-        xr, xg = fwd.x(), job.volume()
-        d = np.abs(xg - xr)
-        scale = np.abs(xr).max()
-        far = int(np.count_nonzero(d > 1e-3 * scale))
-        assert d.mean() <= 1e-5 * scale and far <= 0.005 * d.size and d.max() <= 5e-2 * scale, (
-            it, d.max(), d.mean(), far)
+        _replay_matches(job, g, ang, y, prm, x0, e0, steps, it)
+
+
+def _replay_matches(job, g, ang, y, prm, x0, e0, steps, it):
+    """The oracle's replay of a GPU sweep's schedule from the GPU's state before it:
+    step order and reverse order within each step bit for bit equal, and the GPU's
+    volume within the float32 bound of the schedule-order test."""
+    units = np.concatenate(steps)
+    fwd = O.Engine(g, ang, y, prm, x0=x0)
+    rev = O.Engine(g, ang, y, prm, x0=x0)
+    fwd.set_e(e0)
+    rev.set_e(e0)
+    fwd.update_units(units, 0, it)
+    rev.update_units(np.concatenate([st[::-1] for st in steps]), 0, it)
+    np.testing.assert_array_equal(fwd.x(), rev.x())
+    np.testing.assert_array_equal(fwd.e(), rev.e())
+    xr, xg = fwd.x(), job.volume()
+    d = np.abs(xg - xr)
+    scale = np.abs(xr).max()
+    far = int(np.count_nonzero(d > 1e-3 * scale))
+    assert d.mean() <= 1e-5 * scale and far <= 0.005 * d.size and d.max() <= 5e-2 * scale, (
+        it, d.max(), d.mean(), far)
+
+
+@pytest.mark.parametrize("n_t", [1, 13, 100, 200])
+def test_line_lengths_match_the_sequential_order(gpu_ctx, n_t):
+    """Line lengths that take other paths through the line update: one slice (64
+    streams per wave, a solve with no odd slice), an odd count (the last even slice's
+    upper neighbour outside the unit), 65-128 slices (two per lane, both phases still
+    in one wave) and 129-256 (four per lane: both phases on every wave, a barrier
+    between).  Two sweeps each, every one against the oracle's replay of its schedule
+    from the GPU's state before it."""
+    g, ang, vol, y = small_problem(n_r=24, n_t=n_t, n_views=21, thickness=8, seed=11)
+    assert g.N_y == n_t
+    job = _job(gpu_ctx, g, ang, y, default_recon=0.1, zero_skipping=0)
+    prm = O.params(zero_skipping=0)
+    for it in range(2):
+        x0 = job.volume().astype(np.float64)
+        e0 = job.error_sinogram().astype(np.float64)
+        job.sweep(0, it)
+        steps = job.schedule()
+        cover = np.zeros((g.N_z * g.N_x, g.N_y), np.int32)
+        for l, y0, ny in np.concatenate(steps):
+            cover[l, y0:y0 + ny] += 1
+        assert cover.max() == 1
+        _replay_matches(job, g, ang, y, prm, x0, e0, steps, it)
+    job.close()
 
 
 def test_converged_volume_matches_oracle(gpu_ctx):
