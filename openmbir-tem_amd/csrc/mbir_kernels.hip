@@ -2024,6 +2024,8 @@ hipError_t launch_icd_batch_step(int ypl, int nv, const DevJob* jobs, const Batc
 #undef MBIR_BATCHF
     return hipGetLastError();
   }
+  // the batched forms cover units of at most 128 slices and two detector rows per slice
+  if (ypl > 2 || nv > 2) return hipErrorInvalidValue;
 #define MBIR_BATCH(Y, N)                                                                      \
   if (pf)                                                                                     \
     hipLaunchKernelGGL((k_icd_batch_step<Y, N, true>), dim3(nwg), dim3(kIcdBlock), dyn, s,    \
@@ -2066,6 +2068,8 @@ hipError_t launch_icd_tick(int ypl, int nv, const DevJob* jobs, const TickArgs& 
 #undef MBIR_TICKF
     return hipGetLastError();
   }
+  // the server's forms cover units of at most 128 slices and two detector rows per slice
+  if (ypl > 2 || nv > 2) return hipErrorInvalidValue;
 #define MBIR_TICK(Y, N)                                                                        \
   if (pf)                                                                                      \
     hipLaunchKernelGGL((k_icd_tick<Y, N, true>), dim3(nwg), dim3(kIcdBlock), dyn, s, jobs,      \

@@ -70,6 +70,29 @@ def test_batch_result_same_alone_and_in_mix(gpu_ctx):
         np.testing.assert_array_equal(a.error_sinogram(), m.error_sinogram())
 
 
+def test_batch_line_lengths_same_alone_and_in_mix(gpu_ctx):
+    """Lines of 1, 13 and 100 slices (100: two per lane) and of 200 slices cut into y
+    slabs by the automatic choice: each job alone on its own step kernel and all four
+    in one batched sweep (three kernel shapes per launch) give the same bits."""
+    shapes = [(1, 1), (13, 1), (100, 1), (200, 0)]  # (slices, y slabs; 0: automatic)
+    probs = [small_problem(n_r=24, n_t=nt, n_views=21, thickness=8, seed=60 + i)
+             for i, (nt, _) in enumerate(shapes)]
+    alone = [_job(gpu_ctx, *p[:2], p[3], kernel_mode=0, y_slabs=ys, seed=70 + i)
+             for i, (p, (_, ys)) in enumerate(zip(probs, shapes))]
+    mixed = [_job(gpu_ctx, *p[:2], p[3], kernel_mode=1, y_slabs=ys, seed=70 + i)
+             for i, (p, (_, ys)) in enumerate(zip(probs, shapes))]
+    for it in range(3):
+        for j in alone:
+            j.sweep(0, it)
+        st = capi.batch_sweep(gpu_ctx, mixed, outer=[0] * 4, inner=[it] * 4)
+        assert all(s.updated + s.zero_skipped > 0 for s in st)
+    for a, m in zip(alone, mixed):
+        np.testing.assert_array_equal(a.volume(), m.volume())
+        np.testing.assert_array_equal(a.error_sinogram(), m.error_sinogram())
+        a.close()
+        m.close()
+
+
 def test_autotune_keeps_results(gpu_ctx):
     """The autotuner times both forms of the batched line kernels and keeps the faster
     per shape and launch-size bucket; since the forms compute identical values, jobs

@@ -109,6 +109,24 @@ def test_server_two_lanes_results_equal_alone(gpu_ctx):
     srv.close()
 
 
+def test_server_line_lengths_equal_alone(gpu_ctx):
+    """Lines of 1, 13 and 100 slices (100 as whole lines: two per lane) served together
+    end bit for bit as each run alone."""
+    cases = []
+    for i, nt in enumerate((1, 13, 100)):
+        g, ang, vol, y = small_problem(n_r=24, n_t=nt, n_views=21, thickness=8, seed=80 + i)
+        cases.append((g, ang, y, dict(seed=90 + i, y_slabs=1, stop_threshold=2e-3,
+                                      num_inner=15)))
+    refs = [_alone(gpu_ctx, c) for c in cases]
+    srv = capi.Server(gpu_ctx, max_jobs=4)
+    tickets = [srv.submit(_geom(g, ang), ang, y, **kw) for g, ang, y, kw in cases]
+    for t, ref in zip(tickets, refs):
+        rep = srv.wait(t, timeout=120.0)
+        _check_same(srv.job(t), ref, rep)
+        srv.release(t)
+    srv.close()
+
+
 def test_server_budget_admission(gpu_ctx):
     cases = _cases()[:4]
     need = [capi.problem_device_bytes(_geom(g, ang), ang, y, **kw) for g, ang, y, kw in cases]
