@@ -256,6 +256,41 @@ def test_bf_jobs_in_a_batched_sweep_equal_alone(gpu_ctx):
         j.close()
 
 
+@pytest.mark.parametrize("n_t", [1, 13, 100])
+def test_bf_line_lengths(gpu_ctx, n_t):
+    """Bright-field lines of 1, 13 and 100 slices (100 as whole lines: two per lane):
+    the setup against the oracle, the cost never rises over sweeps, e stays
+    y - A x - mu for the GPU's own volume, the selector follows its rule, and the same
+    job in a batched sweep ends every sweep bit for bit as alone."""
+    g, ang, vol, counts = bf_problem(n_r=24, n_t=n_t, n_views=21, seed=13, bragg_frac=0.05)
+    assert g.N_y == n_t
+    offs = np.full(len(ang), -np.log(2000.0))
+    eng = O.Engine(g, ang, counts, O.params(**BF), offsets=offs)
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
+    jobs = [capi.Job(gpu_ctx, geom, ang, counts.astype(np.float32), init_offset=offs,
+                     y_slabs=1, seed=5, kernel_mode=m, **BF) for m in (0, 1)]
+    job, bat = jobs
+    ref_e = eng.e()
+    assert np.max(np.abs(job.error_sinogram() - ref_e)) <= 1e-5 * np.max(np.abs(ref_e))
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+    costs = [job.cost()]
+    for it in range(6):
+        job.sweep(0, it)
+        capi.batch_sweep(gpu_ctx, [bat], outer=[0], inner=[it])
+        np.testing.assert_array_equal(bat.volume(), job.volume())
+        np.testing.assert_array_equal(bat.error_sinogram(), job.error_sinogram())
+        costs.append(job.cost())
+    assert all(b <= a * (1 + 1e-6) + 1e-9 for a, b in zip(costs, costs[1:])), costs
+    y = -np.log(counts)
+    e_ref = y - eng.project(job.volume().astype(np.float64)) - offs[:, None, None]
+    assert np.max(np.abs(job.error_sinogram() - e_ref)) <= 2e-5 * np.max(np.abs(e_ref))
+    e, w, sel = job.error_sinogram(), job.weights(), job.selector()
+    rule = (np.abs(e * np.sqrt(w)) < 3.0).astype(np.uint8)
+    assert np.mean(sel != rule) < 1e-3
+    for j in jobs:
+        j.close()
+
+
 def test_bf_four_rows_per_slice_matches_oracle(gpu_ctx):
     """Bright field at four detector rows per slice (level 0 of three resolution levels,
     the reference's default for its bright-field reconstruction): setup against the
