@@ -202,6 +202,42 @@ def test_line_lengths_match_the_sequential_order(gpu_ctx, n_t):
     job.close()
 
 
+@pytest.mark.parametrize("views,ratio", [(512, 1), (227, 4)])
+def test_widest_columns_match_the_sequential_order(gpu_ctx, views, ratio):
+    """The longest columns the line update takes: 512 views (one per thread of the
+    column staging) at three bins per view, and 227 views of voxels four detector
+    pitches wide, nine bins per view (2043 of the 2048 column slots).  One sweep each
+    against the oracle's replay of its schedule from the GPU's state before it."""
+    n_r, n_t = 24, 8
+    ang = P.tilt_series(views, 70.0)
+    g = O.geometry(n_r, n_t, ang, thickness=8, delta_xz=float(ratio),
+                   interpolate_factor=float(ratio))
+    shape = (g.N_z, g.N_x, g.N_y)
+    vol = P.random_ellipsoids(shape, n=3, seed=17).astype(np.float64)
+    prm = O.params(zero_skipping=0)
+    proj = O.Engine(g, ang, np.zeros((views, n_r, n_t)), prm, x0=vol).yest()
+    y = P.noisy_sinogram(proj, 0.02, 18)
+    geom = capi.geometry_init(n_r, n_t, ang, g.LengthZ / 2, delta_xz_in=float(ratio),
+                              interpolate_factor=float(ratio))
+    assert (geom.N_x, geom.N_y, geom.N_z) == (g.N_x, g.N_y, g.N_z)
+    job = capi.Job(gpu_ctx, geom, ang, y, default_recon=0.1, zero_skipping=0)
+    x0 = job.volume().astype(np.float64)
+    e0 = job.error_sinogram().astype(np.float64)
+    job.sweep(0, 0)
+    _replay_matches(job, g, ang, y, prm, x0, e0, job.schedule(), 0)
+    job.close()
+
+
+def test_more_views_than_the_column_staging_takes_are_refused(gpu_ctx):
+    """513 views exceed the one-thread-per-view column staging: the job is refused with
+    MBIR_ERR_UNSUPPORTED (-3), before any device work."""
+    ang = P.tilt_series(513, 70.0)
+    geom = capi.geometry_init(16, 4, ang, 4.0)
+    with pytest.raises(capi.MbirError) as ei:
+        capi.Job(gpu_ctx, geom, ang, np.zeros((513, 16, 4), np.float32))
+    assert ei.value.code == -3
+
+
 def test_converged_volume_matches_oracle(gpu_ctx):
     g, ang, vol, y = small_problem()
     eng = _engine(g, ang, y)
