@@ -1884,15 +1884,15 @@ static void launch_step_wide(const DevJob& J, const LineRef* lines, int nlines, 
 // compute units of the current device (all devices of a process alike), once
 static int device_cus()
 {
-  static int n = 0;
-  if (n == 0) {
+  // read once (a function-local static: initialised once even when the server's setup
+  // and tick threads call this together)
+  static const int n = [] {
     int dev = 0, v = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
-      n = v;
-    else
-      n = 256;
-  }
+      return v;
+    return 256;
+  }();
   return n;
 }
 
