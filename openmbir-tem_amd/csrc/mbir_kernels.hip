@@ -395,9 +395,9 @@ __global__ __launch_bounds__(kBlock) void k_line_kk(DevJob J, const int* col_ptr
 //      terms of one voxel are spread over G lanes and combined by DPP moves;
 //   6. sweep statistics: per-wave shares formed in the solve, summed by one thread
 //      in a fixed order, one set of global atomics per line;
-//   7. e -= I0 a vlr dx: fp32 atomics (one job alone) or a plain read-modify-write
-//      (full batches); no two lines of a conflict-free step share a detector bin,
-//      so both give the same e, and both round alike;
+//   7. e -= I0 a vlr dx: from the registers of step 2 when a stream fits one batch,
+//      else a plain read-modify-write (no two lines of a conflict-free step share a
+//      detector bin), or fp32 atomics on the relaxed schedule; all round alike;
 
 // Byte-offset access through a uniform base pointer: base in scalar registers plus
 // a 32-bit offset in one vector register (global load/store with a scalar base).
@@ -1112,8 +1112,7 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
         }
       }
     } else if (J.e_atomic) {
-      // one job alone: fp32 atomics, no load round trip (no two lines of a
-      // conflict-free step share a detector bin, so both forms give the same e)
+      // relaxed schedule (lines of a step may share detector bins): fp32 atomics
 #pragma unroll
       for (int q = 0; q < U / 4; ++q) {
         if (base + 4 * q < cnt) {
