@@ -1029,13 +1029,15 @@ __device__ void icd_line(const DevJob& J, const LineRef& ref, int first, int zs_
   auto enew_s = [&](float e, float ka, int jy, int k, float sc) {
     return __fadd_rn(e, -__fmul_rn(__fmul_rn(ka, vd[jy][k]), sc));
   };
-  // PF with one batch per stream: the stream's error-sinogram values (and the
-  // bright-field weights or the scales) are still in registers from the theta pass,
-  // and no other line of the launch touches these bins, so the new values go out as
-  // plain stores with no second round trip
+  // PF with one batch per stream on a conflict-free step: the stream's error-sinogram
+  // values (and the bright-field weights or the scales) are still in registers from
+  // the theta pass, and no other line of the launch touches these bins, so the new
+  // values go out as plain stores with no second round trip.  Not on the relaxed
+  // schedule (e_atomic), whose lines may share bins: a plain store there would drop
+  // another line's update.
   bool from_regs = false;
   if constexpr (PF && KR && YPL * NV <= (FW || SB ? 2 : 4)) {  // (wider shapes would spill)
-    if (C <= U) {  // uniform
+    if (C <= U && !J.e_atomic) {  // uniform
       from_regs = true;
       // per lane and row: the slice changed (updated, bright field) and has that row
       bool pr[YPL][NV];
