@@ -119,6 +119,50 @@ def test_relaxed_schedule(gpu_ctx, kernel_mode):
     exact.close()
 
 
+@pytest.mark.parametrize("gain_scale", [0.9, 1.1])
+def test_initial_nuisance_matches_oracle(gpu_ctx, gain_scale):
+    """gainAndOffsetInitialization (HAADF_ReconstructionEngine_Extra.cpp:280-307) and
+    the initial variances: per-view arrays, or the scalar defaults.  The gains are
+    rescaled to the target mean only when their mean exceeds it by more than 1e-5
+    (one-sided, as there): a mean below the target keeps the given gains.  I0, mu,
+    alpha, e and the cost at setup against the oracle."""
+    g, ang, vol, y = small_problem()
+    rng = np.random.default_rng(3)
+    nv = len(ang)
+    gains = gain_scale * rng.uniform(0.9, 1.1, nv)
+    gains *= gain_scale / gains.mean()  # mean exactly gain_scale x the target (1)
+    offs = rng.uniform(-0.05, 0.05, nv)
+    var = rng.uniform(0.5, 2.0, nv)
+    job = _job(gpu_ctx, g, ang, y, init_gain=gains, init_offset=offs, init_variance=var,
+               noise_model=1)
+    eng = O.Engine(g, ang, y, O.params(noise_model=1), gains=gains, offsets=offs, variances=var)
+    (I0, mu, al), (I0r, mur, alr) = job.nuisance(), eng.nuisance()
+    np.testing.assert_allclose(I0, I0r, rtol=1e-12)
+    np.testing.assert_allclose(mu, mur, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(al, alr, rtol=1e-15)
+    if gain_scale > 1:
+        assert I0.mean() == pytest.approx(1.0, rel=1e-12)
+    else:
+        np.testing.assert_allclose(I0, gains, rtol=1e-15)
+    np.testing.assert_allclose(mu, offs, rtol=1e-15)
+    np.testing.assert_allclose(al, var, rtol=1e-15)
+    ref_e = eng.e()
+    assert np.max(np.abs(job.error_sinogram() - ref_e)) <= 1e-5 * np.max(np.abs(ref_e))
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+    job.close()
+    # the scalar defaults: target gain, default offset and variance for every view
+    job = _job(gpu_ctx, g, ang, y, target_gain=1.5, default_offset=0.02, default_variance=0.3,
+               noise_model=1)
+    eng = O.Engine(g, ang, y, O.params(noise_model=1, target_gain=1.5), gains=np.full(nv, 1.5),
+                   offsets=np.full(nv, 0.02), variances=np.full(nv, 0.3))
+    I0, mu, al = job.nuisance()
+    np.testing.assert_array_equal(I0, np.full(nv, 1.5))
+    np.testing.assert_array_equal(mu, np.full(nv, 0.02))
+    np.testing.assert_array_equal(al, np.full(nv, 0.3))
+    assert job.cost() == pytest.approx(eng.cost(), rel=1e-6)
+    job.close()
+
+
 def test_first_sweep_statistics_match_oracle(gpu_ctx):
     """The stop-test sums of the first sweep do not depend on the update order: every
     visited voxel's old value is the initial one, and nothing is zero-skipped.  The
