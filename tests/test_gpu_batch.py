@@ -31,12 +31,15 @@ def test_batched_kernel_equals_per_step(gpu_ctx):
     np.testing.assert_array_equal(a.error_sinogram(), b.error_sinogram())
 
 
-def test_graph_sweeps_equal_per_step(gpu_ctx):
+@pytest.mark.parametrize("n_t,y_slabs", [(8, 1), (16, 2), (100, 1)])
+def test_graph_sweeps_equal_per_step(gpu_ctx, n_t, y_slabs):
     """Sweeps replayed from a HIP graph (kernel_mode 3) give the per-step launches'
     result bit for bit, replay after replay, also across nuisance updates (which
-    change the job descriptor and make the graph be captured again)."""
-    g, ang, vol, y = small_problem()
-    kw = dict(seed=3, joint_estimation=1, noise_model=1)
+    change the job descriptor and make the graph be captured again); whole lines, y-slab
+    units (the graph's step slots carry each unit's y-range) and lines of two slices
+    per lane."""
+    g, ang, vol, y = small_problem(n_t=n_t)
+    kw = dict(seed=3, joint_estimation=1, noise_model=1, y_slabs=y_slabs)
     a = _job(gpu_ctx, g, ang, y, kernel_mode=0, **kw)
     b = _job(gpu_ctx, g, ang, y, kernel_mode=3, **kw)
     for outer in range(3):
