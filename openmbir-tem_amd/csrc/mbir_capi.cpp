@@ -766,25 +766,49 @@ double nhicd_threshold(std::vector<double> v)
   return v[k];
 }
 
-// Conflict-free steps of a list of lines: classes (z, x mod S), lines of a class in
-// increasing x, classes in a seeded random order; lines with empty columns dropped.
+// Conflict-free steps of a list of lines (an NHICD call: a twentieth of the lines or
+// fewer).  Two lines of one row conflict exactly when they lie less than S apart (the
+// spacing of a sweep's classes (z, x mod S), which also keeps prior neighbours apart),
+// so a row's lines, in increasing x, go to the first step of that row whose last line
+// lies at least S to the left: the fewest steps the row needs (interval colouring),
+// where the classes of a full sweep would take up to S mostly nearly empty ones.  Steps
+// in a seeded random order; lines with empty columns dropped.
 void build_list_schedule(mbir_job* j, const std::vector<int>& lines)
 {
   const int Nx = j->g.N_x, S = j->spacing;
-  std::vector<std::vector<int>> cls((size_t)j->g.N_z * S);
-  std::vector<int> sorted(lines);
-  std::sort(sorted.begin(), sorted.end());
-  for (int l : sorted)
-    if (j->nnz_h[l] > 0) cls[(size_t)(l / Nx) * S + (l % Nx) % S].push_back(l);
-  std::vector<int> order;
-  for (size_t c = 0; c < cls.size(); ++c)
-    if (!cls[c].empty()) order.push_back((int)c);
+  std::vector<int> sorted;
+  sorted.reserve(lines.size());
+  for (int l : lines)
+    if (j->nnz_h[l] > 0) sorted.push_back(l);
+  std::sort(sorted.begin(), sorted.end());  // by row, then x
+  std::vector<std::vector<int>> steps;
+  std::vector<int> last;  // x of the last line of each step of the current row
+  size_t row0 = 0;        // the current row's first step
+  int row = -1;
+  for (int l : sorted) {
+    const int z = l / Nx, x = l % Nx;
+    if (z != row) {
+      row = z;
+      row0 = steps.size();
+      last.clear();
+    }
+    size_t k = 0;
+    while (k < last.size() && last[k] > x - S) ++k;
+    if (k == last.size()) {
+      steps.emplace_back();
+      last.push_back(x);
+    }
+    steps[row0 + k].push_back(l);
+    last[k] = x;
+  }
+  std::vector<int> order(steps.size());
+  std::iota(order.begin(), order.end(), 0);
   std::mt19937 rng(j->prob.seed + 7919u * (uint32_t)j->sweep_counter);
   std::shuffle(order.begin(), order.end(), rng);
   j->sched_h.clear();
   j->step_ptr.assign(1, 0);
   for (int c : order) {
-    for (int l : cls[c]) j->sched_h.push_back(LineRef{l, j->cptr_h[l], j->nnz_h[l], 0});
+    for (int l : steps[c]) j->sched_h.push_back(LineRef{l, j->cptr_h[l], j->nnz_h[l], 0});
     j->step_ptr.push_back((int)j->sched_h.size());
   }
 }

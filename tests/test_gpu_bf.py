@@ -145,6 +145,44 @@ def test_bf_nhicd_lists_and_stop_call_match_oracle(gpu_ctx):
     assert len(np.intersect1d(a, b)) >= 0.9 * max(len(a), len(b)), (np.sort(a), np.sort(b))
 
 
+def test_bf_nhicd_call_steps_are_conflict_free(gpu_ctx):
+    """An NHICD call's lines are packed row by row into as few steps as the row needs.
+    The steps of the last call (homogeneous and non-homogeneous calls alike) hold every
+    line of the call once, and no two lines of a step share a detector bin (their
+    exported columns) or are prior neighbours, so each step is still equivalent to any
+    sequential order of its lines; each step stays within one row."""
+    g, ang, vol, counts = bf_problem(n_r=64, n_t=6, n_views=31, bragg_frac=0.03)
+    offs = np.full(len(ang), -np.log(2000.0))
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
+    for calls in (5, 6):  # the last call homogeneous (even), then non-homogeneous
+        kw = dict(BF, bf_nhicd=1, num_outer=1, num_inner=calls, stop_threshold=0.0,
+                  zero_skipping=0, seed=5)
+        job = capi.Job(gpu_ctx, geom, ang, counts.astype(np.float32), init_offset=offs, **kw)
+        job.set_nhicd_log(True)
+        rs, _ = job.run()
+        assert rs.sweeps == calls
+        steps = job.schedule()
+        last = job.nhicd_lists()[-1]
+        units = np.concatenate(steps)
+        nx = geom.N_x
+        cols = {}
+        for l in np.unique(last):
+            idx, _ = job.column(int(l) // nx, int(l) % nx)
+            if len(idx):
+                cols[int(l)] = set(idx.tolist())
+        np.testing.assert_array_equal(np.sort(units[:, 0]), np.sort(list(cols)))
+        assert len(steps) < len(units)
+        for st in steps:
+            ls = [int(l) for l in st[:, 0]]
+            assert len({l // nx for l in ls}) == 1
+            for a in range(len(ls)):
+                for b in range(a + 1, len(ls)):
+                    za, xa, zb, xb = ls[a] // nx, ls[a] % nx, ls[b] // nx, ls[b] % nx
+                    assert abs(za - zb) > 1 or abs(xa - xb) > 1, (ls[a], ls[b])
+                    assert not (cols[ls[a]] & cols[ls[b]]), (ls[a], ls[b])
+        job.close()
+
+
 def test_bf_nhicd_converges_to_the_full_sweep_minimum(gpu_ctx):
     """The NHICD schedule (homogeneous twentieths alternating with the filtered top-5 %
     lists) only reorders the updates.  With one outer iteration the Bragg threshold
