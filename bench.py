@@ -583,6 +583,20 @@ def bf_figures(ctx, steps):
     value = V * steps / dt
     log(f"bf: {steps} steps in {dt:.3f}s -> {value:.4g} visits/s")
     job.close()
+    # NHICD calls, the reference engine's bright-field schedule (BrightField/BFConstants.h:51
+    # defines NHICD): 42 calls (21 homogeneous, 21 non-homogeneous) from the start
+    kw_n = dict(w["job_kw"], bf_nhicd=1, num_outer=1, num_inner=42, stop_threshold=0.0,
+                joint_estimation=0, noise_model=0)
+    jn = capi.Job(ctx, w["geom"], w["angles"], w["counts"], **kw_n)
+    jn.set_nhicd_log(True)
+    t0 = time.perf_counter()
+    rs_n, _ = jn.run()
+    dt_n = time.perf_counter() - t0
+    lines_n = sum(len(c) for c in jn.nhicd_lists())
+    jn.close()
+    nhicd = {"calls": int(rs_n.sweeps), "ms_per_call": 1e3 * dt_n / rs_n.sweeps,
+             "lines_per_call": lines_n / rs_n.sweeps, "voxel_updates_per_s": lines_n * g.N_y / dt_n}
+    log(f"bf: NHICD {nhicd}")
 
     # float32 engine vs FP64 oracle from identical states
     s = workloads.BF_128x16x64
@@ -636,7 +650,7 @@ def bf_figures(ctx, steps):
     return {"workload": wl.name, "value": value, "unit": "voxel-updates/s",
             "ms_per_step": 1e3 * dt / steps,
             "step": "ICD sweep + offset (log dose) + shared variance update",
-            "fp32_vs_fp64": err, "fp32_vs_fp64_sample": s.name}
+            "nhicd": nhicd, "fp32_vs_fp64": err, "fp32_vs_fp64_sample": s.name}
 
 
 def main():
