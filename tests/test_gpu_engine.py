@@ -338,3 +338,20 @@ def test_reference_shepp_logan_fixture(gpu_ctx):
     rel = np.sqrt(np.mean((xr - xg) ** 2)) / np.sqrt(np.mean(xr ** 2))
     assert rel <= 2e-2, rel
     assert job.cost() == pytest.approx(eng.cost(), rel=2e-5)
+
+
+def test_adapter_reports_unreadable_tilt_series(gpu_ctx, tmp_path):
+    """A tilt series that does not exist, or an MRC file cut short after its header,
+    ends execute() with a negative error condition (the reference's input error) and
+    no outputs, instead of a crash or a reconstruction of garbage."""
+    short = tmp_path / "short.mrc"
+    short.write_bytes(b"\0" * 100)
+    for src in (str(tmp_path / "missing.mrc"), str(short)):
+        out = tmp_path / ("out_" + os.path.basename(src))
+        out.mkdir()
+        r = subprocess.run([CLI, "--mrc", src, "32", "0.1", "1.2", "1", "2", "0", "0", "1",
+                            str(out)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 1, r.stdout + r.stderr
+        info = (out / "run.txt").read_text().split("\n")
+        assert int(info[0].split()[1]) < 0, info
+        assert not (out / "recon.mrc").exists()
