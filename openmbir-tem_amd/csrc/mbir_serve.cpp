@@ -84,8 +84,8 @@ int64_t estimate_bytes(const mbir_problem& p)
          (p.bf_mode ? M * 8 : 0);
 }
 
-// The job's descriptor for the server's launches: under a full device the plain
-// error-sinogram update is the faster form (conflict-free schedules only).
+// The job's descriptor for the server's launches: the plain error-sinogram update on
+// conflict-free schedules, fp32 atomics on the relaxed one (as the job's own path).
 DevJob tick_desc(const mbir_job* j)
 {
   DevJob d = j->J;
