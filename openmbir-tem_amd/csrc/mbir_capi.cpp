@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <mutex>
 #include <numeric>
 #include <random>
@@ -742,28 +743,76 @@ constexpr double kNhHamming[5][5] = {{0.0013, 0.0086, 0.0159, 0.0086, 0.0013},
 constexpr int kNhParts = 20;  // k_NumHomogeniousIter = k_NumNonHomogeniousIter
 
 // ComputeVSC (Extra.cpp:888-925): the magnitude map filtered with the window, zero
-// outside the volume.
+// outside the volume.  Cells with every tap inside take a loop without bound tests over
+// the same 25 terms in the same order (bit for bit the bounded sum); large maps split
+// their rows over a few host threads.
 void nhicd_filter(const float* mag, int Nz, int Nx, double* out)
 {
-  for (int i = 0; i < Nz; ++i)
-    for (int j = 0; j < Nx; ++j) {
-      double f = 0;
-      for (int p = -2; p <= 2; ++p)
-        for (int q = -2; q <= 2; ++q)
-          if (i + p >= 0 && i + p < Nz && j + q >= 0 && j + q < Nx)
-            f += kNhHamming[p + 2][q + 2] * (double)mag[(size_t)(i + p) * Nx + (j + q)];
-      out[(size_t)i * Nx + j] = f;
+  auto cell = [&](int i, int j) {
+    double f = 0;
+    for (int p = -2; p <= 2; ++p)
+      for (int q = -2; q <= 2; ++q)
+        if (i + p >= 0 && i + p < Nz && j + q >= 0 && j + q < Nx)
+          f += kNhHamming[p + 2][q + 2] * (double)mag[(size_t)(i + p) * Nx + (j + q)];
+    return f;
+  };
+  auto rows = [&](int i0, int i1) {
+    for (int i = i0; i < i1; ++i) {
+      double* o = out + (size_t)i * Nx;
+      if (i < 2 || i + 2 >= Nz || Nx < 5) {
+        for (int j = 0; j < Nx; ++j) o[j] = cell(i, j);
+        continue;
+      }
+      for (int j = 0; j < 2; ++j) o[j] = cell(i, j);
+      for (int j = Nx - 2; j < Nx; ++j) o[j] = cell(i, j);
+      const float* r[5];
+      for (int p = 0; p < 5; ++p) r[p] = mag + (size_t)(i - 2 + p) * Nx;
+      for (int j = 2; j < Nx - 2; ++j) {
+        double f = 0;
+        for (int p = 0; p < 5; ++p)
+          for (int q = 0; q < 5; ++q) f += kNhHamming[p][q] * (double)r[p][j - 2 + q];
+        o[j] = f;
+      }
     }
+  };
+  const int nt = (int64_t)Nz * Nx >= (1 << 16)
+                     ? std::max(1, std::min(8, (int)std::thread::hardware_concurrency()))
+                     : 1;
+  if (nt == 1) {
+    rows(0, Nz);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back(rows, (int)((int64_t)Nz * t / nt), (int)((int64_t)Nz * (t + 1) / nt));
+  for (auto& x : th) x.join();
 }
 
 // SetNonHomThreshold (Extra.cpp:934-987): the (n - n/20)-th smallest filtered value.
+// A histogram of the top 16 bits of an order-preserving key (the bit pattern, sign
+// folded) finds the bucket holding that rank; nth_element over that bucket alone then
+// returns the value nth_element over all values would.
 double nhicd_threshold(std::vector<double> v)
 {
   const size_t n = v.size();
   const size_t k = n - n / kNhParts;
   if (k >= n) return v.empty() ? 0.0 : *std::max_element(v.begin(), v.end());
-  std::nth_element(v.begin(), v.begin() + (long)k, v.end());
-  return v[k];
+  auto key = [](double d) {
+    uint64_t u;
+    std::memcpy(&u, &d, sizeof u);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+  };
+  std::vector<uint32_t> hist(1u << 16, 0);
+  for (double d : v) hist[key(d) >> 48]++;
+  size_t below = 0;
+  uint32_t b = 0;
+  while (below + hist[b] <= k) below += hist[b++];
+  std::vector<double> cand;
+  cand.reserve(hist[b]);
+  for (double d : v)
+    if ((key(d) >> 48) == b) cand.push_back(d);
+  std::nth_element(cand.begin(), cand.begin() + (long)(k - below), cand.end());
+  return cand[k - below];
 }
 
 // Conflict-free steps of a list of lines (an NHICD call: a twentieth of the lines or

@@ -41,3 +41,27 @@ def test_threshold_is_the_95th_percentile_order_statistic(n):
     assert t == np.sort(v)[n - n // 20]
     # with distinct values the list (values above the threshold) holds n // 20 - 1 lines
     assert np.count_nonzero(v > t) == n // 20 - 1
+
+
+def test_filter_of_a_large_map():
+    """A map of more than 2^16 lines (filtered in row blocks on several host threads)
+    against the same convolution."""
+    rng = np.random.default_rng(2)
+    mag = rng.exponential(1.0, (300, 260)).astype(np.float32)
+    mag[rng.uniform(size=mag.shape) < 0.5] = 0.0
+    got = capi.nhicd_filter(mag)
+    want = convolve2d(mag.astype(np.float64), WINDOW, mode="same", boundary="fill")
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["ties", "zeros", "signs", "tiny"])
+def test_threshold_of_a_large_list(kind):
+    """131072 values (a 512 x 256 map) with repeated values, many zeros, both signs
+    and values near the smallest doubles: the order statistic of a full sort."""
+    rng = np.random.default_rng(3)
+    n = 131072
+    v = {"ties": np.floor(rng.uniform(0, 8, n)),
+         "zeros": np.where(rng.uniform(size=n) < 0.97, 0.0, rng.exponential(1.0, n)),
+         "signs": rng.normal(size=n) * 10.0 ** rng.integers(-30, 30, n),
+         "tiny": rng.uniform(size=n) * 1e-300}[kind]
+    assert capi.nhicd_threshold(v) == np.sort(v)[n - n // 20]
