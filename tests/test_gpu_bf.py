@@ -329,6 +329,34 @@ def test_bf_line_lengths(gpu_ctx, n_t):
         j.close()
 
 
+def test_bf_graph_sweeps_equal_per_step(gpu_ctx):
+    """Bright-field sweeps replayed from a HIP graph (kernel_mode 3) end bit for bit as the
+    per-step launches: e, weights with the selector and the volume, also when the Bragg
+    threshold changes (the descriptor changes and the graph is captured again) and
+    across offset and variance updates."""
+    g, ang, vol, counts = bf_problem(bragg_frac=0.05)
+    offs = np.full(len(ang), -np.log(2000.0))
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
+    kw = dict(BF, joint_estimation=1, noise_model=1, seed=9)
+    a, b = (capi.Job(gpu_ctx, geom, ang, counts.astype(np.float32), init_offset=offs,
+                     kernel_mode=m, **kw) for m in (0, 3))
+    for outer in range(3):
+        for j in (a, b):
+            j.set_bragg(5e10 if outer == 0 else 3.0)
+        for it in range(3):
+            a.sweep(outer, it)
+            b.sweep(outer, it)
+            np.testing.assert_array_equal(a.volume(), b.volume())
+            np.testing.assert_array_equal(a.error_sinogram(), b.error_sinogram())
+            np.testing.assert_array_equal(a.weights(), b.weights())
+        a.nuisance_update()
+        b.nuisance_update()
+    for u, v in zip(a.nuisance(), b.nuisance()):
+        np.testing.assert_array_equal(u, v)
+    a.close()
+    b.close()
+
+
 def test_bf_four_rows_per_slice_matches_oracle(gpu_ctx):
     """Bright field at four detector rows per slice (level 0 of three resolution levels,
     the reference's default for its bright-field reconstruction): setup against the
