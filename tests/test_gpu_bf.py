@@ -294,19 +294,19 @@ def test_bf_jobs_in_a_batched_sweep_equal_alone(gpu_ctx):
         j.close()
 
 
-@pytest.mark.parametrize("n_t", [1, 13, 100])
-def test_bf_line_lengths(gpu_ctx, n_t):
-    """Bright-field lines of 1, 13 and 100 slices (100 as whole lines: two per lane):
-    the setup against the oracle, the cost never rises over sweeps, e stays
-    y - A x - mu for the GPU's own volume, the selector follows its rule, and the same
-    job in a batched sweep ends every sweep bit for bit as alone."""
+@pytest.mark.parametrize("n_t,y_slabs", [(1, 1), (13, 1), (100, 1), (64, 2)])
+def test_bf_line_lengths(gpu_ctx, n_t, y_slabs):
+    """Bright-field lines of 1, 13 and 100 slices (100 as whole lines: two per lane) and
+    64 slices in two y-slab units: the setup against the oracle, the cost never rises
+    over sweeps, e stays y - A x - mu for the GPU's own volume, the selector follows its
+    rule, and the same job in a batched sweep ends every sweep bit for bit as alone."""
     g, ang, vol, counts = bf_problem(n_r=24, n_t=n_t, n_views=21, seed=13, bragg_frac=0.05)
     assert g.N_y == n_t
     offs = np.full(len(ang), -np.log(2000.0))
     eng = O.Engine(g, ang, counts, O.params(**BF), offsets=offs)
     geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
     jobs = [capi.Job(gpu_ctx, geom, ang, counts.astype(np.float32), init_offset=offs,
-                     y_slabs=1, seed=5, kernel_mode=m, **BF) for m in (0, 1)]
+                     y_slabs=y_slabs, seed=5, kernel_mode=m, **BF) for m in (0, 1)]
     job, bat = jobs
     ref_e = eng.e()
     assert np.max(np.abs(job.error_sinogram() - ref_e)) <= 1e-5 * np.max(np.abs(ref_e))
