@@ -164,3 +164,38 @@ def test_adapter_reads_bright_field_tilt_series(gpu_ctx, tmp_path):
                            timeout=120)
         assert r.returncode != 0, bad
         assert "error -1" in open(os.path.join(o, "run.txt")).read()
+
+
+def test_scaled_job_graph_replay_and_y_slabs(gpu_ctx):
+    """The scaled line update in its other single-job forms: sweeps replayed from a HIP
+    graph (kernel_mode 3) end bit for bit as the per-step launches, across offset steps
+    (the descriptor changes: the graph is captured again); and y-slab units (two per
+    line) lower the cost on every sweep and keep e = y - s A x - mu."""
+    g, ang, y, bf, s = _case(seed=14)
+    kw = dict(bf_sinogram=bf, default_recon=0.2, joint_estimation=1, seed=4)
+    a = _job(gpu_ctx, g, ang, y, kernel_mode=0, **kw)
+    b = _job(gpu_ctx, g, ang, y, kernel_mode=3, **kw)
+    for outer in range(3):
+        for it in range(3):
+            a.sweep(outer, it)
+            b.sweep(outer, it)
+            np.testing.assert_array_equal(a.volume(), b.volume())
+            np.testing.assert_array_equal(a.error_sinogram(), b.error_sinogram())
+        a.nuisance_update()
+        b.nuisance_update()
+    for u, v in zip(a.nuisance(), b.nuisance()):
+        np.testing.assert_array_equal(u, v)
+    a.close()
+    b.close()
+    job = _job(gpu_ctx, g, ang, y, bf_sinogram=bf, default_recon=0.2, y_slabs=2)
+    costs = [job.cost()]
+    for it in range(8):
+        job.sweep(0, it)
+        costs.append(job.cost())
+    assert all(d <= c * (1 + 1e-6) for c, d in zip(costs, costs[1:])), costs
+    xg = job.volume()
+    e_ref = y.reshape(s.shape) - s * job.forward_project(xg).reshape(s.shape)
+    assert np.max(np.abs(job.error_sinogram().reshape(s.shape) - e_ref)) <= 2e-4 * np.abs(y).max()
+    units = np.concatenate(job.schedule())
+    assert units[:, 2].max() < g.N_y  # units are y-ranges, not whole lines
+    job.close()
