@@ -96,6 +96,42 @@ def test_batch_line_lengths_same_alone_and_in_mix(gpu_ctx):
         m.close()
 
 
+def test_two_rows_per_slice_batched_and_served_equal_alone(gpu_ctx):
+    """Voxels two detector pitches wide (a coarse level: two detector rows per slice, the
+    batched kernels' NV = 2 forms) beside a job of one row per slice: each job alone on
+    its own step kernel, in one batched sweep and in the request server end bit for bit
+    alike."""
+    specs = []
+    for i, (n_t, ratio) in enumerate([(16, 2.0), (12, 2.0), (8, 1.0)]):
+        g, ang, vol, y = small_problem(n_r=32, n_t=n_t, n_views=31, thickness=8, seed=90 + i)
+        geom = capi.geometry_init(32, n_t, ang, g.LengthZ / 2, delta_xz_in=ratio,
+                                  interpolate_factor=ratio)
+        specs.append((geom, ang, y, dict(seed=30 + i, stop_threshold=0.0, num_inner=4)))
+    assert specs[0][0].N_y * 2 == 16
+    alone = [capi.Job(gpu_ctx, *sp[:3], kernel_mode=0, **sp[3]) for sp in specs]
+    mixed = [capi.Job(gpu_ctx, *sp[:3], kernel_mode=1, **sp[3]) for sp in specs]
+    for it in range(4):
+        for j in alone:
+            j.sweep(0, it)
+        capi.batch_sweep(gpu_ctx, mixed, outer=[0] * 3, inner=[it] * 3)
+    for a, m in zip(alone, mixed):
+        np.testing.assert_array_equal(a.volume(), m.volume())
+        np.testing.assert_array_equal(a.error_sinogram(), m.error_sinogram())
+    for j in mixed:
+        j.close()
+    srv = capi.Server(gpu_ctx, max_jobs=4)
+    tickets = [srv.submit(*sp[:3], **sp[3]) for sp in specs]
+    for t, a in zip(tickets, alone):
+        rep = srv.wait(t, timeout=120.0)
+        assert rep.state == capi.JOB_DONE and rep.sweeps == 4, (rep.state, rep.error, rep.sweeps)
+        np.testing.assert_array_equal(srv.job(t).volume(), a.volume())
+        np.testing.assert_array_equal(srv.job(t).error_sinogram(), a.error_sinogram())
+        srv.release(t)
+    srv.close()
+    for j in alone:
+        j.close()
+
+
 def test_autotune_keeps_results(gpu_ctx):
     """The autotuner times both forms of the batched line kernels and keeps the faster
     per shape and launch-size bucket; since the forms compute identical values, jobs
