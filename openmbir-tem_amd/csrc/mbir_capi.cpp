@@ -307,6 +307,27 @@ int setup_job(mbir_job* j)
   int S = cmin > 1e-6 ? (int)std::ceil(need / cmin - 1e-9) : g.N_x;
   S = std::max(2, std::min(S, g.N_x));
   j->spacing = S;
+  if (P.bf_nhicd) {
+    // lines d rows apart and dx >= 0 columns apart share no bin when
+    // |dx cos th - d sin th| >= need for every view, i.e. dx lies above every view's
+    // interval d tan th +- need / |cos th|; reach[d] is the least integer above them all
+    // (at least 2: prior neighbours).  A maximum of terms d t_k + c_k with c_k >= 0 is
+    // subadditive in d, so lines at increasing x, each at least reach[d] after the
+    // previous one, are pairwise free of conflicts.
+    const int Nz = g.N_z;
+    j->nh_reach.assign((size_t)2 * Nz - 1, g.N_x);
+    for (int d = -(Nz - 1); d <= Nz - 1; ++d) {
+      double hi = 0.0;
+      bool ok = true;
+      for (int i = 0; i < NT && ok; ++i) {
+        const double c = std::fabs(cosv[i]);
+        if (c < 1e-6) ok = false;
+        else hi = std::max(hi, d * sinv[i] / cosv[i] + need / c);
+      }
+      if (ok) j->nh_reach[(size_t)(d + Nz - 1)] =
+          std::max(2, (int)std::min<double>(g.N_x, std::ceil(hi + 1e-9)));
+    }
+  }
   // y-slabs: units of a step lie in different slabs, the row classes of neighbouring slabs
   // zshift >= 2 planes apart (and N_z - zshift >= 2 across the wrap), so they are never
   // prior neighbours; slices of different slabs touch disjoint detector rows
@@ -815,14 +836,13 @@ double nhicd_threshold(std::vector<double> v)
   return cand[k - below];
 }
 
-// Conflict-free steps of a list of lines (an NHICD call: a twentieth of the lines or
-// fewer).  Two lines of one row conflict exactly when they lie less than S apart (the
-// spacing of a sweep's classes (z, x mod S), which also keeps prior neighbours apart),
-// so a row's lines, in increasing x, go to the first step of that row whose last line
-// lies at least S to the left: the fewest steps the row needs (interval colouring),
-// where the classes of a full sweep would take up to S mostly nearly empty ones.  Steps
-// in a seeded random order; lines with empty columns dropped.
-void build_list_schedule(mbir_job* j, const std::vector<int>& lines)
+// Conflict-free steps of an NHICD list, row by row: two lines of one row conflict
+// exactly when they lie less than S apart (the spacing of a sweep's classes (z, x mod S),
+// which also keeps prior neighbours apart), so a row's lines, in increasing x, go to the
+// first step of that row whose last line lies at least S to the left: the fewest steps
+// the row needs (interval colouring), where a sweep's classes would take up to S mostly
+// nearly empty ones.  Lines with empty columns dropped.
+std::vector<std::vector<int>> pack_rows(const mbir_job* j, const std::vector<int>& lines)
 {
   const int Nx = j->g.N_x, S = j->spacing;
   std::vector<int> sorted;
@@ -850,6 +870,44 @@ void build_list_schedule(mbir_job* j, const std::vector<int>& lines)
     steps[row0 + k].push_back(l);
     last[k] = x;
   }
+  return steps;
+}
+
+// Conflict-free steps of a list spread over many rows (a homogeneous part: a random
+// twentieth of the lines): chains in increasing x, each line at least reach[d] after the
+// last line of its chain, d rows away (then at least as far from every earlier one), the
+// first chain that takes it.  Lines with empty columns dropped.
+std::vector<std::vector<int>> pack_chains(const mbir_job* j, const std::vector<int>& lines)
+{
+  const int Nx = j->g.N_x, off = j->g.N_z - 1;
+  std::vector<int> sorted;
+  sorted.reserve(lines.size());
+  for (int l : lines)
+    if (j->nnz_h[l] > 0) sorted.push_back(l);
+  std::sort(sorted.begin(), sorted.end(), [Nx](int a, int b) {
+    return a % Nx != b % Nx ? a % Nx < b % Nx : a < b;  // by x, then row
+  });
+  std::vector<std::vector<int>> steps;
+  std::vector<int> tx, tz;  // the last line of each chain
+  for (int l : sorted) {
+    const int z = l / Nx, x = l % Nx;
+    size_t k = 0;
+    while (k < tx.size() && x - tx[k] < j->nh_reach[(size_t)(z - tz[k] + off)]) ++k;
+    if (k == tx.size()) {
+      steps.emplace_back();
+      tx.push_back(x);
+      tz.push_back(z);
+    }
+    steps[k].push_back(l);
+    tx[k] = x;
+    tz[k] = z;
+  }
+  return steps;
+}
+
+// The steps in a seeded random order as the sweep's schedule.
+void schedule_steps(mbir_job* j, const std::vector<std::vector<int>>& steps)
+{
   std::vector<int> order(steps.size());
   std::iota(order.begin(), order.end(), 0);
   std::mt19937 rng(j->prob.seed + 7919u * (uint32_t)j->sweep_counter);
@@ -862,11 +920,11 @@ void build_list_schedule(mbir_job* j, const std::vector<int>& lines)
   }
 }
 
-// One NHICD call: the lines of `lines` in conflict-free steps, per-step launches.
-int run_list_steps(mbir_job* j, const std::vector<int>& lines, int outer, int inner)
+// One NHICD call: the given conflict-free steps in a seeded order, per-step launches.
+int run_list_steps(mbir_job* j, const std::vector<std::vector<int>>& steps, int outer, int inner)
 {
   hipStream_t s = j->stream;
-  build_list_schedule(j, lines);
+  schedule_steps(j, steps);
   int rc = prepare_sweep_on(j, s, nullptr);
   if (rc < 0) return rc;
   const int nsteps = (int)j->step_ptr.size() - 1;
@@ -1454,15 +1512,20 @@ extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, 
         } else {
           if (j->cancel.load()) return MBIR_ERR_CANCELLED;
           std::vector<int> list;
+          const std::vector<std::vector<int>>* steps = nullptr;
+          std::vector<std::vector<int>> nh_steps;
           const size_t n = j->nh_all.size();
           if (eff % 2 == 0) {
             // homogeneous: the next twentieth of the random list, the last one longer
-            // (:681-712)
+            // (:681-712); a part's lines never change, so its steps are packed once
             part %= kNhParts;
             size_t ne = n / kNhParts;
             if (part == kNhParts - 1) ne += n - ne * kNhParts;
             const size_t start = (n / kNhParts) * (size_t)part;
             list.assign(j->nh_all.begin() + (long)start, j->nh_all.begin() + (long)(start + ne));
+            if (j->nh_parts.size() != (size_t)kNhParts) j->nh_parts.assign(kNhParts, {});
+            if (j->nh_parts[part].empty()) j->nh_parts[part] = pack_chains(j, list);
+            steps = &j->nh_parts[part];
             part++;
           } else {
             // non-homogeneous: lines whose filtered update magnitude exceeds the
@@ -1473,12 +1536,14 @@ extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, 
             const double thr = nhicd_threshold(filt);
             for (size_t l = 0; l < n; ++l)
               if (filt[l] > thr) list.push_back((int)l);
+            nh_steps = pack_rows(j, list);
+            steps = &nh_steps;
           }
           if (j->nh_log_on) {
             j->nh_log.insert(j->nh_log.end(), list.begin(), list.end());
             j->nh_log_ptr.push_back((int64_t)j->nh_log.size());
           }
-          rc = run_list_steps(j, list, outer, inner);
+          rc = run_list_steps(j, *steps, outer, inner);
           if (rc < 0) return rc;
           // stop test every 20 calls: ROI update magnitude against the ROI volume sum
           // of 20 calls before (Extra.cpp:1217-1268; BFReconstructionEngine.cpp:739-750)

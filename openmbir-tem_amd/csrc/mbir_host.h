@@ -107,6 +107,11 @@ struct mbir_job {
   std::vector<float> magmap_h;
   std::vector<int> nh_all;
   std::vector<uint8_t> mask_h;
+  // reach[d + N_z - 1]: the least x distance at which a line d rows away never shares a
+  // detector bin with this one (every view) nor is a prior neighbour; subadditive in d
+  std::vector<int> nh_reach;
+  // the conflict-free steps of each homogeneous part (fixed lists: packed once)
+  std::vector<std::vector<std::vector<int>>> nh_parts;
   int nh_log_on = 0;                 // record the lines of every NHICD call
   std::vector<int> nh_log;           // lines of all calls of the last run
   std::vector<int64_t> nh_log_ptr;   // call c: nh_log[nh_log_ptr[c] .. nh_log_ptr[c + 1])
