@@ -836,47 +836,12 @@ double nhicd_threshold(std::vector<double> v)
   return cand[k - below];
 }
 
-// Conflict-free steps of an NHICD list, row by row: two lines of one row conflict
-// exactly when they lie less than S apart (the spacing of a sweep's classes (z, x mod S),
-// which also keeps prior neighbours apart), so a row's lines, in increasing x, go to the
-// first step of that row whose last line lies at least S to the left: the fewest steps
-// the row needs (interval colouring), where a sweep's classes would take up to S mostly
-// nearly empty ones.  Lines with empty columns dropped.
-std::vector<std::vector<int>> pack_rows(const mbir_job* j, const std::vector<int>& lines)
-{
-  const int Nx = j->g.N_x, S = j->spacing;
-  std::vector<int> sorted;
-  sorted.reserve(lines.size());
-  for (int l : lines)
-    if (j->nnz_h[l] > 0) sorted.push_back(l);
-  std::sort(sorted.begin(), sorted.end());  // by row, then x
-  std::vector<std::vector<int>> steps;
-  std::vector<int> last;  // x of the last line of each step of the current row
-  size_t row0 = 0;        // the current row's first step
-  int row = -1;
-  for (int l : sorted) {
-    const int z = l / Nx, x = l % Nx;
-    if (z != row) {
-      row = z;
-      row0 = steps.size();
-      last.clear();
-    }
-    size_t k = 0;
-    while (k < last.size() && last[k] > x - S) ++k;
-    if (k == last.size()) {
-      steps.emplace_back();
-      last.push_back(x);
-    }
-    steps[row0 + k].push_back(l);
-    last[k] = x;
-  }
-  return steps;
-}
-
-// Conflict-free steps of a list spread over many rows (a homogeneous part: a random
-// twentieth of the lines): chains in increasing x, each line at least reach[d] after the
-// last line of its chain, d rows away (then at least as far from every earlier one), the
-// first chain that takes it.  Lines with empty columns dropped.
+// Conflict-free steps of an NHICD list: chains in increasing x, each line at least
+// reach[d] after the last line of its chain, d rows away (then at least as far from every
+// earlier one), the first chain that takes it.  Lines of one row need at least S between
+// them, as in a sweep's classes (z, x mod S), but lines of other rows far enough along x
+// share the step too, so a call of a twentieth of the lines takes a few hundred steps
+// where the classes would take most of a sweep's.  Lines with empty columns dropped.
 std::vector<std::vector<int>> pack_chains(const mbir_job* j, const std::vector<int>& lines)
 {
   const int Nx = j->g.N_x, off = j->g.N_z - 1;
@@ -1536,7 +1501,7 @@ extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, 
             const double thr = nhicd_threshold(filt);
             for (size_t l = 0; l < n; ++l)
               if (filt[l] > thr) list.push_back((int)l);
-            nh_steps = pack_rows(j, list);
+            nh_steps = pack_chains(j, list);
             steps = &nh_steps;
           }
           if (j->nh_log_on) {
