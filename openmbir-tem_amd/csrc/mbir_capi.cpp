@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <thread>
 #include <mutex>
 #include <numeric>
@@ -841,32 +842,32 @@ double nhicd_threshold(std::vector<double> v)
 // earlier one), the first chain that takes it.  Lines of one row need at least S between
 // them, as in a sweep's classes (z, x mod S), but lines of other rows far enough along x
 // share the step too, so a call of a twentieth of the lines takes a few hundred steps
-// where the classes would take most of a sweep's.  Lines with empty columns dropped.
+// where the classes would take most of a sweep's.  The lines are visited by x, then row,
+// through a map of the list (each line at most once in a list); lines with empty columns
+// are dropped.
 std::vector<std::vector<int>> pack_chains(const mbir_job* j, const std::vector<int>& lines)
 {
-  const int Nx = j->g.N_x, off = j->g.N_z - 1;
-  std::vector<int> sorted;
-  sorted.reserve(lines.size());
+  const int Nx = j->g.N_x, Nz = j->g.N_z, off = Nz - 1;
+  std::vector<uint8_t> in((size_t)Nx * Nz, 0);
   for (int l : lines)
-    if (j->nnz_h[l] > 0) sorted.push_back(l);
-  std::sort(sorted.begin(), sorted.end(), [Nx](int a, int b) {
-    return a % Nx != b % Nx ? a % Nx < b % Nx : a < b;  // by x, then row
-  });
+    if (j->nnz_h[l] > 0) in[(size_t)l] = 1;
   std::vector<std::vector<int>> steps;
   std::vector<int> tx, tz;  // the last line of each chain
-  for (int l : sorted) {
-    const int z = l / Nx, x = l % Nx;
-    size_t k = 0;
-    while (k < tx.size() && x - tx[k] < j->nh_reach[(size_t)(z - tz[k] + off)]) ++k;
-    if (k == tx.size()) {
-      steps.emplace_back();
-      tx.push_back(x);
-      tz.push_back(z);
+  for (int x = 0; x < Nx; ++x)
+    for (int z = 0; z < Nz; ++z) {
+      const int l = z * Nx + x;
+      if (!in[(size_t)l]) continue;
+      size_t k = 0;
+      while (k < tx.size() && x - tx[k] < j->nh_reach[(size_t)(z - tz[k] + off)]) ++k;
+      if (k == tx.size()) {
+        steps.emplace_back();
+        tx.push_back(x);
+        tz.push_back(z);
+      }
+      steps[k].push_back(l);
+      tx[k] = x;
+      tz[k] = z;
     }
-    steps[k].push_back(l);
-    tx[k] = x;
-    tz[k] = z;
-  }
   return steps;
 }
 
@@ -886,7 +887,10 @@ void schedule_steps(mbir_job* j, const std::vector<std::vector<int>>& steps)
 }
 
 // One NHICD call: the given conflict-free steps in a seeded order, per-step launches.
-int run_list_steps(mbir_job* j, const std::vector<std::vector<int>>& steps, int outer, int inner)
+// while_running (may be empty) is host work done after the launches are queued, while
+// the device runs them; it must not touch the call's steps.
+int run_list_steps(mbir_job* j, const std::vector<std::vector<int>>& steps, int outer, int inner,
+                   const std::function<void()>& while_running)
 {
   hipStream_t s = j->stream;
   schedule_steps(j, steps);
@@ -899,6 +903,7 @@ int run_list_steps(mbir_job* j, const std::vector<std::vector<int>>& steps, int 
                             j->sched_h.data() + b));
   }
   HIP_TRY(launch_sweep_reduce(j->sched.p, j->ustat.p, (int)j->sched_h.size(), j->acc.p, j->red.p, s));
+  if (while_running) while_running();
   return finish_sweep(j, 0.0, nullptr);
 }
 
@@ -1508,7 +1513,19 @@ extern "C" int32_t mbir_job_run(mbir_job* j, mbir_run_stats* rs, double* costs, 
             j->nh_log.insert(j->nh_log.end(), list.begin(), list.end());
             j->nh_log_ptr.push_back((int64_t)j->nh_log.size());
           }
-          rc = run_list_steps(j, *steps, outer, inner);
+          // the next homogeneous part, when a later call may need it and it is not packed
+          // yet, is packed while the device runs this call
+          const bool later = inner + 1 < num_iter || outer + 1 < j->prob.num_outer;
+          const int next = part % kNhParts;
+          rc = run_list_steps(j, *steps, outer, inner, [&]() {
+            if (!later || j->nh_parts.size() != (size_t)kNhParts || !j->nh_parts[next].empty())
+              return;
+            const size_t nn = n / kNhParts;
+            const size_t s0 = nn * (size_t)next, ne = next == kNhParts - 1 ? n - s0 : nn;
+            const std::vector<int> nl(j->nh_all.begin() + (long)s0,
+                                      j->nh_all.begin() + (long)(s0 + ne));
+            j->nh_parts[next] = pack_chains(j, nl);
+          });
           if (rc < 0) return rc;
           // stop test every 20 calls: ROI update magnitude against the ROI volume sum
           // of 20 calls before (Extra.cpp:1217-1268; BFReconstructionEngine.cpp:739-750)

@@ -445,10 +445,15 @@ class Job:
         return (tot.value, t) if terms else tot.value
 
     def run(self, cap=4096):
+        """Run the job; returns the run statistics and up to cap costs (one after the
+        setup, one after each sweep and nuisance step).  cap=0 records no costs, so the
+        run computes none (the reference computes them only in its COST_CALCULATE
+        debugging build, BrightField/BFReconstructionEngine.cpp:806-876)."""
         rs = RunStats()
-        costs = np.zeros(cap)
+        costs = np.zeros(max(cap, 0))
         n = C.c_int32()
-        _chk("mbir_job_run", lib().mbir_job_run(self.h, C.byref(rs), _dptr(costs), cap, C.byref(n)))
+        ptr = _dptr(costs) if cap > 0 else None
+        _chk("mbir_job_run", lib().mbir_job_run(self.h, C.byref(rs), ptr, max(cap, 0), C.byref(n)))
         return rs, costs[: n.value]
 
     def forward_project(self, volume=None):

@@ -223,6 +223,29 @@ def test_bf_nhicd_stop_rule_every_twenty_calls(gpu_ctx):
         assert rs.sweeps == calls, (stop, rs.sweeps)
 
 
+def test_bf_nhicd_run_without_costs_equals_run_with_costs(gpu_ctx):
+    """A run that records no costs (costs NULL, as the reference runs without its
+    COST_CALCULATE debugging build) takes the same calls and leaves the same volume,
+    offsets and variance, bit for bit, as one that records a cost after every call."""
+    g, ang, vol, counts = bf_problem()
+    offs = np.full(len(ang), -np.log(2000.0))
+    geom = capi.geometry_init(g.N_r, g.N_t, ang, g.LengthZ / 2)
+    c32 = counts.astype(np.float32)
+    out = []
+    for cap in (4096, 0):
+        job = capi.Job(gpu_ctx, geom, ang, c32, init_offset=offs, seed=3, num_outer=2,
+                       num_inner=25, stop_threshold=0.0, bf_nhicd=1, **BF)
+        rs, costs = job.run(cap=cap)
+        assert (len(costs) == 0) if cap == 0 else (len(costs) > rs.sweeps), (cap, len(costs))
+        out.append((rs.sweeps, job.volume(), job.nuisance()))
+        job.close()
+    (s0, v0, n0), (s1, v1, n1) = out
+    assert s0 == s1 == 26
+    assert np.array_equal(v0, v1)
+    for a, b in zip(n0, n1):
+        assert np.array_equal(a, b)
+
+
 def test_bf_jobs_in_the_server_equal_alone(gpu_ctx):
     """Bright-field jobs in the request server (their own launches, beside HAADF jobs)
     follow the same loop as mbir_job_run: threshold schedule, stop test before the
