@@ -336,9 +336,9 @@ void HAADF_ReconstructionEngine::execute()
     if (m_Cancel.load()) mbir_job_cancel(job);
   }
   mbir_run_stats rs{};
-  std::vector<double> costs(4096);
+  std::vector<double> costs(m_RecordCosts ? 4096 : 0);
   int32_t nc = 0;
-  rc = mbir_job_run(job, &rs, costs.data(), (int32_t)costs.size(), &nc);
+  rc = mbir_job_run(job, &rs, m_RecordCosts ? costs.data() : nullptr, (int32_t)costs.size(), &nc);
   if (rc == 0) {
     m_Sweeps = rs.sweeps;
     m_Costs.assign(costs.begin(), costs.begin() + nc);

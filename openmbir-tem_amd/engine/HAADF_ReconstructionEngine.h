@@ -142,6 +142,11 @@ class HAADF_ReconstructionEngine {
   bool getCancel() const { return m_Cancel.load(); }
 
   int getSweeps() const { return m_Sweeps; }
+  // The cost after the setup, every sweep and every nuisance step, recorded only when
+  // asked for (off by default): the reference computes these only in its COST_CALCULATE
+  // debugging build (HAADF/HAADF_ReconstructionEngine.cpp:448, 700, 780), and each costs
+  // a pass over the sinogram and the volume.
+  void setCostRecording(bool on) { m_RecordCosts = on; }
   const std::vector<Real_t>& getCosts() const { return m_Costs; }
 
   // Device-resident multi-resolution hand-over (no counterpart in the reference,
@@ -198,6 +203,7 @@ class HAADF_ReconstructionEngine {
   mbir_job* m_KeptJob = nullptr;
   mbir_ctx* m_KeptCtx = nullptr;
   int m_Sweeps = 0;
+  bool m_RecordCosts = false;
   std::vector<Real_t> m_Costs;
 };
 

@@ -94,6 +94,7 @@ static int run_mrc(int argc, char** argv)
     eng->setBFTomoInputs(bf);
     eng->setBFSinogram(std::make_shared<Sinogram>());
   }
+  eng->setCostRecording(true);  // the costs go to the run's text file
   eng->execute();
   std::ofstream txt(dir + "/run.txt");
   txt << std::setprecision(17);
@@ -194,6 +195,7 @@ int main(int argc, char** argv)
   eng->setGeometry(geom);
   eng->setAdvParams(adv);
   eng->setForwardModel(fm);
+  eng->setCostRecording(true);  // the costs go to the run's text file
   eng->execute();
   const std::string pre = argv[14];
   std::ofstream txt(pre + ".txt");
